@@ -1,0 +1,77 @@
+// Boys functions F_m(T) = int_0^1 t^{2m} exp(-T t^2) dt on the device.
+//
+// Scheme (standard; Helgaker, Jorgensen & Olsen, "Molecular Electronic-Structure
+// Theory", sec. 9.8.2; Gill, Johnson & Pople, IJQC 40, 745 (1991)):
+//  * T < FE_BOYS_TSW: nearest grid point T_r = r * dT (dT = 1/8), Taylor series
+//    of the top order F_M(T) = sum_{k<9} (-1)^k F_{M+k}(T_r) (T-T_r)^k / k!
+//    (|T-T_r| <= 1/16, truncation < 4e-17 relative), exp(-T) = exp(-T_r) *
+//    exp(-(T-T_r)) with a degree-9 polynomial, then the stable downward
+//    recursion F_{m-1} = (2T F_m + exp(-T)) / (2m-1).
+//  * T >= FE_BOYS_TSW = 60: asymptotic F_0 = sqrt(pi/T)/2 and the upward
+//    recursion F_m = F_{m-1} (2m-1)/(2T); the neglected incomplete-gamma tail is
+//    below 1.3e-17 relative for m <= 8 at T = 60.
+// The reference evaluator switches at T = M - 0.5 between an 80-term Taylor
+// series at T = 0 and erf + upward recursion (fastesp/native/esp_gpu.cu:36-66);
+// that series cancels just below the switch (2.4e-13 relative at M = 7), which
+// the tabulated expansion avoids.
+//
+// The grid values F_j(T_r) are computed on the host in 80-bit long double from
+// the all-positive series F_j(T) = exp(-T) sum_i (2T)^i / prod_{k<=i} (2j+2k+1).
+#pragma once
+
+#define FE_BOYS_NR 481        // grid rows, T_r = r/8, r = 0..480 (T < 60)
+#define FE_BOYS_COLS 10       // 9 Taylor coefficients + exp(-T_r)
+#define FE_BOYS_NK 9
+#define FE_BOYS_MMAX 8        // highest top order with a table
+#define FE_BOYS_TSW 60.0
+#define FE_BOYS_DT 0.125
+#define FE_BOYS_INVDT 8.0
+
+// exp(x) for |x| <= 1/16, degree-9 Taylor polynomial (error < 3e-19 relative)
+__device__ __forceinline__ double fe_exp_small(const double x) {
+  double e = 2.7557319223985893e-06;         // 1/9!
+  e = fma(e, x, 2.4801587301587302e-05);     // 1/8!
+  e = fma(e, x, 1.9841269841269841e-04);     // 1/7!
+  e = fma(e, x, 1.3888888888888889e-03);     // 1/6!
+  e = fma(e, x, 8.3333333333333333e-03);     // 1/5!
+  e = fma(e, x, 4.1666666666666667e-02);     // 1/4!
+  e = fma(e, x, 1.6666666666666667e-01);     // 1/3!
+  e = fma(e, x, 0.5);
+  e = fma(e, x, 1.0);
+  return fma(e, x, 1.0);
+}
+
+// F[0..M] at T; `tab` is the (FE_BOYS_NR x FE_BOYS_COLS) table of top order M
+// (normally staged in LDS).
+template <int M>
+__device__ __forceinline__ void fe_boys(const double T, double* F, const double* tab) {
+  if (T < FE_BOYS_TSW) {
+    const int r = (int)fma(T, FE_BOYS_INVDT, 0.5);
+    const double dt = fma(-FE_BOYS_DT, (double)r, T);
+    const double* c = tab + r * FE_BOYS_COLS;
+    double f = c[8];
+    f = fma(f, dt, c[7]);
+    f = fma(f, dt, c[6]);
+    f = fma(f, dt, c[5]);
+    f = fma(f, dt, c[4]);
+    f = fma(f, dt, c[3]);
+    f = fma(f, dt, c[2]);
+    f = fma(f, dt, c[1]);
+    f = fma(f, dt, c[0]);
+    F[M] = f;
+    if constexpr (M > 0) {
+      const double e = c[9] * fe_exp_small(-dt);
+      const double t2 = T + T;
+#pragma unroll
+      for (int m = M; m > 0; --m) F[m - 1] = fma(t2, F[m], e) * (1.0 / (2 * m - 1));
+    }
+  } else {
+    const double r = rsqrt(T);
+    F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
+    if constexpr (M > 0) {
+      const double x = 0.5 / T;
+#pragma unroll
+      for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((2 * m - 1) * x);
+    }
+  }
+}

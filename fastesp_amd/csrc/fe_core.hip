@@ -1,0 +1,546 @@
+// libfastesp_hip: context, device pair tables and the ESP forward evaluation.
+//
+// Device table layout ("per-L records"): the pairs of each class
+// L = l_A + l_B (0..FE_LMAX) are stored contiguously, one 64-byte aligned
+// record of FeL<L>::STRIDE doubles per primitive pair:
+//
+//     [0] Px [1] Py [2] Pz [3] p [4] K [5] K*bound [6] reff [7] spare
+//     [8 .. 8+NT)  g_k in canonical key order, zero padded to a multiple of 8
+//
+// A wavefront evaluates 64 points (x points-per-lane) against the same pair,
+// so the whole record is wave-uniform and is fetched with scalar loads into
+// SGPRs; g_k then feeds v_fma_f64 directly as the scalar operand.  The
+// per-point state (R, Boys vector, monomials) lives in VGPRs, the Boys grid of
+// the kernel's top order is staged once per workgroup in LDS.
+//
+// Work decomposition: grid = (point blocks) x (pair chunks of one class).
+// Every (class, chunk) writes one row of a partial-sum buffer; a second
+// kernel adds the rows in a fixed order plus the nuclear term, so results are
+// bitwise reproducible run to run (the reference's two-kernel split,
+// fastesp/native/esp_gpu.cu:161-229, with the chunk count sized to the chip
+// instead of a fixed 4096).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fastesp_hip.h"
+#include "fe_boys.h"
+#include "fe_lclass.inc"
+#include "fe_internal.h"
+
+#define FE_VERSION 100
+
+namespace {
+thread_local std::string g_err;
+}
+
+int fe_set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" int fe_version(void) { return FE_VERSION; }
+extern "C" const char* fe_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Boys grid (host, long double)
+// ---------------------------------------------------------------------------
+static void boys_series(long double T, int jmax, long double* F) {
+  const long double e = expl(-T);
+  for (int j = 0; j <= jmax; ++j) {
+    long double term = 1.0L / (2 * j + 1), sum = term;
+    for (int i = 1; i < 4000; ++i) {
+      term *= (2.0L * T) / (2 * j + 2 * i + 1);
+      sum += term;
+      if (term < 1e-24L * sum) break;
+    }
+    F[j] = e * sum;
+  }
+}
+
+static std::vector<double> make_boys_tables() {
+  const int jmax = FE_BOYS_MMAX + FE_BOYS_NK - 1;
+  std::vector<double> tab((size_t)(FE_BOYS_MMAX + 1) * FE_BOYS_NR * FE_BOYS_COLS);
+  std::vector<long double> F(jmax + 1);
+  long double fact[FE_BOYS_NK];
+  fact[0] = 1.0L;
+  for (int k = 1; k < FE_BOYS_NK; ++k) fact[k] = fact[k - 1] * k;
+  for (int r = 0; r < FE_BOYS_NR; ++r) {
+    const long double T = (long double)r / 8.0L;
+    boys_series(T, jmax, F.data());
+    for (int M = 0; M <= FE_BOYS_MMAX; ++M) {
+      double* row = &tab[((size_t)M * FE_BOYS_NR + r) * FE_BOYS_COLS];
+      for (int k = 0; k < FE_BOYS_NK; ++k)
+        row[k] = (double)(((k & 1) ? -1.0L : 1.0L) * F[M + k] / fact[k]);
+      row[FE_BOYS_NK] = (double)expl(-T);
+    }
+  }
+  return tab;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes) {
+  if (c->ws_bytes[slot] >= bytes) return FE_OK;
+  if (c->ws[slot]) FE_HIPCHK(hipFree(c->ws[slot]));
+  c->ws[slot] = nullptr;
+  c->ws_bytes[slot] = 0;
+  size_t b = std::max<size_t>(bytes, 4096);
+  b = (b + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+  if (hipMalloc(&c->ws[slot], b) != hipSuccess) {
+    (void)hipGetLastError();
+    return fe_set_error(FE_ERR_ALLOC, "device allocation of %zu bytes failed", b);
+  }
+  c->ws_bytes[slot] = b;
+  return FE_OK;
+}
+
+extern "C" int fe_init(int device, fe_ctx** out) {
+  if (!out) return fe_set_error(FE_ERR_ARG, "fe_init: null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  FE_HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev)
+    return fe_set_error(FE_ERR_ARG, "fe_init: device %d out of range (%d devices)", device, ndev);
+  FE_HIPCHK(hipSetDevice(device));
+  fe_ctx* c = new fe_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return fe_set_error(FE_ERR_HIP, "hipGetDeviceProperties failed");
+  }
+  c->n_cu = prop.multiProcessorCount;
+  std::vector<double> tab = make_boys_tables();
+  if (hipMalloc(&c->d_boys, tab.size() * sizeof(double)) != hipSuccess) {
+    delete c;
+    return fe_set_error(FE_ERR_ALLOC, "fe_init: Boys table allocation failed");
+  }
+  if (hipMemcpy(c->d_boys, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(c->d_boys);
+    delete c;
+    return fe_set_error(FE_ERR_HIP, "fe_init: Boys table upload failed");
+  }
+  c->ppl = 1;
+  c->target_wg = 8 * c->n_cu;
+  *out = c;
+  return FE_OK;
+}
+
+extern "C" int fe_finalize(fe_ctx* c) {
+  if (!c) return fe_set_error(FE_ERR_ARG, "fe_finalize: null context");
+  FE_HIPCHK(hipSetDevice(c->device));
+  for (int s = 0; s < FE_WS_SLOTS; ++s)
+    if (c->ws[s]) (void)hipFree(c->ws[s]);
+  if (c->d_boys) (void)hipFree(c->d_boys);
+  delete c;
+  return FE_OK;
+}
+
+extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
+  if (!c) return fe_set_error(FE_ERR_ARG, "fe_set_tuning: null context");
+  switch (key) {
+    case 0:
+      if (value != 1 && value != 2) return fe_set_error(FE_ERR_ARG, "points per lane must be 1 or 2");
+      c->ppl = (int)value;
+      return FE_OK;
+    case 1:
+      if (value < 1 || value > (1 << 24)) return fe_set_error(FE_ERR_ARG, "bad target workgroups");
+      c->target_wg = (int)value;
+      return FE_OK;
+    case 2:
+      c->tile_screen = value != 0;
+      return FE_OK;
+    default:
+      return fe_set_error(FE_ERR_ARG, "unknown tuning key %d", key);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// table upload (pairs.bin fields -> per-L records)
+// ---------------------------------------------------------------------------
+static int class_of_nterm(int nt) {
+  for (int L = 0; L <= FE_LMAX; ++L)
+    if (fe_host::NTERM[L] == nt) return L;
+  return -1;
+}
+
+int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec) {
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    t->n_L[L] = rec[L].size() / fe_host::STRIDE[L];
+    if (t->n_L[L] == 0) continue;
+    const size_t b = rec[L].size() * sizeof(double);
+    if (hipMalloc(&t->d_rec[L], b) != hipSuccess) {
+      (void)hipGetLastError();
+      return fe_set_error(FE_ERR_ALLOC, "table: allocation of %zu bytes failed", b);
+    }
+    FE_HIPCHK(hipMemcpy(t->d_rec[L], rec[L].data(), b, hipMemcpyHostToDevice));
+    t->dev_bytes += b;
+  }
+  return FE_OK;
+}
+
+int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz) {
+  t->n_atoms = n_atoms;
+  if (n_atoms <= 0) return FE_OK;
+  std::vector<double> a((size_t)4 * n_atoms);
+  for (int i = 0; i < n_atoms; ++i) {
+    a[4 * i + 0] = xyz[3 * i + 0];
+    a[4 * i + 1] = xyz[3 * i + 1];
+    a[4 * i + 2] = xyz[3 * i + 2];
+    a[4 * i + 3] = Z[i];
+  }
+  if (hipMalloc(&t->d_atoms, a.size() * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fe_set_error(FE_ERR_ALLOC, "table: atom allocation failed");
+  }
+  FE_HIPCHK(hipMemcpy(t->d_atoms, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice));
+  t->dev_bytes += a.size() * sizeof(double);
+  return FE_OK;
+}
+
+extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const double* pp_exp,
+                                   const double* pp_cx, const double* pp_cy, const double* pp_cz,
+                                   const double* pp_K, const int32_t* pp_nt,
+                                   const int32_t* pp_off, const int32_t* pp_nmono,
+                                   const int32_t* pp_monooff, const int32_t* mo_parent,
+                                   const int32_t* mo_axis, const int32_t* tm_aidx,
+                                   const int32_t* tm_m, const double* tm_g, int n_atoms,
+                                   const double* Z, const double* xyz, fe_table** out) {
+  if (!c || !out) return fe_set_error(FE_ERR_ARG, "fe_table_from_pairs: null argument");
+  *out = nullptr;
+  if (n_pairs < 0 || n_terms < 0 || n_atoms < 0)
+    return fe_set_error(FE_ERR_ARG, "fe_table_from_pairs: negative size");
+  if (n_pairs > 0 && (!pp_exp || !pp_cx || !pp_cy || !pp_cz || !pp_K || !pp_nt || !pp_off ||
+                      !pp_nmono || !pp_monooff || !mo_parent || !mo_axis || !tm_aidx || !tm_m ||
+                      !tm_g))
+    return fe_set_error(FE_ERR_ARG, "fe_table_from_pairs: null array");
+  if (n_atoms > 0 && (!Z || !xyz)) return fe_set_error(FE_ERR_ARG, "fe_table_from_pairs: null atoms");
+  std::vector<double> rec[FE_LMAX + 1];
+  std::vector<int32_t> orig[FE_LMAX + 1];
+  for (int i = 0; i < n_pairs; ++i) {
+    const int nt = pp_nt[i];
+    const int L = class_of_nterm(nt);
+    if (L < 0)
+      return fe_set_error(FE_ERR_UNSUPPORTED,
+                          "pair %d: %d terms matches no class L <= %d", i, nt, FE_LMAX);
+    const int off = pp_off[i], mo = pp_monooff[i];
+    if (off < 0 || (int64_t)off + nt > n_terms)
+      return fe_set_error(FE_ERR_FORMAT, "pair %d: term range out of bounds", i);
+    if (pp_nmono[i] != fe_host::NMONO[L] || mo < 0)
+      return fe_set_error(FE_ERR_FORMAT, "pair %d: %d monomials, class %d needs %d", i,
+                          pp_nmono[i], L, fe_host::NMONO[L]);
+    for (int j = 0; j < nt; ++j) {
+      const int* k = fe_host::KEYS[fe_host::KEY_OFF[L] + j];
+      if (tm_m[off + j] != k[3] || tm_aidx[off + j] != k[4])
+        return fe_set_error(FE_ERR_FORMAT, "pair %d term %d: (aidx,m)=(%d,%d), canonical (%d,%d)",
+                            i, j, tm_aidx[off + j], tm_m[off + j], k[4], k[3]);
+    }
+    for (int j = 1; j < fe_host::NMONO[L]; ++j) {
+      const int* m = fe_host::MONOS[fe_host::MONO_OFF[L] + j];
+      if (mo_parent[mo + j] != m[3] || mo_axis[mo + j] != m[4])
+        return fe_set_error(FE_ERR_FORMAT, "pair %d monomial %d: non-canonical schedule", i, j);
+    }
+    const int S = fe_host::STRIDE[L];
+    const size_t base = rec[L].size();
+    rec[L].resize(base + S, 0.0);
+    double* r = &rec[L][base];
+    r[0] = pp_cx[i];
+    r[1] = pp_cy[i];
+    r[2] = pp_cz[i];
+    r[3] = pp_exp[i];
+    r[4] = pp_K[i];
+    r[5] = INFINITY;  // K*bound unknown for uploaded tables
+    r[6] = 0.0;
+    for (int j = 0; j < nt; ++j) r[8 + j] = tm_g[off + j];
+    orig[L].push_back(i);
+  }
+  FE_HIPCHK(hipSetDevice(c->device));
+  fe_table* t = new fe_table();
+  t->ctx = c;
+  t->n_pairs = n_pairs;
+  t->n_terms = n_terms;
+  int rc = fe_table_alloc_classes(t, rec);
+  if (rc == FE_OK) rc = fe_table_set_atoms(t, n_atoms, Z, xyz);
+  if (rc != FE_OK) {
+    fe_table_free(t);
+    return rc;
+  }
+  for (int L = 0; L <= FE_LMAX; ++L) t->orig[L] = std::move(orig[L]);
+  *out = t;
+  return FE_OK;
+}
+
+extern "C" int fe_table_free(fe_table* t) {
+  if (!t) return fe_set_error(FE_ERR_ARG, "fe_table_free: null table");
+  (void)hipSetDevice(t->ctx->device);
+  for (int L = 0; L <= FE_LMAX; ++L)
+    if (t->d_rec[L]) (void)hipFree(t->d_rec[L]);
+  if (t->d_atoms) (void)hipFree(t->d_atoms);
+  delete t;
+  return FE_OK;
+}
+
+extern "C" int fe_table_info(const fe_table* t, int64_t* info, int n_info) {
+  if (!t || !info) return fe_set_error(FE_ERR_ARG, "fe_table_info: null argument");
+  int64_t v[16] = {0};
+  v[0] = t->n_pairs;
+  v[1] = t->n_terms;
+  v[2] = t->n_atoms;
+  for (int L = 0; L <= FE_LMAX; ++L) v[3 + L] = t->n_L[L];
+  v[10] = (int64_t)t->dev_bytes;
+  v[11] = t->n_candidates;
+  for (int i = 0; i < n_info && i < 16; ++i) info[i] = v[i];
+  return FE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int L, int PPL>
+__global__ __launch_bounds__(FE_WG) void k_esp_fwd(const double* __restrict__ rec, const int n_pairs,
+                                                   const int ppc, const double* __restrict__ pts,
+                                                   const int n_pts,
+                                                   const double* __restrict__ boys_tab,
+                                                   double* __restrict__ partial, const int row0,
+                                                   const int ld) {
+  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
+  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
+  __syncthreads();
+
+  constexpr int S = FeL<L>::STRIDE;
+  const int chunk = blockIdx.y;
+  const int p0 = chunk * ppc;
+  const int p1 = min(n_pairs, p0 + ppc);
+  double cx[PPL], cy[PPL], cz[PPL], acc[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
+    const int ii = min(i, n_pts - 1);
+    cx[q] = pts[3 * ii + 0];
+    cy[q] = pts[3 * ii + 1];
+    cz[q] = pts[3 * ii + 2];
+    acc[q] = 0.0;
+  }
+  for (int ip = p0; ip < p1; ++ip) {
+    const double* __restrict__ r = rec + (size_t)ip * S;
+    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      const double Rx = cx[q] - Px, Ry = cy[q] - Py, Rz = cz[q] - Pz;
+      const double T = p * fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
+      double F[L + 1];
+      fe_boys<L>(T, F, s_tab);
+      const double s = fe_contract<L>(Rx, Ry, Rz, F, r + 8);
+      acc[q] = fma(K, s, acc[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
+    if (i < n_pts) partial[(size_t)(row0 + chunk) * ld + i] = acc[q];
+  }
+}
+
+// V = [sum_A Z_A/|C-R_A|] - [sum of partial rows], rows added in a fixed order.
+__global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ partial,
+                                                    const int n_rows, const int ld,
+                                                    const double* __restrict__ pts, const int n_pts,
+                                                    const double* __restrict__ atoms,
+                                                    const int n_atoms, const int parts,
+                                                    double* __restrict__ V) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_pts) return;
+  double e = 0.0;
+  for (int r = 0; r < n_rows; ++r) e += partial[(size_t)r * ld + i];
+  double nuc = 0.0;
+  if (parts & FE_PART_NUCLEAR) {
+    const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    for (int a = 0; a < n_atoms; ++a) {
+      const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
+      nuc += atoms[4 * a + 3] / sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
+    }
+  }
+  V[i] = (parts & FE_PART_ELECTRON) ? nuc - e : nuc;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_boys_eval(const double* __restrict__ T, const int64_t n,
+                                                   const double* __restrict__ tab,
+                                                   double* __restrict__ F) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double f[M + 1];
+  fe_boys<M>(T[i], f, tab);
+#pragma unroll
+  for (int m = 0; m <= M; ++m) F[i * (M + 1) + m] = f[m];
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+template <int L, int PPL>
+static void launch_fwd_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                         double* partial, hipStream_t st) {
+  if (pl.chunks[L] == 0) return;
+  dim3 grid(pl.n_pb, pl.chunks[L]);
+  const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
+  hipLaunchKernelGGL((k_esp_fwd<L, PPL>), grid, dim3(FE_WG), 0, st, t->d_rec[L], (int)t->n_L[L],
+                     pl.ppc[L], pts, n, tab, partial, pl.row0[L], pl.ld);
+}
+
+template <int PPL>
+static void launch_fwd_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                           double* partial, hipStream_t st) {
+  launch_fwd_L<0, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<1, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<2, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<3, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<4, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<5, PPL>(t, pl, pts, n, partial, st);
+  launch_fwd_L<6, PPL>(t, pl, pts, n, partial, st);
+}
+
+FwdPlan fe_plan(const fe_table* t, int64_t n, int ppl, int target_wg) {
+  FwdPlan pl{};
+  const int per_block = FE_WG * ppl;
+  pl.n_pb = (int)((n + per_block - 1) / per_block);
+  pl.ld = pl.n_pb * per_block;
+  int rows = 0;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    const int64_t nL = t->n_L[L];
+    if (nL == 0) {
+      pl.chunks[L] = 0;
+      continue;
+    }
+    int64_t c = std::max<int64_t>(1, target_wg / std::max(pl.n_pb, 1));
+    c = std::min<int64_t>(c, nL);
+    c = std::min<int64_t>(c, 65535);
+    const int64_t ppc = (nL + c - 1) / c;
+    c = (nL + ppc - 1) / ppc;
+    pl.ppc[L] = (int)ppc;
+    pl.chunks[L] = (int)c;
+    pl.row0[L] = rows;
+    rows += (int)c;
+  }
+  pl.rows = rows;
+  return pl;
+}
+
+int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipStream_t st,
+                    const double** d_pts) {
+  if (ptr_kind == FE_PTR_DEVICE) {
+    *d_pts = pts;
+    return FE_OK;
+  }
+  int rc = fe_ctx_reserve(c, FE_WS_PTS, (size_t)n * 3 * sizeof(double));
+  if (rc) return rc;
+  FE_HIPCHK(hipMemcpyAsync(c->ws[FE_WS_PTS], pts, (size_t)n * 3 * sizeof(double),
+                           hipMemcpyHostToDevice, st));
+  *d_pts = (const double*)c->ws[FE_WS_PTS];
+  return FE_OK;
+}
+
+extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int precision, int parts,
+                              int ptr_kind, void* stream, double* V) {
+  if (!t || (!pts && n > 0) || (!V && n > 0))
+    return fe_set_error(FE_ERR_ARG, "fe_esp_forward: null argument");
+  if (n < 0 || n > (int64_t)INT32_MAX / 4)
+    return fe_set_error(FE_ERR_ARG, "fe_esp_forward: point count %lld out of range", (long long)n);
+  if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
+    return fe_set_error(FE_ERR_ARG, "fe_esp_forward: bad ptr_kind %d", ptr_kind);
+  if (precision != FE_PREC_FP64 && precision != FE_PREC_MIXED)
+    return fe_set_error(FE_ERR_ARG, "fe_esp_forward: bad precision %d", precision);
+  if (parts < 1 || parts > 3) return fe_set_error(FE_ERR_ARG, "fe_esp_forward: bad parts %d", parts);
+  if (n == 0) return FE_OK;
+  fe_ctx* c = t->ctx;
+  FE_HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  const double* d_pts = nullptr;
+  int rc = fe_stage_points(c, pts, n, ptr_kind, st, &d_pts);
+  if (rc) return rc;
+  double* d_V = V;
+  if (ptr_kind == FE_PTR_HOST) {
+    rc = fe_ctx_reserve(c, FE_WS_OUT, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    d_V = (double*)c->ws[FE_WS_OUT];
+  }
+  FwdPlan pl = fe_plan(t, n, c->ppl, c->target_wg);
+  double* partial = nullptr;
+  if ((parts & FE_PART_ELECTRON) && pl.rows > 0) {
+    rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)pl.rows * pl.ld * sizeof(double));
+    if (rc) return rc;
+    partial = (double*)c->ws[FE_WS_PARTIAL];
+    if (precision == FE_PREC_MIXED) {
+      rc = fe_launch_fwd_mixed(t, pl, d_pts, (int)n, partial, st);
+      if (rc) return rc;
+    } else if (c->ppl == 2) {
+      launch_fwd_all<2>(t, pl, d_pts, (int)n, partial, st);
+    } else {
+      launch_fwd_all<1>(t, pl, d_pts, (int)n, partial, st);
+    }
+    FE_HIPCHK(hipGetLastError());
+  }
+  const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
+  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial,
+                     nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, d_V);
+  FE_HIPCHK(hipGetLastError());
+  if (ptr_kind == FE_PTR_HOST) {
+    FE_HIPCHK(hipMemcpyAsync(V, d_V, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    FE_HIPCHK(hipStreamSynchronize(st));
+  }
+  return FE_OK;
+}
+
+extern "C" int fe_boys_eval(fe_ctx* c, const double* T, int64_t n, int M, int ptr_kind,
+                            void* stream, double* F) {
+  if (!c || (n > 0 && (!T || !F))) return fe_set_error(FE_ERR_ARG, "fe_boys_eval: null argument");
+  if (M < 0 || M > FE_BOYS_MMAX) return fe_set_error(FE_ERR_ARG, "fe_boys_eval: M=%d out of range", M);
+  if (n < 0) return fe_set_error(FE_ERR_ARG, "fe_boys_eval: negative n");
+  if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
+    return fe_set_error(FE_ERR_ARG, "fe_boys_eval: bad ptr_kind");
+  if (n == 0) return FE_OK;
+  FE_HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  const double* dT = T;
+  double* dF = F;
+  if (ptr_kind == FE_PTR_HOST) {
+    int rc = fe_ctx_reserve(c, FE_WS_PTS, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    rc = fe_ctx_reserve(c, FE_WS_OUT, (size_t)n * (M + 1) * sizeof(double));
+    if (rc) return rc;
+    FE_HIPCHK(hipMemcpyAsync(c->ws[FE_WS_PTS], T, (size_t)n * sizeof(double),
+                             hipMemcpyHostToDevice, st));
+    dT = (const double*)c->ws[FE_WS_PTS];
+    dF = (double*)c->ws[FE_WS_OUT];
+  }
+  const dim3 grid((unsigned)((n + 255) / 256));
+  const double* tab = c->d_boys + (size_t)M * FE_BOYS_NR * FE_BOYS_COLS;
+  switch (M) {
+#define FE_BOYS_CASE(m) \
+  case m: hipLaunchKernelGGL(k_boys_eval<m>, grid, dim3(256), 0, st, dT, n, tab, dF); break;
+    FE_BOYS_CASE(0) FE_BOYS_CASE(1) FE_BOYS_CASE(2) FE_BOYS_CASE(3) FE_BOYS_CASE(4)
+    FE_BOYS_CASE(5) FE_BOYS_CASE(6) FE_BOYS_CASE(7) FE_BOYS_CASE(8)
+#undef FE_BOYS_CASE
+  }
+  FE_HIPCHK(hipGetLastError());
+  if (ptr_kind == FE_PTR_HOST) {
+    FE_HIPCHK(hipMemcpyAsync(F, dF, (size_t)n * (M + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    FE_HIPCHK(hipStreamSynchronize(st));
+  }
+  return FE_OK;
+}

@@ -1,0 +1,266 @@
+"""Generate ``fe_lclass.inc``: straight-line per-L code for the HIP kernels.
+
+For a primitive pair of class L = l_A + l_B the term list (alpha, m) of the
+contracted table and its monomial schedule depend on L only
+(``fastesp/esp.py:54-105``, ``fastesp/table.py:198-221``,
+``fastesp/native/build_pairs.cpp:104-151,330-369``).  This script derives them
+independently (own implementation of the a-table recurrence of
+``fastesp/ints.py:46-59`` and the key enumeration) and emits, per L:
+
+* host tables of the canonical keys and monomial schedule (used to validate
+  uploaded tables and to build tables on the GPU);
+* ``fe_contract<L>``: I(R) = sum_k g_k R^alpha_k F_mk with the monomials built
+  once and the terms grouped by Boys order m (one FMA per term);
+* ``fe_contract_grad<L>``: the same plus the gradient with respect to C;
+* ``fe_dg_accum<L>``: the per-term products w R^alpha F_m for dL/dG;
+* ``fe_contract_f32<L>``: the mixed-precision form (FP32 monomials/products).
+
+Run: ``python gen_lclass.py [LMAX]`` (default 6).  Output is deterministic.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+
+def a_table(lmax):
+    a = [[0.0] * (lmax // 2 + 2) for _ in range(lmax + 1)]
+    a[0][0] = 1.0
+    for t in range(lmax):
+        for j in range(lmax // 2 + 2):
+            v = -a[t][j]
+            if j >= 1:
+                v += (t - 2 * j + 2) * a[t][j - 1]
+            a[t + 1][j] = v
+    return a
+
+
+def keys_of(L):
+    """Sorted (ax, ay, az, m) keys of class L and the t -> key coefficient map."""
+    a = a_table(max(L, 1))
+    T = L + 1
+    acc = {}
+    for tx in range(T):
+        for ty in range(T - tx):
+            for tz in range(T - tx - ty):
+                lt = tx + ty + tz
+                sgn = -1.0 if lt % 2 else 1.0
+                for jx in range(tx // 2 + 1):
+                    for jy in range(ty // 2 + 1):
+                        for jz in range(tz // 2 + 1):
+                            c = a[tx][jx] * a[ty][jy] * a[tz][jz]
+                            if c == 0.0:
+                                continue
+                            k = (tx - 2 * jx, ty - 2 * jy, tz - 2 * jz, lt - jx - jy - jz)
+                            acc.setdefault(k, {})
+                            col = (tx * T + ty) * T + tz
+                            acc[k][col] = acc[k].get(col, 0.0) + sgn * c
+    keys = sorted(acc)
+    return keys, acc
+
+
+def schedule_of(L):
+    keys, _ = keys_of(L)
+    grown = set(k[:3] for k in keys)
+    stack = list(grown)
+    while stack:
+        u = stack.pop()
+        for ax in range(3):
+            if u[ax] > 0:
+                w = list(u)
+                w[ax] -= 1
+                w = tuple(w)
+                if w not in grown:
+                    grown.add(w)
+                    stack.append(w)
+    order = sorted(grown)
+    idx = {m: i for i, m in enumerate(order)}
+    parent, axis = [], []
+    for m in order:
+        par, axv = 0, 0
+        for ax in range(3):
+            if m[ax] > 0:
+                w = list(m)
+                w[ax] -= 1
+                par, axv = idx[tuple(w)], ax
+                break
+        parent.append(par)
+        axis.append(axv)
+    aidx = [idx[k[:3]] for k in keys]
+    return order, parent, axis, aidx
+
+
+def stride_of(nt):
+    return 8 + ((nt + 7) // 8) * 8
+
+
+def emit_monos(out, L, order, parent, axis, T="double", pre="m", R=("Rx", "Ry", "Rz")):
+    for j, m in enumerate(order):
+        if j == 0:
+            continue
+        par = parent[j]
+        src = f"{pre}{par}" if par != 0 else None
+        r = R[axis[j]]
+        out.append(f"  const {T} {pre}{j} = {src + ' * ' + r if src else r};")
+
+
+def gen_L(L):
+    keys, acc = keys_of(L)
+    order, parent, axis, aidx = schedule_of(L)
+    nt, nm = len(keys), len(order)
+    o = []
+    o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}")
+    o.append(f"template <> struct FeL<{L}> {{ static constexpr int NT = {nt}, NM = {nm}, "
+             f"STRIDE = {stride_of(nt)}; }};")
+
+    def mono(j, pre="m", one="1.0"):
+        return one if j == 0 else f"{pre}{j}"
+
+    # ---------------- forward contraction, FP64
+    o.append(f"template <> __device__ __forceinline__ double fe_contract<{L}>("
+             "const double Rx, const double Ry, const double Rz, const double* F, "
+             "const double* __restrict__ g) {")
+    emit_monos(o, L, order, parent, axis)
+    seen = set()
+    for k, (ax, ay, az, m) in enumerate(keys):
+        mj = aidx[k]
+        if m not in seen:
+            seen.add(m)
+            o.append(f"  double a{m} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
+        else:
+            o.append(f"  a{m} = " + (f"a{m} + g[{k}];" if mj == 0 else f"fma(g[{k}], m{mj}, a{m});"))
+    expr = f"a{L} * F[{L}]" if L in seen else "0.0"
+    for m in range(L - 1, -1, -1):
+        if m in seen:
+            expr = f"fma(a{m}, F[{m}], {expr})"
+    o.append(f"  return {expr};")
+    o.append("}")
+
+    # ---------------- forward + gradient wrt C, FP64 (F has L+2 entries)
+    # I = sum_k g R^a F_m ; dI/dC = sum_k g [ d(R^a) F_m - 2p R R^a F_{m+1} ]
+    o.append(f"template <> __device__ __forceinline__ void fe_contract_grad<{L}>("
+             "const double Rx, const double Ry, const double Rz, const double* F, "
+             "const double* __restrict__ g, const double p, double& I, double& Gx, "
+             "double& Gy, double& Gz) {")
+    emit_monos(o, L, order, parent, axis)
+    midx = {m: i for i, m in enumerate(order)}
+    # accumulators: per m -> value poly a_m, and per (axis, m) derivative polys
+    lines = []
+    init = set()
+
+    def add(var, coef_expr, mono_j, scale=1):
+        g = coef_expr if scale == 1 else f"({scale}.0 * {coef_expr})"
+        if var not in init:
+            init.add(var)
+            lines.append(f"  double {var} = " + (f"{g};" if mono_j == 0 else f"{g} * m{mono_j};"))
+        else:
+            lines.append(f"  {var} = " + (f"{var} + {g};" if mono_j == 0 else f"fma({g}, m{mono_j}, {var});"))
+
+    for k, (ax, ay, az, m) in enumerate(keys):
+        add(f"a{m}", f"g[{k}]", aidx[k])
+        for d, al in enumerate((ax, ay, az)):
+            if al > 0:
+                lo = [ax, ay, az]
+                lo[d] -= 1
+                add(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+    o.extend(lines)
+    ms = sorted({k[3] for k in keys})
+    # I = sum_m a_m F_m ; S1 = sum_m a_m F_{m+1}
+    i_expr = "0.0"
+    s_expr = "0.0"
+    for m in ms:
+        i_expr = f"fma(a{m}, F[{m}], {i_expr})"
+        s_expr = f"fma(a{m}, F[{m + 1}], {s_expr})"
+    o.append(f"  I = {i_expr};")
+    o.append(f"  const double s1 = -2.0 * p * ({s_expr});")
+    for d in "xyz":
+        e = f"s1 * R{d}"
+        for m in ms:
+            if f"d{d}{m}" in init:
+                e = f"fma(d{d}{m}, F[{m}], {e})"
+        o.append(f"  G{d} = {e};")
+    o.append("}")
+
+    # ---------------- dL/dG accumulation: acc[k] += w * R^a_k * F_mk
+    o.append(f"template <> __device__ __forceinline__ void fe_dg_accum<{L}>("
+             "const double Rx, const double Ry, const double Rz, const double* F, "
+             "const double w, double* __restrict__ acc) {")
+    emit_monos(o, L, order, parent, axis)
+    wf = {}
+    for m in ms:
+        o.append(f"  const double w{m} = w * F[{m}];")
+    for k, (ax, ay, az, m) in enumerate(keys):
+        mj = aidx[k]
+        o.append(f"  acc[{k}] = " + (f"acc[{k}] + w{m};" if mj == 0 else f"fma(w{m}, m{mj}, acc[{k}]);"))
+    o.append("}")
+
+    # ---------------- mixed: FP32 monomials and per-m polynomials, FP64 F and final sum
+    o.append(f"template <> __device__ __forceinline__ double fe_contract_f32<{L}>("
+             "const float Rx, const float Ry, const float Rz, const double* F, "
+             "const float* __restrict__ g) {")
+    emit_monos(o, L, order, parent, axis, T="float")
+    seen = set()
+    for k, (ax, ay, az, m) in enumerate(keys):
+        mj = aidx[k]
+        if m not in seen:
+            seen.add(m)
+            o.append(f"  float a{m} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
+        else:
+            o.append(f"  a{m} = " + (f"a{m} + g[{k}];" if mj == 0 else f"fmaf(g[{k}], m{mj}, a{m});"))
+    expr = "0.0"
+    for m in range(L, -1, -1):
+        if m in seen:
+            expr = f"fma((double)a{m}, F[{m}], {expr})"
+    o.append(f"  return {expr};")
+    o.append("}")
+    return o, keys, order, parent, axis, aidx
+
+
+def main():
+    lmax = int(sys.argv[1]) if len(sys.argv) > 1 else 6
+    out = ["// GENERATED by gen_lclass.py -- do not edit by hand.",
+           f"// Per-L straight-line code for L = 0..{lmax}.", "#pragma once",
+           f"#define FE_LMAX {lmax}",
+           "template <int L> struct FeL;",
+           "template <int L> __device__ __forceinline__ double fe_contract(const double, const double, const double, const double*, const double* __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_contract_grad(const double, const double, const double, const double*, const double* __restrict__, const double, double&, double&, double&, double&);",
+           "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, const double, double* __restrict__);",
+           "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, const float* __restrict__);",
+           ""]
+    host = ["// host-side canonical tables", "namespace fe_host {"]
+    nts, nms, koff, moff = [], [], [0], [0]
+    allkeys, allmono = [], []
+    for L in range(lmax + 1):
+        o, keys, order, parent, axis, aidx = gen_L(L)
+        out.extend(o)
+        out.append("")
+        nts.append(len(keys))
+        nms.append(len(order))
+        for k, key in enumerate(keys):
+            allkeys.append((*key, aidx[k]))
+        for j in range(len(order)):
+            allmono.append((*order[j], parent[j], axis[j]))
+        koff.append(koff[-1] + len(keys))
+        moff.append(moff[-1] + len(order))
+    host.append(f"static const int NTERM[{lmax + 1}] = {{{', '.join(map(str, nts))}}};")
+    host.append(f"static const int NMONO[{lmax + 1}] = {{{', '.join(map(str, nms))}}};")
+    host.append(f"static const int STRIDE[{lmax + 1}] = {{{', '.join(str(stride_of(n)) for n in nts)}}};")
+    host.append(f"static const int KEY_OFF[{lmax + 2}] = {{{', '.join(map(str, koff))}}};")
+    host.append(f"static const int MONO_OFF[{lmax + 2}] = {{{', '.join(map(str, moff))}}};")
+    host.append("// ax, ay, az, m, aidx")
+    host.append(f"static const int KEYS[{len(allkeys)}][5] = {{")
+    host.extend("  {" + ", ".join(map(str, k)) + "}," for k in allkeys)
+    host.append("};")
+    host.append("// ax, ay, az, parent, axis")
+    host.append(f"static const int MONOS[{len(allmono)}][5] = {{")
+    host.extend("  {" + ", ".join(map(str, k)) + "}," for k in allmono)
+    host.append("};")
+    host.append("}  // namespace fe_host")
+    out.extend(host)
+    dst = Path(__file__).with_name("fe_lclass.inc")
+    dst.write_text("\n".join(out) + "\n")
+    print(f"wrote {dst} (L <= {lmax})")
+
+
+if __name__ == "__main__":
+    main()

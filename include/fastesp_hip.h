@@ -1,0 +1,119 @@
+/*
+ * fastesp_hip.h -- C ABI of libfastesp_hip.so, the MI355X (gfx950) backend of
+ * fastesp's ESP hot path.
+ *
+ * The reference exposes this path as two processes exchanging files
+ * (fastesp/cli.py:143-163): the table builder `build_pairs <basis.bin>
+ * <pairs.bin> [tol]` (fastesp/native/build_pairs.cpp:162-168) and the CUDA
+ * evaluator `esp_gpu <pairs.bin> <out.txt> [--repeats N]`
+ * (fastesp/native/esp_gpu.cu:232-242), wrapped in Python by
+ * `build_table(...)` / `PairTable.esp()` (fastesp/table.py:73-104,176-259).
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 (FE_OK) on success and a negative FE_ERR_* code
+ *    on failure; it never calls exit().  fe_last_error() returns a
+ *    thread-local message describing the last failure of the calling thread.
+ *  - Host arrays passed in are borrowed (C-contiguous, little-endian f64 /
+ *    i32, caller-owned) and not retained after the call returns.
+ *  - `ptr_kind` says where point/output buffers live: FE_PTR_HOST (the library
+ *    copies through device staging buffers and synchronises the stream before
+ *    returning) or FE_PTR_DEVICE (device pointers; work is only enqueued on
+ *    `stream`, which is a hipStream_t passed as void*, NULL = default stream).
+ *  - One fe_ctx per device.  Calls on one ctx (and the tables made from it)
+ *    must be serialised by the caller.
+ *  - Units: Bohr in, Hartree/e out (fastesp/table.py:73-104).
+ */
+#ifndef FASTESP_HIP_H
+#define FASTESP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FE_OK 0
+#define FE_ERR_ARG (-1)         /* invalid argument / shape                    */
+#define FE_ERR_HIP (-2)         /* HIP runtime error                           */
+#define FE_ERR_UNSUPPORTED (-3) /* e.g. L = l_A + l_B above FE_LMAX            */
+#define FE_ERR_FORMAT (-4)      /* table not in the canonical per-L layout     */
+#define FE_ERR_ALLOC (-5)       /* device or host allocation failed            */
+#define FE_ERR_STATE (-6)       /* wrong object / use after free               */
+
+#define FE_PTR_HOST 0
+#define FE_PTR_DEVICE 1
+
+#define FE_PREC_FP64 0          /* FP64 throughout                             */
+#define FE_PREC_MIXED 1         /* FP32 monomial products, FP64 Boys + sums    */
+
+#define FE_PART_NUCLEAR 1       /* + sum_A Z_A/|C-R_A|   (fastesp/table.py:95-100) */
+#define FE_PART_ELECTRON 2      /* - sum_pairs K sum_k g R^a F_m (table.py:73-93)  */
+#define FE_PART_ESP 3           /* nuclear - electron      (table.py:102-104)     */
+
+typedef struct fe_ctx fe_ctx;
+typedef struct fe_table fe_table;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int fe_version(void);
+
+/* Thread-local description of the last error ("" if none). */
+const char* fe_last_error(void);
+
+/* Create a context on HIP device `device`: uploads the Boys tables and
+ * caches device properties.  Replaces the per-process CUDA context set-up and
+ * warm-up launch of esp_gpu (fastesp/native/esp_gpu.cu:244-297). */
+int fe_init(int device, fe_ctx** ctx);
+
+/* Free every device buffer of the context (tables must be freed first). */
+int fe_finalize(fe_ctx* ctx);
+
+/* Upload an existing flat pair table (the pairs.bin fields of
+ * fastesp/table.py:11-24, as written by build_pairs.cpp:386-421 or
+ * PairTable.write, fastesp/table.py:119-141) and repack it into the per-L
+ * device layout.  Replaces the pairs.bin loader + H2D copies of esp_gpu
+ * (fastesp/native/esp_gpu.cu:96-134,251-277).  Every pair must use the
+ * canonical term list and monomial schedule of its class (FE_ERR_FORMAT
+ * otherwise).  Z/xyz describe the nuclei (n_atoms may be 0). */
+int fe_table_from_pairs(fe_ctx* ctx, int n_pairs, int n_terms,
+                        const double* pp_exp, const double* pp_cx,
+                        const double* pp_cy, const double* pp_cz,
+                        const double* pp_K, const int32_t* pp_nt,
+                        const int32_t* pp_off, const int32_t* pp_nmono,
+                        const int32_t* pp_monooff, const int32_t* mo_parent,
+                        const int32_t* mo_axis, const int32_t* tm_aidx,
+                        const int32_t* tm_m, const double* tm_g, int n_atoms,
+                        const double* Z, const double* xyz, fe_table** out);
+
+/* Release a table and its device memory. */
+int fe_table_free(fe_table* t);
+
+/* Table statistics: info[0] n_pairs, info[1] n_terms, info[2] n_atoms,
+ * info[3..3+FE_LMAX] pairs per class L, info[10] device bytes,
+ * info[11] candidate pairs before the point screen (0 if unknown). */
+int fe_table_info(const fe_table* t, int64_t* info, int n_info);
+
+/* ESP at n points: V = [nuclear] - [electron] as selected by `parts`.
+ * pts is (n,3) f64; V is (n,) f64.  Replaces esp_pairs_kernel +
+ * esp_reduce_kernel (fastesp/native/esp_gpu.cu:161-229) and
+ * PairTable.esp/electron/nuclear (fastesp/table.py:73-104).
+ * `precision` is FE_PREC_FP64 or FE_PREC_MIXED. */
+int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int precision,
+                   int parts, int ptr_kind, void* stream, double* V);
+
+/* Boys functions F_0..F_M(T) for n arguments with the device algorithm
+ * (F is (n, M+1) f64, row-major).  Replaces boys_all
+ * (fastesp/native/esp_gpu.cu:36-66) / fastesp.ints.boys (fastesp/ints.py:35-40)
+ * for parity testing of the device scheme.  0 <= M <= FE_BOYS_MMAX (8). */
+int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
+                 void* stream, double* F);
+
+/* Tuning knobs of the evaluation kernels (autotuner output).
+ * key 0: points per lane (1 or 2); key 1: target workgroups per launch;
+ * key 2: use per-tile screening (0/1) -- reserved. */
+int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTESP_HIP_H */

@@ -1,0 +1,206 @@
+"""Pins of the oracle's table build, evaluator and adjoint (CPU)."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fastesp_amd import synth
+from fastesp_amd.basis import cart_order
+from fastesp_amd.formats import basis_payload, read_pairs, write_basis_payload
+from oracle import engine, evaluate
+from oracle.ints import primitive_pair_integral
+from tests.helpers import cf_toy, oh_toy, quad_electron
+
+ROOT = Path(__file__).resolve().parents[1]
+REF_BUILDER = ROOT / "oracle" / "_ref" / "build_pairs"
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    return synth.config0()
+
+
+def _ref_pairs(w, tmp_path, tol):
+    if not REF_BUILDER.exists():
+        pytest.skip("reference builder not compiled (make -C oracle ref)")
+    b = tmp_path / "basis.bin"
+    p = tmp_path / "pairs.bin"
+    write_basis_payload(b, basis_payload(w.wfn, w.basis, w.density, w.points))
+    r = subprocess.run([str(REF_BUILDER), str(b), str(p), repr(tol)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    return read_pairs(p), r.stderr
+
+
+def _key(p, P, K):
+    return (round(float(p), 11), round(float(P[0]), 9), round(float(P[1]), 9),
+            round(float(P[2]), 9), round(float(np.log(K)), 7))
+
+
+def test_table_matches_reference_builder(cfg0, tmp_path):
+    """Every primitive pair of the compiled reference builder (tol = 0) is in the
+    oracle table with the same p, P, K and G (order differs: OpenMP merge)."""
+    ref, err = _ref_pairs(cfg0, tmp_path, 0.0)
+    tab = engine.build_table(cfg0.wfn, cfg0.basis, cfg0.density, cfg0.points, 0.0)
+    assert ref.n_pairs == tab.n_pairs and ref.n_terms == tab.n_terms
+    pool = {}
+    for j in range(ref.n_pairs):
+        pool.setdefault(_key(ref.pp_exp[j], (ref.pp_cx[j], ref.pp_cy[j], ref.pp_cz[j]),
+                             ref.pp_K[j]), []).append(j)
+    worst = 0.0
+    for i in range(tab.n_pairs):
+        j = pool[_key(tab.pp_exp[i], (tab.pp_cx[i], tab.pp_cy[i], tab.pp_cz[i]),
+                      tab.pp_K[i])].pop()
+        a, b = tab.pp_off[i], ref.pp_off[j]
+        n = tab.pp_nt[i]
+        assert n == ref.pp_nt[j]
+        assert (tab.tm_aidx[a:a + n] == ref.tm_aidx[b:b + n]).all()
+        assert (tab.tm_m[a:a + n] == ref.tm_m[b:b + n]).all()
+        gt, gr = tab.tm_g[a:a + n], ref.tm_g[b:b + n]
+        worst = max(worst, np.abs(gt - gr).max() / (np.abs(gr).max() + 1e-300))
+    assert worst < 1e-13
+    # the reference prints the sum rule; the oracle accumulates the same quantity
+    nel = float(err.split("Tr(P S) =")[1].split()[0])
+    assert nel == pytest.approx(tab.sum_rule, abs=1e-8)
+    assert tab.sum_rule == pytest.approx(cfg0.wfn.nelec, abs=1e-9)
+
+
+def test_screen_bound_matches_reference_builder(cfg0, tmp_path):
+    """The reference C++ screen keeps K*bound/max(dmin-reff, 0.05) >= tol; the
+    oracle's bound and reff reproduce its kept set exactly (pins the |He|
+    moment table, the (2p)^(|t|/2) weights and reff)."""
+    tol = 1e-9
+    ref, _ = _ref_pairs(cfg0, tmp_path, tol)
+    prims = engine.build_prim_pairs(cfg0.basis, cfg0.density)
+    dmin = engine.min_distance(prims.P, cfg0.points)
+    val = prims.K * prims.bound / np.maximum(dmin - prims.reff, 0.05)
+    keep = ~(val < tol)
+    anyg = np.array([np.any(g != 0.0) for g in prims.G])
+    keep &= anyg
+    assert keep.sum() == ref.n_pairs
+    mine = sorted(_key(prims.p[i], prims.P[i], prims.K[i]) for i in np.nonzero(keep)[0])
+    theirs = sorted(_key(ref.pp_exp[j], (ref.pp_cx[j], ref.pp_cy[j], ref.pp_cz[j]), ref.pp_K[j])
+                    for j in range(ref.n_pairs))
+    assert mine == theirs
+
+
+def test_sum_rule_equals_electron_count(cfg0):
+    prims = engine.build_prim_pairs(cfg0.basis, cfg0.density)
+    assert prims.sum_rule == pytest.approx(cfg0.wfn.nelec, abs=1e-9)
+
+
+def test_table_equals_primitive_integrals():
+    """Table evaluation == sum_{mu nu} P_{mu nu} sum_prims c c <a|1/r_C|b> (tol = 0)."""
+    wfn, basis, P, _ = cf_toy()
+    pts = np.array([[2.5, -0.3, 0.4], [-1.2, 1.9, 0.8], [0.4, 0.5, -0.2]])
+    tab = engine.build_table(wfn, basis, P, pts, tol=0.0)
+    got = evaluate.electron(tab)
+    want = np.zeros(len(pts))
+    for k, C in enumerate(pts):
+        s = 0.0
+        for mu in range(basis.n):
+            for nu in range(basis.n):
+                if P[mu, nu] == 0.0:
+                    continue
+                for a, ca in zip(basis.exps[mu], basis.coefs[mu]):
+                    for b, cb in zip(basis.exps[nu], basis.coefs[nu]):
+                        s += P[mu, nu] * ca * cb * primitive_pair_integral(
+                            a, basis.centers[mu], basis.ang[mu], b, basis.centers[nu],
+                            basis.ang[nu], C)
+        want[k] = s
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13)
+
+
+def test_table_matches_direct_quadrature():
+    """``tests/test_pipeline.py:114-126``: electron part vs quadrature of rho/|C-r|."""
+    wfn, basis, P, _ = oh_toy()
+    pts = np.array([[3.0, 0.4, 0.3], [-1.5, 1.2, -0.7]])
+    got = evaluate.electron(engine.build_table(wfn, basis, P, pts, tol=0.0))
+    want = quad_electron(basis, P, pts)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-10)
+
+
+def test_screening_does_not_change_the_answer():
+    """``tests/test_pipeline.py:167-178``."""
+    wfn, basis, P, _ = oh_toy(seed=3)
+    pts = np.array([[3.2, 0.5, -1.1], [-2.0, -1.0, 0.9], [0.1, 2.4, 0.4]])
+    full = evaluate.electron(engine.build_table(wfn, basis, P, pts, tol=0.0))
+    for tol in (1e-8, 1e-10, 1e-12):
+        got = evaluate.electron(engine.build_table(wfn, basis, P, pts, tol=tol))
+        assert np.abs(got - full).max() < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# adjoint
+# ---------------------------------------------------------------------------
+def test_field_matches_finite_differences():
+    wfn, basis, P, _ = cf_toy()
+    pts = np.array([[2.5, -0.3, 0.4], [-1.2, 1.9, 0.8], [3.1, 2.2, -1.4]])
+    tab = engine.build_table(wfn, basis, P, pts, tol=0.0)
+    Fld = evaluate.field(tab, pts)
+    h = 1e-4
+    for ax in range(3):
+        d = np.zeros(3)
+        d[ax] = h
+        fd = (evaluate.esp(tab, pts + d) - evaluate.esp(tab, pts - d)) / (2 * h)
+        np.testing.assert_allclose(Fld[:, ax], fd, rtol=1e-6, atol=1e-9)
+
+
+def test_field_ss_closed_form():
+    """One s-s pair: V_el = K (pi/p)^1.5 erf(sqrt(p) R)/R * c; field from the closed form."""
+    from math import erf, exp, pi, sqrt
+    wfn, basis, P, _ = oh_toy()
+    # single-pair table built by hand from the oracle for an s-s diagonal block
+    tab = engine.build_table(wfn, basis, P, np.zeros((1, 3)), tol=0.0)
+    sel = np.nonzero(tab.pp_L == 0)[0][0]
+    p = tab.pp_exp[sel]
+    c = np.array([tab.pp_cx[sel], tab.pp_cy[sel], tab.pp_cz[sel]])
+    K = tab.pp_K[sel]
+    g0 = tab.tm_g[tab.pp_off[sel]]
+    C = c + np.array([0.7, -0.4, 1.1])
+    R = np.linalg.norm(C - c)
+    # I = K g0 F_0(p R^2) with g0 = 2 pi/p * w; dI/dC = K g0 dF0/dR * (C-c)/R
+    F0 = 0.5 * sqrt(pi / (p * R * R)) * erf(sqrt(p) * R)
+    dF0 = (exp(-p * R * R) / R - F0 / R)  # d/dR [sqrt(pi)/2 erf(sqrt p R)/(sqrt p R)]
+    want = K * g0 * dF0 * (C - c) / R
+    sub = engine.OracleTable(**{k: getattr(tab, k) for k in (
+        "pp_exp", "pp_cx", "pp_cy", "pp_cz", "pp_K", "pp_nt", "pp_off", "pp_nmono",
+        "pp_monooff", "mo_parent", "mo_axis", "tm_aidx", "tm_m", "tm_g", "pts", "Z", "xyz")})
+    keep = np.array([sel])
+    for k in ("pp_exp", "pp_cx", "pp_cy", "pp_cz", "pp_K", "pp_nt", "pp_off", "pp_nmono",
+              "pp_monooff"):
+        setattr(sub, k, getattr(tab, k)[keep])
+    sub.pp_L = tab.pp_L[keep]
+    sub.Z = np.zeros(0)
+    sub.xyz = np.zeros((0, 3))
+    got = evaluate.field(sub, C[None, :])[0]
+    assert F0 * K * g0 == pytest.approx(evaluate.electron(sub, C[None, :])[0], rel=1e-13)
+    np.testing.assert_allclose(-got, want, rtol=1e-11, atol=1e-14)
+
+
+def test_grad_P_matches_finite_differences():
+    """V is linear in P, so central differences are exact up to rounding."""
+    wfn, basis, P, _ = oh_toy(seed=5)
+    pts = np.array([[3.0, 0.4, 0.3], [-1.5, 1.2, -0.7], [0.5, -2.8, 1.0]])
+    rng = np.random.default_rng(9)
+    w = rng.uniform(0.5, 1.0, len(pts))
+    tab = engine.build_table(wfn, basis, P, pts, tol=0.0)
+    V = evaluate.esp(tab, pts)
+    target = V + rng.normal(0, 1e-3, len(pts))
+    _, gV = evaluate.loss_and_gV(V, target, w)
+    dG = evaluate.grad_G(tab, pts, gV)
+    dP = evaluate.grad_P(tab, basis, dG)
+    np.testing.assert_allclose(dP, dP.T, atol=1e-14)
+    h = 1e-6
+    for (mu, nu) in [(0, 0), (1, 3), (4, 2), (5, 11), (12, 13), (9, 9)]:
+        E = np.zeros_like(P)
+        E[mu, nu] = h
+        Vp = evaluate.esp(engine.build_table(wfn, basis, P + E, pts, tol=0.0), pts)
+        Vm = evaluate.esp(engine.build_table(wfn, basis, P - E, pts, tol=0.0), pts)
+        fd = float((gV * (Vp - Vm)).sum() / (2 * h))
+        assert dP[mu, nu] == pytest.approx(fd, rel=1e-7, abs=1e-11)
+
+
+def test_cart_order_matches_reference_convention():
+    assert cart_order(2) == ((2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2))
