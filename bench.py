@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""ESP hot-path benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--points 30000]
+                    [--mode fwd|fwdbwd] [--precision fp64|mixed] [--no-cpu]
+
+Workload: the 1634-function protein-fragment-sized synthetic molecule (54
+atoms, C/N/O/H + 2 S, TZ-like s/p/d/f basis, seed 1001) with 30000 points on
+its 1.4 x vdW surface.  The pair table is built on the GPU (timed and reported
+separately); a step is one pass of the evaluation over all 30000 points with
+the table and the points already resident in HBM.  `value` is ms per 1e4
+points of a step (lower is better).
+
+Printed JSON (one line): the driver's contract fields plus
+  roofline     -- algorithmic FP64 FLOPs of the evaluation kernels per step /
+                  their HIP-event time, against the FP64 vector peak;
+  cpu_baseline -- the C++/OpenMP port of the reference path (table build +
+                  evaluation) timed on this host's cores on a bounded point
+                  sample, scaled to ms per 1e4 points;
+  accuracy     -- max |dV| against the FP64 oracle at 256 sampled points.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=30000)
+    ap.add_argument("--mode", choices=["fwd", "fwdbwd"], default="fwd")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--ppl", type=int, default=0, help="points per lane (0 = library default)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, budget_s: float) -> dict:
+    """C++/OpenMP port (oracle/cpu_port) on this host: build + evaluate a sample."""
+    sys.path.insert(0, str(ROOT))
+    from oracle.cpu_port import CpuTable, threads
+    nt = threads()
+    t0 = time.perf_counter()
+    ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, nthreads=nt)
+    t_build = time.perf_counter() - t0
+    # calibrate the sample so the evaluation takes about `budget_s`
+    n = 32
+    t0 = time.perf_counter()
+    ct.esp(w.points[:n], nthreads=nt)
+    dt = time.perf_counter() - t0
+    n2 = int(min(len(w.points), max(n, n * budget_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    ct.esp(w.points[:n2], nthreads=nt)
+    dt2 = time.perf_counter() - t0
+    ms_per_1e4 = dt2 * 1e3 / n2 * 1e4
+    out = dict(value=round(ms_per_1e4, 3), unit="ms/1e4 pts", cores=nt, kind="port",
+               sample=f"C++/OpenMP port (oracle/cpu_port.cpp): FP64 forward on the first "
+                      f"{n2} of {len(w.points)} points with its own table "
+                      f"({ct.n_pairs} pairs, built in {t_build:.2f} s), scaled linearly to 1e4",
+               build_s=round(t_build, 3))
+    ref = ROOT / "oracle" / "_ref" / "build_pairs"
+    if ref.exists():
+        from fastesp_amd.formats import basis_payload, write_basis_payload
+        tmp = Path(os.environ.get("TMPDIR", "/tmp"))
+        b, p = tmp / f"fe_bench_basis_{os.getpid()}.bin", tmp / f"fe_bench_pairs_{os.getpid()}.bin"
+        write_basis_payload(b, basis_payload(w.wfn, w.basis, w.density, w.points))
+        t0 = time.perf_counter()
+        r = subprocess.run([str(ref), str(b), str(p), "1e-12"], capture_output=True, text=True)
+        out["reference_build_pairs_s"] = round(time.perf_counter() - t0, 3)
+        out["reference_build_pairs_rc"] = r.returncode
+        for f in (b, p):
+            f.unlink(missing_ok=True)
+    return out
+
+
+def main() -> int:
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
+                                   "design note, not an implemented path", "n_gpus": a.gpus}))
+        return 2
+    import torch
+    from fastesp_amd import _lib, flops, synth
+    from fastesp_amd.table import build_table
+
+    log(f"generating workload (config 1, {a.points} points)")
+    w = synth.config1(a.points)
+    ctx = _lib.Context.get(0)
+    if a.ppl:
+        ctx.set_tuning(0, a.ppl)
+    # --- table build on the GPU (reported, not part of the step) ----------
+    build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    t_build = time.perf_counter() - t0
+    info = tab.info()
+    log(f"table: {info['n_pairs']} pairs, per L {info['pairs_per_L']}, build {t_build*1e3:.1f} ms")
+
+    dev = torch.device("cuda:0")
+    pts = torch.from_numpy(np.ascontiguousarray(w.points)).to(dev)
+    V = torch.empty(a.points, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    prec = _lib.FE_PREC_MIXED if a.precision == "mixed" else _lib.FE_PREC_FP64
+    lib = ctx.lib
+
+    def fwd():
+        _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), a.points, prec, _lib.FE_PART_ESP,
+                                      _lib.FE_PTR_DEVICE, sh, V.data_ptr()), "fe_esp_forward")
+
+    step = fwd
+    if a.mode == "fwdbwd":
+        raise SystemExit("fwdbwd mode needs the backward kernels")
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(a.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / a.steps
+    log(f"{a.steps} steps: {ms_step:.3f} ms/step (HIP events), wall {wall*1e3/a.steps:.3f} ms/step")
+
+    # --- roofline of the evaluation kernels (FP64 VALU) --------------------
+    fl = flops.forward_flops(info["pairs_per_L"], a.points, info["n_atoms"])
+    achieved = fl / (ms_step * 1e-3)
+    roof = dict(bound="valu", achieved=round(achieved / 1e12, 3),
+                peak=flops.PEAK_FP64_VALU / 1e12, unit="TFLOP/s",
+                frac=round(achieved / flops.PEAK_FP64_VALU, 4), traffic=None,
+                kernel="k_esp_fwd<L> x7 + k_esp_reduce (one fe_esp_forward)",
+                flops_per_launch=fl)
+
+    # --- accuracy against the FP64 oracle (golden, 256 sampled points) -----
+    acc = {}
+    gold = ROOT / "tests" / "golden" / "cfg1_30k.npz"
+    if gold.exists() and a.points == 30000:
+        g = np.load(gold)
+        Vh = V.cpu().numpy()
+        d = Vh[g["idx"]] - g["V"]
+        acc = dict(max_abs_dV_au=float(np.abs(d).max()), rms_dV_au=float(np.sqrt((d * d).mean())),
+                   n_check=int(len(d)), oracle_pairs=int(g["npairs"]), gpu_pairs=info["n_pairs"])
+
+    cpu = None
+    if not a.no_cpu:
+        log("CPU baseline (C++/OpenMP port)")
+        cpu = cpu_baseline(w, a.cpu_seconds)
+
+    value = ms_step / (a.points / 1e4)
+    out = {
+        "metric": "ESP ms/1e4 pts (1634 fns) FP64 vs mixed, fwd+bwd; max err vs FP64 oracle",
+        "value": round(value, 4),
+        "unit": "ms/1e4 pts",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded config-1 molecule and vdW-surface points)",
+        "config": {"workload": f"esp_{a.mode}_{a.precision}", "molecule": "config1 54 atoms",
+                   "n_functions": int(w.basis.n), "n_points": a.points,
+                   "n_pairs": info["pairs_per_L"], "tol": 1e-12, "parallelism": "single GPU"},
+        "table_build_ms": round(t_build * 1e3, 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -43,6 +43,14 @@ SIGNATURES = {
     "fe_esp_forward": [_vp, _vp, C.c_int64, C.c_int, C.c_int, C.c_int, _vp, _vp],
     "fe_boys_eval": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp],
     "fe_set_tuning": [_vp, C.c_int, C.c_int64],
+    "fe_table_build": [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp,
+                       _vp, C.c_int, _vp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, _vp,
+                       C.POINTER(_vp)],
+    "fe_table_stats": [_vp, _dp, C.c_int],
+    "fe_esp_forward_grad": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp, _vp],
+    "fe_esp_backward": [_vp, _vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, _vp, _vp],
+    "fe_table_export": [_vp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip,
+                        _dp, _dp, _dp],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 

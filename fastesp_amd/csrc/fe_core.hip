@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fastesp_hip.h"
@@ -91,6 +92,57 @@ static std::vector<double> make_boys_tables() {
 }
 
 // ---------------------------------------------------------------------------
+// per-L tables on the device
+// ---------------------------------------------------------------------------
+static int upload_ltab(fe_ctx* c) {
+  using namespace fe_host;
+  const int nkeys = KEY_OFF[FE_LMAX + 1];
+  const int nmono = MONO_OFF[FE_LMAX + 1];
+  const int nrow = (int)(sizeof(M0_ROWPTR) / sizeof(int));
+  const int nnz = (int)(sizeof(M0_TIDX) / sizeof(int));
+  std::vector<int> iv;
+  auto put = [&](const int* p, int n) {
+    const size_t o = iv.size();
+    iv.insert(iv.end(), p, p + n);
+    return o;
+  };
+  std::vector<int> key_m(nkeys), tl(3 * nmono);
+  for (int k = 0; k < nkeys; ++k) key_m[k] = KEYS[k][3];
+  for (int m = 0; m < nmono; ++m)
+    for (int a = 0; a < 3; ++a) tl[3 * m + a] = MONOS[m][a];
+  const size_t o_koff = put(KEY_OFF, FE_LMAX + 2);
+  const size_t o_km = put(key_m.data(), nkeys);
+  const size_t o_rp = put(M0_ROWPTR, nrow);
+  const size_t o_ti = put(M0_TIDX, nnz);
+  const size_t o_mo = put(MONO_OFF, FE_LMAX + 2);
+  const size_t o_tl = put(tl.data(), 3 * nmono);
+  const int ncp = (int)(sizeof(M0T_COLPTR) / sizeof(int));
+  const int nnzt = (int)(sizeof(M0T_K) / sizeof(int));
+  const size_t o_cp = put(M0T_COLPTR, ncp);
+  const size_t o_tk = put(M0T_K, nnzt);
+  const size_t ibytes = ((iv.size() * sizeof(int)) + 255) & ~(size_t)255;
+  const size_t bytes = ibytes + (nnz + nnzt) * sizeof(double);
+  FE_HIPCHK(hipMalloc(&c->d_ltab_mem, bytes));
+  FE_HIPCHK(hipMemcpy(c->d_ltab_mem, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
+  FE_HIPCHK(hipMemcpy((char*)c->d_ltab_mem + ibytes, M0_VAL, nnz * sizeof(double),
+                      hipMemcpyHostToDevice));
+  FE_HIPCHK(hipMemcpy((char*)c->d_ltab_mem + ibytes + nnz * sizeof(double), M0T_VAL,
+                      nnzt * sizeof(double), hipMemcpyHostToDevice));
+  const int* ib = (const int*)c->d_ltab_mem;
+  c->ltab.key_off = ib + o_koff;
+  c->ltab.key_m = ib + o_km;
+  c->ltab.m0_rowptr = ib + o_rp;
+  c->ltab.m0_tidx = ib + o_ti;
+  c->ltab.mono_off = ib + o_mo;
+  c->ltab.tlist = ib + o_tl;
+  c->ltab.m0_val = (const double*)((const char*)c->d_ltab_mem + ibytes);
+  c->ltab.m0t_colptr = ib + o_cp;
+  c->ltab.m0t_k = ib + o_tk;
+  c->ltab.m0t_val = c->ltab.m0_val + nnz;
+  return FE_OK;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes) {
@@ -135,6 +187,12 @@ extern "C" int fe_init(int device, fe_ctx** out) {
     delete c;
     return fe_set_error(FE_ERR_HIP, "fe_init: Boys table upload failed");
   }
+  if (int rc = upload_ltab(c)) {
+    if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
+    (void)hipFree(c->d_boys);
+    delete c;
+    return rc;
+  }
   c->ppl = 1;
   c->target_wg = 8 * c->n_cu;
   *out = c;
@@ -147,6 +205,7 @@ extern "C" int fe_finalize(fe_ctx* c) {
   for (int s = 0; s < FE_WS_SLOTS; ++s)
     if (c->ws[s]) (void)hipFree(c->ws[s]);
   if (c->d_boys) (void)hipFree(c->d_boys);
+  if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
   delete c;
   return FE_OK;
 }
@@ -164,6 +223,9 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       return FE_OK;
     case 2:
       c->tile_screen = value != 0;
+      return FE_OK;
+    case 3:
+      c->sort_points = value != 0;
       return FE_OK;
     default:
       return fe_set_error(FE_ERR_ARG, "unknown tuning key %d", key);
@@ -194,6 +256,32 @@ int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec) {
   return FE_OK;
 }
 
+int fe_table_index(fe_table* t) {
+  std::vector<std::tuple<int32_t, int, int>> order;
+  order.reserve(t->n_pairs);
+  for (int L = 0; L <= FE_LMAX; ++L)
+    for (int64_t k = 0; k < t->n_L[L]; ++k) order.emplace_back(t->orig[L][k], L, (int)k);
+  std::sort(order.begin(), order.end());
+  std::vector<std::vector<int>> off(FE_LMAX + 1);
+  for (int L = 0; L <= FE_LMAX; ++L) off[L].resize(t->n_L[L]);
+  int32_t toff = 0;
+  for (auto& e : order) {
+    off[std::get<1>(e)][std::get<2>(e)] = toff;
+    toff += fe_host::NTERM[std::get<1>(e)];
+  }
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    if (!t->n_L[L]) continue;
+    if (hipMalloc(&t->d_flatoff[L], sizeof(int) * t->n_L[L]) != hipSuccess) {
+      (void)hipGetLastError();
+      return fe_set_error(FE_ERR_ALLOC, "table: index allocation failed");
+    }
+    FE_HIPCHK(hipMemcpy(t->d_flatoff[L], off[L].data(), sizeof(int) * t->n_L[L],
+                        hipMemcpyHostToDevice));
+    t->dev_bytes += sizeof(int) * t->n_L[L];
+  }
+  return FE_OK;
+}
+
 int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz) {
   t->n_atoms = n_atoms;
   if (n_atoms <= 0) return FE_OK;
@@ -204,6 +292,7 @@ int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* 
     a[4 * i + 2] = xyz[3 * i + 2];
     a[4 * i + 3] = Z[i];
   }
+  t->h_atoms = a;
   if (hipMalloc(&t->d_atoms, a.size() * sizeof(double)) != hipSuccess) {
     (void)hipGetLastError();
     return fe_set_error(FE_ERR_ALLOC, "table: atom allocation failed");
@@ -281,6 +370,11 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
     return rc;
   }
   for (int L = 0; L <= FE_LMAX; ++L) t->orig[L] = std::move(orig[L]);
+  rc = fe_table_index(t);
+  if (rc != FE_OK) {
+    fe_table_free(t);
+    return rc;
+  }
   *out = t;
   return FE_OK;
 }
@@ -288,9 +382,14 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
 extern "C" int fe_table_free(fe_table* t) {
   if (!t) return fe_set_error(FE_ERR_ARG, "fe_table_free: null table");
   (void)hipSetDevice(t->ctx->device);
-  for (int L = 0; L <= FE_LMAX; ++L)
+  for (int L = 0; L <= FE_LMAX; ++L) {
     if (t->d_rec[L]) (void)hipFree(t->d_rec[L]);
+    if (t->d_rec32[L]) (void)hipFree(t->d_rec32[L]);
+    if (t->d_slot2kept[L]) (void)hipFree(t->d_slot2kept[L]);
+    if (t->d_flatoff[L]) (void)hipFree(t->d_flatoff[L]);
+  }
   if (t->d_atoms) (void)hipFree(t->d_atoms);
+  if (t->d_basis_mem) (void)hipFree(t->d_basis_mem);
   delete t;
   return FE_OK;
 }
@@ -305,6 +404,61 @@ extern "C" int fe_table_info(const fe_table* t, int64_t* info, int n_info) {
   v[10] = (int64_t)t->dev_bytes;
   v[11] = t->n_candidates;
   for (int i = 0; i < n_info && i < 16; ++i) info[i] = v[i];
+  return FE_OK;
+}
+
+extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx, double* pp_cy,
+                               double* pp_cz, double* pp_K, int32_t* pp_nt, int32_t* pp_off,
+                               int32_t* pp_nmono, int32_t* pp_monooff, int32_t* mo_parent,
+                               int32_t* mo_axis, int32_t* tm_aidx, int32_t* tm_m, double* tm_g,
+                               double* pp_kbound, double* pp_reff) {
+  if (!t || !pp_exp || !pp_cx || !pp_cy || !pp_cz || !pp_K || !pp_nt || !pp_off || !pp_nmono ||
+      !pp_monooff || !mo_parent || !mo_axis || !tm_aidx || !tm_m || !tm_g)
+    return fe_set_error(FE_ERR_ARG, "fe_table_export: null argument");
+  FE_HIPCHK(hipSetDevice(t->ctx->device));
+  // (orig position, class, index within class), sorted by original position
+  std::vector<std::tuple<int32_t, int, int>> order;
+  order.reserve(t->n_pairs);
+  std::vector<std::vector<double>> rec(FE_LMAX + 1);
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    if (!t->n_L[L]) continue;
+    rec[L].resize((size_t)t->n_L[L] * fe_host::STRIDE[L]);
+    FE_HIPCHK(hipMemcpy(rec[L].data(), t->d_rec[L], rec[L].size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < t->n_L[L]; ++k) order.emplace_back(t->orig[L][k], L, (int)k);
+  }
+  std::sort(order.begin(), order.end());
+  int32_t toff = 0, moff = 0;
+  for (size_t i = 0; i < order.size(); ++i) {
+    const int L = std::get<1>(order[i]);
+    const int k = std::get<2>(order[i]);
+    const double* r = &rec[L][(size_t)k * fe_host::STRIDE[L]];
+    pp_cx[i] = r[0];
+    pp_cy[i] = r[1];
+    pp_cz[i] = r[2];
+    pp_exp[i] = r[3];
+    pp_K[i] = r[4];
+    if (pp_kbound) pp_kbound[i] = r[5];
+    if (pp_reff) pp_reff[i] = r[6];
+    const int nt = fe_host::NTERM[L], nm = fe_host::NMONO[L];
+    pp_nt[i] = nt;
+    pp_off[i] = toff;
+    pp_nmono[i] = nm;
+    pp_monooff[i] = moff;
+    for (int j = 0; j < nt; ++j) {
+      const int* key = fe_host::KEYS[fe_host::KEY_OFF[L] + j];
+      tm_aidx[toff + j] = key[4];
+      tm_m[toff + j] = key[3];
+      tm_g[toff + j] = r[8 + j];
+    }
+    for (int j = 0; j < nm; ++j) {
+      const int* m = fe_host::MONOS[fe_host::MONO_OFF[L] + j];
+      mo_parent[moff + j] = m[3];
+      mo_axis[moff + j] = m[4];
+    }
+    toff += nt;
+    moff += nm;
+  }
   return FE_OK;
 }
 
@@ -362,9 +516,11 @@ __global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ p
                                                     const double* __restrict__ pts, const int n_pts,
                                                     const double* __restrict__ atoms,
                                                     const int n_atoms, const int parts,
+                                                    const int* __restrict__ perm,
                                                     double* __restrict__ V) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_pts) return;
+  const int o = perm ? perm[i] : i;
   double e = 0.0;
   for (int r = 0; r < n_rows; ++r) e += partial[(size_t)r * ld + i];
   double nuc = 0.0;
@@ -375,7 +531,7 @@ __global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ p
       nuc += atoms[4 * a + 3] / sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
     }
   }
-  V[i] = (parts & FE_PART_ELECTRON) ? nuc - e : nuc;
+  V[o] = (parts & FE_PART_ELECTRON) ? nuc - e : nuc;
 }
 
 template <int M>
@@ -479,6 +635,11 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     if (rc) return rc;
     d_V = (double*)c->ws[FE_WS_OUT];
   }
+  const int32_t* perm = nullptr;
+  if (c->sort_points && n >= 2 * FE_WG && (parts & FE_PART_ELECTRON)) {
+    rc = fe_sort_points(c, t, d_pts, (int)n, st, &d_pts, &perm);
+    if (rc) return rc;
+  }
   FwdPlan pl = fe_plan(t, n, c->ppl, c->target_wg);
   double* partial = nullptr;
   if ((parts & FE_PART_ELECTRON) && pl.rows > 0) {
@@ -497,7 +658,7 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
   }
   const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
   hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial,
-                     nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, d_V);
+                     nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, perm, d_V);
   FE_HIPCHK(hipGetLastError());
   if (ptr_kind == FE_PTR_HOST) {
     FE_HIPCHK(hipMemcpyAsync(V, d_V, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -34,18 +34,37 @@ enum {
   FE_WS_AUX1,
   FE_WS_AUX2,
   FE_WS_AUX3,
+  FE_WS_SORT,
   FE_WS_SLOTS
+};
+
+// device copies of the per-L tables of fe_lclass.inc (used by the table build
+// and the adjoint contraction)
+struct LTabDev {
+  const int* key_off;    // [FE_LMAX+2]
+  const int* key_m;      // Boys order of every key, KEYS order
+  const int* m0_rowptr;  // CSR rows = keys
+  const int* m0_tidx;    // simplex index of t
+  const double* m0_val;
+  const int* mono_off;   // [FE_LMAX+2]
+  const int* tlist;      // (tx, ty, tz) of every simplex entry, MONOS order
+  const int* m0t_colptr; // transpose: per simplex entry (MONOS order) its keys
+  const int* m0t_k;      // key index within the class
+  const double* m0t_val;
 };
 
 struct fe_ctx {
   int device = 0;
   int n_cu = 256;
   double* d_boys = nullptr;  // (FE_BOYS_MMAX+1) x NR x COLS
+  void* d_ltab_mem = nullptr;
+  LTabDev ltab{};
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
   int ppl = 1;          // points per lane
   int target_wg = 2048; // workgroups per class launch
   bool tile_screen = false;
+  bool sort_points = true;  // Morton-order the points before evaluation
 };
 
 struct fe_table {
@@ -59,7 +78,26 @@ struct fe_table {
   int64_t n_L[FE_LMAX + 1] = {};
   std::vector<int32_t> orig[FE_LMAX + 1];  // flat-table index of every record
   double* d_atoms = nullptr;          // (x, y, z, Z) per atom
+  std::vector<double> h_atoms;        // host copy of d_atoms
   size_t dev_bytes = 0;
+  double sum_rule = 0.0;              // sum over candidates of K (pi/p)^1.5 etab_0
+  // --- present when built on the device from a basis (fe_table_build) ---
+  bool has_basis = false;
+  int n_func = 0;
+  int n_shells = 0;
+  int n_sp = 0;
+  void* d_basis_mem = nullptr;        // shells, primitives, components, shell pairs
+  struct {
+    const int* atom; const int* l; const int* fn0; const int* nfn; const int* prim0;
+    const int* nprim; const double* c; const double* pexp; const double* pcoef;
+    const int* fn_lx; const int* fn_ly; const int* fn_lz;
+    const int* sp_i; const int* sp_j; const int* sp_slot;
+  } b{};
+  int* d_flatoff[FE_LMAX + 1] = {};   // term offset of every record in the flat layout
+  int64_t n_cand_L[FE_LMAX + 1] = {};
+  int* d_slot2kept[FE_LMAX + 1] = {}; // candidate slot -> compact record (-1 dropped)
+  std::vector<int> h_sp_i, h_sp_j;    // shell pair list (host)
+  std::vector<int64_t> h_sp_slot;
 };
 
 struct FwdPlan {
@@ -72,8 +110,13 @@ struct FwdPlan {
 };
 
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);
+int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz);
+int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec);
+int fe_table_index(fe_table* t);  // flat term offsets of the records (from orig order)
 FwdPlan fe_plan(const fe_table* t, int64_t n, int ppl, int target_wg);
 int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipStream_t st,
                     const double** d_pts);
+int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
+                   const double** d_sorted, const int32_t** d_perm);
 int fe_launch_fwd_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n,
                         double* partial, hipStream_t st);

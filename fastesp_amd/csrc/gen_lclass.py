@@ -255,6 +255,40 @@ def main():
     host.append(f"static const int MONOS[{len(allmono)}][5] = {{")
     host.extend("  {" + ", ".join(map(str, k)) + "}," for k in allmono)
     host.append("};")
+    # sparse map etab_t -> G_k per class: rows in KEYS order, t as an index into
+    # the class's simplex list (the MONOS order: all t with |t| <= L)
+    rowptr, tidx, vals = [0], [], []
+    for L in range(lmax + 1):
+        keys, acc = keys_of(L)
+        order = schedule_of(L)[0]
+        T = L + 1
+        pos = {((tx * T + ty) * T + tz): i for i, (tx, ty, tz) in enumerate(order)}
+        for k in keys:
+            for col, v in sorted(acc[k].items()):
+                if v != 0.0:
+                    tidx.append(pos[col])
+                    vals.append(v)
+            rowptr.append(len(tidx))
+    # transpose (per simplex t: the keys k it feeds), for the adjoint dG -> d etab
+    colptr, kidx, tvals = [0], [], []
+    for L in range(lmax + 1):
+        keys, acc = keys_of(L)
+        order = schedule_of(L)[0]
+        T = L + 1
+        for (tx, ty, tz) in order:
+            col = (tx * T + ty) * T + tz
+            for k, key in enumerate(keys):
+                v = acc[key].get(col, 0.0)
+                if v != 0.0:
+                    kidx.append(k)
+                    tvals.append(v)
+            colptr.append(len(kidx))
+    host.append(f"static const int M0T_COLPTR[{len(colptr)}] = {{{', '.join(map(str, colptr))}}};")
+    host.append(f"static const int M0T_K[{len(kidx)}] = {{{', '.join(map(str, kidx))}}};")
+    host.append(f"static const double M0T_VAL[{len(tvals)}] = {{{', '.join(repr(v) for v in tvals)}}};")
+    host.append(f"static const int M0_ROWPTR[{len(rowptr)}] = {{{', '.join(map(str, rowptr))}}};")
+    host.append(f"static const int M0_TIDX[{len(tidx)}] = {{{', '.join(map(str, tidx))}}};")
+    host.append(f"static const double M0_VAL[{len(vals)}] = {{{', '.join(repr(v) for v in vals)}}};")
     host.append("}  // namespace fe_host")
     out.extend(host)
     dst = Path(__file__).with_name("fe_lclass.inc")

@@ -1,0 +1,59 @@
+"""Algorithmic work model of the ESP path (the single definition used for the
+roofline figures).
+
+Forward, per evaluated (point, pair) of class L:
+    F_fwd(L) = 9 (R, T) + B(L) + (nmono(L) - 1) + 3 nterm(L) + 2,
+    B(M) = 20 + 3M  (one exp/Taylor seed counted as 20, 3 per recursion step)
+    -> 34, 49, 79, 131, 215, 341, 522 for L = 0..6;  nuclear: 11 per (point, atom).
+Backward (field + dL/dG), per (point, pair):
+    F_bwd(L) = 19 + B(L+1) + (nmono(L) - 1) + 14 nterm(L);  nuclear: 17 per (point, atom).
+Mixed precision: (nmono - 1) + 3 nterm of the forward (and of the backward
+contraction) are FP32, the rest FP64.
+
+Bytes per pair record of the device table: 8 * STRIDE(L) (header + g).
+"""
+from __future__ import annotations
+
+NTERM = [1, 4, 11, 24, 46, 80, 130]
+NMONO = [1, 4, 10, 20, 35, 56, 84]
+STRIDE = [16, 16, 24, 32, 56, 88, 144]
+
+PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
+PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
+PEAK_HBM = 8.0e12          # B/s
+
+
+def boys_cost(M: int) -> int:
+    return 20 + 3 * M
+
+
+def f_fwd(L: int) -> int:
+    return 9 + boys_cost(L) + (NMONO[L] - 1) + 3 * NTERM[L] + 2
+
+
+def f_bwd(L: int) -> int:
+    return 19 + boys_cost(L + 1) + (NMONO[L] - 1) + 14 * NTERM[L]
+
+
+def forward_flops(pairs_per_L, n_points: int, n_atoms: int) -> float:
+    return float(n_points) * (sum(n * f_fwd(L) for L, n in enumerate(pairs_per_L))
+                              + 11 * n_atoms)
+
+
+def backward_flops(pairs_per_L, n_points: int, n_atoms: int) -> float:
+    return float(n_points) * (sum(n * f_bwd(L) for L, n in enumerate(pairs_per_L))
+                              + 17 * n_atoms)
+
+
+def fp32_share_fwd(pairs_per_L) -> float:
+    """Fraction of the forward pair work that the mixed path does in FP32."""
+    tot = sum(n * f_fwd(L) for L, n in enumerate(pairs_per_L))
+    f32 = sum(n * ((NMONO[L] - 1) + 3 * NTERM[L]) for L, n in enumerate(pairs_per_L))
+    return f32 / tot if tot else 0.0
+
+
+def table_bytes(pairs_per_L) -> int:
+    return sum(8 * STRIDE[L] * n for L, n in enumerate(pairs_per_L))
+
+
+assert [f_fwd(L) for L in range(7)] == [34, 49, 79, 131, 215, 341, 522]

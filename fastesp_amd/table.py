@@ -72,9 +72,19 @@ class PairTable:
         return cls.from_flat(read_pairs(path))
 
     def write(self, path) -> None:
+        """pairs.bin in the reference layout (``fastesp/table.py:119-141``)."""
+        write_pairs(path, self.to_flat())
+
+    def to_flat(self) -> FlatTable:
         if self._flat is None:
-            raise ValueError("this table has no host copy to write")
-        write_pairs(path, self._flat)
+            from .builder import export_flat
+            self._flat = export_flat(self)
+        return self._flat
+
+    def stats(self) -> dict:
+        v = (C.c_double * 4)()
+        _lib.check(self._ctx.lib.fe_table_stats(self._h, v, 4), "fe_table_stats")
+        return dict(sum_rule=v[0], n_pairs=int(v[1]), n_func=int(v[2]), n_shell_pairs=int(v[3]))
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -85,6 +85,47 @@ int fe_table_from_pairs(fe_ctx* ctx, int n_pairs, int n_terms,
                         const int32_t* tm_m, const double* tm_g, int n_atoms,
                         const double* Z, const double* xyz, fe_table** out);
 
+/* Build the pair table on the device from a Cartesian basis, a density
+ * matrix and the ESP points (used only for the screen).  Arguments are the
+ * basis.bin fields of fastesp/table.py:268-316 (one entry per Cartesian
+ * component, primitives duplicated per component, components of a shell
+ * contiguous and in cart_order).  Replaces `build_pairs <basis.bin> <pairs.bin>
+ * [tol]` (fastesp/native/build_pairs.cpp:162-425) and the NumPy builder
+ * ESPEngine + build_table (fastesp/esp.py:168-257, fastesp/table.py:176-259).
+ * rule 0: the NumPy screen, keep = !(K bound / max(dmin, 0.05) < tol)
+ *         (fastesp/table.py:190-194);
+ * rule 1: the C++ screen, keep = !(K bound / max(dmin - reff, 0.05) < tol) and
+ *         G not all zero and c_i c_j != 0 (fastesp/native/build_pairs.cpp:
+ *         243,309,318-328).
+ * `ptr_kind` applies to dens (n_func x n_func, row-major) and pts (n_points x 3);
+ * the basis arrays are host arrays.  Shells up to f (L = l_A + l_B <= 6). */
+int fe_table_build(fe_ctx* ctx, int n_func, int n_prim, const int32_t* fn_atom,
+                   const int32_t* fn_nprim, const int32_t* fn_off,
+                   const int32_t* fn_lx, const int32_t* fn_ly, const int32_t* fn_lz,
+                   const int32_t* fn_shell, const double* fc, const double* pexp,
+                   const double* pcoef, const double* dens, int n_points,
+                   const double* pts, int n_atoms, const double* Z,
+                   const double* xyz, double tol, int rule, int ptr_kind,
+                   void* stream, fe_table** out);
+
+/* Floating-point statistics of a table: out[0] the sum rule
+ * sum_pairs K (pi/p)^{3/2} etab_0 = Tr(P S) over all candidate primitive
+ * pairs (printed by build_pairs.cpp:423), out[1] kept pairs, out[2] basis
+ * functions, out[3] shell pairs (zeros for uploaded tables). */
+int fe_table_stats(const fe_table* t, double* out, int n);
+
+/* Copy a table back to the host in the flat pairs.bin layout
+ * (fastesp/table.py:11-24), pairs in reference order (shell pairs si <= sj,
+ * primitives i-major; fastesp/esp.py:168-193, fastesp/table.py:223-259).
+ * Array sizes: n_pairs, n_terms (fe_table_info) and n_mono = sum of pp_nmono
+ * (= sum over pairs of the class monomial count).  pp_kbound (K * bound) and
+ * pp_reff may be NULL.  Replaces the pairs.bin writer of build_pairs.cpp:386-421. */
+int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx, double* pp_cy,
+                    double* pp_cz, double* pp_K, int32_t* pp_nt, int32_t* pp_off,
+                    int32_t* pp_nmono, int32_t* pp_monooff, int32_t* mo_parent,
+                    int32_t* mo_axis, int32_t* tm_aidx, int32_t* tm_m, double* tm_g,
+                    double* pp_kbound, double* pp_reff);
+
 /* Release a table and its device memory. */
 int fe_table_free(fe_table* t);
 
@@ -100,6 +141,21 @@ int fe_table_info(const fe_table* t, int64_t* info, int n_info);
  * `precision` is FE_PREC_FP64 or FE_PREC_MIXED. */
 int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int precision,
                    int parts, int ptr_kind, void* stream, double* V);
+
+/* V and the field dV/dC (n x 3, f64) in one pass.  No counterpart in the
+ * reference (it has no gradients, fastesp/table.py:73-104 is value-only);
+ * V is the same quantity as fe_esp_forward(parts = FE_PART_ESP). */
+int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, int precision,
+                        int ptr_kind, void* stream, double* V, double* dVdC);
+
+/* Adjoint of V for a loss with dL/dV = gV (n values): dL/dG_k =
+ * -sum_i gV_i K R_i^a F_m (n_terms values, aligned with the exported tm_g,
+ * NULL to skip) and dL/dP (n_func x n_func, both triangles filled because the
+ * table folds P_ij + P_ji^T, fastesp/esp.py:185-187; NULL to skip; needs a
+ * table from fe_table_build).  No counterpart in the reference. */
+int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const double* gV,
+                    int precision, int ptr_kind, void* stream, double* dLdG,
+                    double* dLdP);
 
 /* Boys functions F_0..F_M(T) for n arguments with the device algorithm
  * (F is (n, M+1) f64, row-major).  Replaces boys_all

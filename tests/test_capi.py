@@ -1,0 +1,63 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/fastesp_hip.h (no compute calls, runs without a GPU)."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from fastesp_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "fastesp_hip.h"
+
+
+def declared():
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fe_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_the_api():
+    names = declared()
+    for must in ("fe_init", "fe_finalize", "fe_last_error", "fe_table_build", "fe_table_from_pairs",
+                 "fe_table_export", "fe_table_free", "fe_esp_forward", "fe_esp_forward_grad",
+                 "fe_esp_backward", "fe_boys_eval"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert _lib.LIB_PATH.exists(), "build the library first (python -m fastesp_amd.build)"
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r"\bT (fe_[a-z0-9_]+)$", out, flags=re.M))
+    assert set(declared()) <= exported
+
+
+def test_python_binding_covers_the_header():
+    assert set(declared()) <= set(_lib.SIGNATURES)
+
+
+def test_errors_are_reported_not_fatal():
+    lib = _lib.load()
+    assert lib.fe_version() >= 100
+    # null arguments come back as FE_ERR_ARG with a message, never a crash
+    assert lib.fe_init(0, None) == -1
+    assert b"null" in lib.fe_last_error()
+    assert lib.fe_table_free(None) == -1
+    assert lib.fe_esp_forward(None, None, 0, 0, 3, 0, None, None) == -1
+
+
+def test_no_device_is_a_clean_error():
+    from tests.conftest import have_gpu
+    if have_gpu():
+        pytest.skip("a device is present")
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.fe_init(0, ctypes.byref(h))
+    assert rc in (-1, -2)
+    assert lib.fe_last_error()

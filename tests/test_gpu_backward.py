@@ -1,0 +1,115 @@
+"""GPU parity of the adjoint: field dV/dC, dL/dG and dL/dP.
+
+Pinned against the oracle's analytic adjoint (itself pinned by central finite
+differences in tests/test_oracle_table.py), against central finite
+differences of the GPU forward, and at full size through identities that do
+not depend on the size: V is linear and homogeneous in P (and in G), so
+<dL/dP, P> = <dL/dG, G> = sum_i gV_i (V_i - nuc_i) exactly.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fastesp_amd import synth
+from fastesp_amd.autograd import ESPModel, esp_backward, esp_with_grad
+from fastesp_amd.builder import build_table_hip, export_flat
+from oracle import engine, evaluate
+from tests.helpers import cf_toy
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def test_field_and_dP_match_oracle_on_toy():
+    g = np.load(GOLD / "grad_toy.npz")
+    wfn, basis, P, _ = cf_toy()
+    pts = g["pts"]
+    say("toy forward_grad")
+    t = build_table_hip(wfn, basis, P, pts, tol=0.0)
+    V, G = esp_with_grad(t, pts)
+    assert np.abs(V - g["V"]).max() < 1e-10
+    assert np.abs(G - g["field"]).max() < 1e-9
+    say("toy backward")
+    dG, dP = esp_backward(t, pts, g["gV"], n_func=basis.n)
+    assert np.abs(dP - g["dP"]).max() < 1e-9 * max(1.0, np.abs(g["dP"]).max())
+    np.testing.assert_allclose(dP, dP.T, rtol=0, atol=1e-13)
+    # dL/dG against the oracle on the same (exported) table
+    tab = engine.build_table(wfn, basis, P, pts, tol=0.0)
+    dG_ref = evaluate.grad_G(tab, pts, g["gV"])
+    assert np.abs(dG - dG_ref).max() < 1e-9 * max(1.0, np.abs(dG_ref).max())
+
+
+def test_field_matches_finite_differences_of_gpu_forward():
+    w = synth.config0(200)
+    say("cfg0 field vs FD")
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    V, G = esp_with_grad(t, w.points)
+    np.testing.assert_allclose(V, t.esp(), rtol=0, atol=1e-12)
+    h = 1e-4
+    for ax in range(3):
+        d = np.zeros(3)
+        d[ax] = h
+        fd = (t.esp(w.points + d) - t.esp(w.points - d)) / (2 * h)
+        assert np.abs(G[:, ax] - fd).max() < 2e-7 * max(1.0, np.abs(fd).max())
+
+
+def test_autograd_function_matches_oracle_gradients():
+    import torch
+    g = np.load(GOLD / "grad_toy.npz")
+    wfn, basis, P, _ = cf_toy()
+    dev = torch.device("cuda:0")
+    pts = torch.tensor(g["pts"], dtype=torch.float64, device=dev, requires_grad=True)
+    dens = torch.tensor(P, dtype=torch.float64, device=dev, requires_grad=True)
+    model = ESPModel(wfn, basis, tol=0.0)
+    say("autograd toy")
+    V = model(pts, dens)
+    w = torch.tensor(g["w"], device=dev)
+    target = torch.tensor(g["target"], device=dev)
+    loss = (w * (V - target) ** 2).sum()
+    loss.backward()
+    assert np.abs(V.detach().cpu().numpy() - g["V"]).max() < 1e-10
+    gpts = pts.grad.cpu().numpy()
+    np.testing.assert_allclose(gpts, g["gV"][:, None] * g["field"], rtol=1e-8, atol=1e-12)
+    assert np.abs(dens.grad.cpu().numpy() - g["dP"]).max() < 1e-9 * max(1.0, np.abs(g["dP"]).max())
+
+
+def test_homogeneity_identities_at_full_size():
+    """<dL/dP, P> = <dL/dG, G> = sum gV (V - nuc) at 30000 points."""
+    w = synth.config1(30000)
+    rng = np.random.default_rng(5)
+    say("cfg1 30000 build")
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    V = t.esp()
+    nuc = t.nuclear()
+    gV = rng.uniform(0.5, 1.0, len(V)) * 2.0 * rng.normal(0, 1e-3, len(V))
+    say("cfg1 30000 backward")
+    dG, dP = esp_backward(t, w.points, gV, n_func=w.basis.n)
+    ref = float((gV * (V - nuc)).sum())
+    flat = export_flat(t)
+    lhs_G = float((dG * flat.tm_g).sum())
+    lhs_P = float((dP * w.density).sum())
+    assert lhs_G == pytest.approx(ref, rel=1e-9, abs=1e-12)
+    assert lhs_P == pytest.approx(ref, rel=1e-9, abs=1e-12)
+    np.testing.assert_allclose(dP, dP.T, rtol=0, atol=1e-12 * np.abs(dP).max())
+
+
+@pytest.mark.skipif(not (GOLD / "cfg2_sub.npz").exists(), reason="cfg2_sub fixture not generated")
+def test_config2_subset_matches_oracle():
+    g = np.load(GOLD / "cfg2_sub.npz")
+    w = synth.config1(30000)
+    assert str(g["hash"]) == w.input_hash()
+    pts = w.points[g["idx"]]
+    say("cfg2 subset build")
+    t = build_table_hip(w.wfn, w.basis, w.density, pts, tol=1e-12)
+    assert t.n_pairs == int(g["npairs"])
+    V, G = esp_with_grad(t, pts)
+    assert np.abs(V - g["V"]).max() < 1e-10
+    assert np.abs(G[:64] - g["field64"]).max() < 1e-9
+    say("cfg2 subset backward")
+    _, dP = esp_backward(t, pts, g["gV"], want_dG=False, n_func=w.basis.n)
+    assert np.abs(dP - g["dP"]).max() < 1e-9 * max(1.0, np.abs(g["dP"]).max())

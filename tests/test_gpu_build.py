@@ -1,0 +1,113 @@
+"""GPU parity of the device table build (fe_table_build) against the oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fastesp_amd import _lib, synth
+from fastesp_amd.builder import SCREEN_CPP, build_table_hip, export_flat
+from fastesp_amd.formats import basis_payload, read_pairs, write_basis_payload
+from fastesp_amd.table import PairTable, build_table
+from oracle import engine
+from tests.helpers import cf_toy, oh_toy
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+REF_BUILDER = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / "build_pairs"
+TOL_FP64 = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def _compare_tables(flat, tab, rtol=1e-13):
+    """Same pairs in the same (reference) order with the same p, P, K, G."""
+    assert flat.n_pairs == tab.n_pairs and flat.n_terms == tab.n_terms
+    np.testing.assert_allclose(flat.pp_exp, tab.pp_exp, rtol=1e-15)
+    for a, b in ((flat.pp_cx, tab.pp_cx), (flat.pp_cy, tab.pp_cy), (flat.pp_cz, tab.pp_cz)):
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(flat.pp_K, tab.pp_K, rtol=1e-13)
+    assert (flat.pp_nt == tab.pp_nt).all() and (flat.pp_off == tab.pp_off).all()
+    assert (flat.tm_aidx == tab.tm_aidx).all() and (flat.tm_m == tab.tm_m).all()
+    assert (flat.mo_parent == tab.mo_parent).all() and (flat.mo_axis == tab.mo_axis).all()
+    # G relative to the largest |G| of its pair
+    scale = np.repeat(np.array([np.abs(tab.tm_g[o:o + n]).max() if n else 1.0
+                                for o, n in zip(tab.pp_off, tab.pp_nt)]), tab.pp_nt)
+    assert (np.abs(flat.tm_g - tab.tm_g) / np.maximum(scale, 1e-300)).max() < rtol
+
+
+@pytest.mark.parametrize("mk", [oh_toy, cf_toy])
+def test_build_toys_match_oracle(mk):
+    wfn, basis, P, _ = mk()
+    rng = np.random.default_rng(8)
+    pts = rng.normal(0, 3.0, (300, 3))
+    say(f"toy build {mk.__name__}")
+    t = build_table_hip(wfn, basis, P, pts, tol=0.0)
+    tab = engine.build_table(wfn, basis, P, pts, tol=0.0)
+    _compare_tables(export_flat(t), tab)
+    assert t.stats()["sum_rule"] == pytest.approx(tab.sum_rule, abs=1e-11)
+
+
+def test_build_cfg0_matches_oracle_and_golden():
+    w = synth.config0()
+    g = np.load(GOLD / "cfg0.npz")
+    prims = engine.build_prim_pairs(w.basis, w.density)
+    for key, tol in (("V_tol0", 0.0), ("V_tol12", 1e-12)):
+        say(f"cfg0 build tol={tol}")
+        t = build_table(w.wfn, w.basis, w.density, w.points, tol=tol)
+        tab = engine.build_table(w.wfn, w.basis, w.density, w.points, tol, prims=prims)
+        _compare_tables(export_flat(t), tab)
+        assert t.n_pairs == int(g["npairs_" + key[2:]])
+        V = t.esp()
+        assert np.abs(V - g[key]).max() < TOL_FP64
+    assert t.stats()["sum_rule"] == pytest.approx(w.wfn.nelec, abs=1e-9)
+
+
+def test_build_cfg1_300_matches_golden():
+    w = synth.config1(300)
+    g = np.load(GOLD / "cfg1_300.npz")
+    assert str(g["hash"]) == w.input_hash()
+    for key in ("V_tol0", "V_tol12"):
+        tol = 0.0 if key == "V_tol0" else 1e-12
+        say(f"cfg1 300 build tol={tol}")
+        t = build_table(w.wfn, w.basis, w.density, w.points, tol=tol)
+        assert t.n_pairs == int(g["npairs_" + key[2:]])
+        assert np.abs(t.esp() - g[key]).max() < TOL_FP64
+    assert t.stats()["sum_rule"] == pytest.approx(float(g["sum_rule"]), abs=1e-9)
+
+
+@pytest.mark.skipif(not REF_BUILDER.exists(), reason="reference builder not compiled")
+def test_cpp_screen_rule_matches_reference_builder(tmp_path):
+    """Rule 1 reproduces build_pairs' kept set at a tolerance that drops pairs."""
+    w = synth.config0()
+    b, p = tmp_path / "b.bin", tmp_path / "p.bin"
+    write_basis_payload(b, basis_payload(w.wfn, w.basis, w.density, w.points))
+    r = subprocess.run([str(REF_BUILDER), str(b), str(p), "1e-9"], capture_output=True, text=True)
+    assert r.returncode == 0
+    ref = read_pairs(p)
+    say("cfg0 build rule=cpp tol=1e-9")
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-9, rule=SCREEN_CPP)
+    flat = export_flat(t)
+    assert flat.n_pairs == ref.n_pairs
+    key = lambda f, i: (round(f.pp_exp[i], 10), round(f.pp_cx[i], 8), round(f.pp_cy[i], 8),  # noqa
+                        round(f.pp_cz[i], 8))
+    assert sorted(key(flat, i) for i in range(flat.n_pairs)) == sorted(
+        key(ref, i) for i in range(ref.n_pairs))
+
+
+def test_written_pairs_bin_roundtrip(tmp_path):
+    wfn, basis, P, _ = cf_toy()
+    pts = np.random.default_rng(1).normal(0, 3, (50, 3))
+    say("write/read pairs.bin")
+    t = build_table_hip(wfn, basis, P, pts, tol=0.0)
+    t.write(tmp_path / "pairs.bin")
+    t2 = PairTable.read(tmp_path / "pairs.bin")
+    assert np.array_equal(t.esp(), t2.esp())
+
+
+def test_build_rejects_bad_input():
+    wfn, basis, P, _ = oh_toy()
+    with pytest.raises(_lib.HipBackendError):
+        build_table_hip(wfn, basis, P, np.zeros((1, 3)), tol=-1.0)
