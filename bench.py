@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=30000)
-    ap.add_argument("--mode", choices=["fwd", "fwdbwd"], default="fwd")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--mode", choices=["fwd", "fwdbwd"], default="fwdbwd")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0 = library default)")
@@ -116,81 +116,120 @@ def main() -> int:
     log(f"table: {info['n_pairs']} pairs, per L {info['pairs_per_L']}, build {t_build*1e3:.1f} ms")
 
     dev = torch.device("cuda:0")
+    n = a.points
     pts = torch.from_numpy(np.ascontiguousarray(w.points)).to(dev)
-    V = torch.empty(a.points, dtype=torch.float64, device=dev)
+    V = torch.empty(n, dtype=torch.float64, device=dev)
+    dVdC = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    dLdP = torch.empty((w.basis.n, w.basis.n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
-    prec = _lib.FE_PREC_MIXED if a.precision == "mixed" else _lib.FE_PREC_FP64
     lib = ctx.lib
+    PREC = {"fp64": _lib.FE_PREC_FP64, "mixed": _lib.FE_PREC_MIXED}
+    # config-2 loss: L = sum w (V - T)^2, w ~ U[0.5, 1], T = V_fp64 + N(0, 1e-3)
+    rng = np.random.default_rng(2002)
+    wts = torch.from_numpy(rng.uniform(0.5, 1.0, n)).to(dev)
+    _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), n, PREC["fp64"], _lib.FE_PART_ESP,
+                                  _lib.FE_PTR_DEVICE, sh, V.data_ptr()), "fe_esp_forward")
+    target = V.clone() + torch.from_numpy(rng.normal(0, 1e-3, n)).to(dev)
 
-    def fwd():
-        _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), a.points, prec, _lib.FE_PART_ESP,
-                                      _lib.FE_PTR_DEVICE, sh, V.data_ptr()), "fe_esp_forward")
+    def make_step(mode, prec):
+        p = PREC[prec]
 
-    step = fwd
-    if a.mode == "fwdbwd":
-        raise SystemExit("fwdbwd mode needs the backward kernels")
+        def fwd():
+            _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), n, p, _lib.FE_PART_ESP,
+                                          _lib.FE_PTR_DEVICE, sh, V.data_ptr()), "fe_esp_forward")
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(a.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / a.steps
-    log(f"{a.steps} steps: {ms_step:.3f} ms/step (HIP events), wall {wall*1e3/a.steps:.3f} ms/step")
+        def fwdbwd():
+            _lib.check(lib.fe_esp_forward_grad(tab._h, pts.data_ptr(), n, p, _lib.FE_PTR_DEVICE,
+                                               sh, V.data_ptr(), dVdC.data_ptr()),
+                       "fe_esp_forward_grad")
+            gV = 2.0 * wts * (V - target)
+            _lib.check(lib.fe_esp_backward(tab._h, pts.data_ptr(), n, gV.data_ptr(), p,
+                                           _lib.FE_PTR_DEVICE, sh, None, dLdP.data_ptr()),
+                       "fe_esp_backward")
+        return fwd if mode == "fwd" else fwdbwd
 
-    # --- roofline of the evaluation kernels (FP64 VALU) --------------------
-    fl = flops.forward_flops(info["pairs_per_L"], a.points, info["n_atoms"])
-    achieved = fl / (ms_step * 1e-3)
-    roof = dict(bound="valu", achieved=round(achieved / 1e12, 3),
-                peak=flops.PEAK_FP64_VALU / 1e12, unit="TFLOP/s",
-                frac=round(achieved / flops.PEAK_FP64_VALU, 4), traffic=None,
-                kernel="k_esp_fwd<L> x7 + k_esp_reduce (one fe_esp_forward)",
-                flops_per_launch=fl)
-
-    # --- accuracy against the FP64 oracle (golden, 256 sampled points) -----
-    acc = {}
     gold = ROOT / "tests" / "golden" / "cfg1_30k.npz"
-    if gold.exists() and a.points == 30000:
-        g = np.load(gold)
-        Vh = V.cpu().numpy()
-        d = Vh[g["idx"]] - g["V"]
-        acc = dict(max_abs_dV_au=float(np.abs(d).max()), rms_dV_au=float(np.sqrt((d * d).mean())),
-                   n_check=int(len(d)), oracle_pairs=int(g["npairs"]), gpu_pairs=info["n_pairs"])
+    g = np.load(gold) if (gold.exists() and n == 30000) else None
+
+    def timed(mode, prec):
+        step = make_step(mode, prec)
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.steps
+        res = dict(ms_per_step=round(ms, 4), ms_per_1e4_pts=round(ms / (n / 1e4), 4))
+        if g is not None:
+            d = V.cpu().numpy()[g["idx"]] - g["V"]
+            res["max_abs_dV_au"] = float(np.abs(d).max())
+            res["rms_dV_au"] = float(np.sqrt((d * d).mean()))
+        log(f"{mode} {prec}: {ms:.3f} ms/step = {res['ms_per_1e4_pts']:.3f} ms/1e4 pts, "
+            f"max |dV| {res.get('max_abs_dV_au', float('nan')):.2e}")
+        return res
+
+    legs = {}
+    for mode, prec in (("fwd", "fp64"), ("fwdbwd", "fp64"), ("fwdbwd", "mixed")):
+        legs[f"{mode}_{prec}"] = timed(mode, prec)
+    key = f"{a.mode}_{a.precision}"
+    if key not in legs:
+        legs[key] = timed(a.mode, a.precision)
+    head = legs[key]
+    ms_step = head["ms_per_step"]
+
+    # --- roofline of the step's kernels (VALU, algorithmic FLOPs) -----------
+    ppl = info["pairs_per_L"]
+    f_fwd = flops.forward_flops(ppl, n, info["n_atoms"])
+    f_bwd = flops.backward_flops(ppl, n, info["n_atoms"]) if a.mode == "fwdbwd" else 0.0
+    fl = f_fwd + f_bwd
+    achieved = fl / (ms_step * 1e-3)
+    if a.precision == "mixed":
+        f32 = flops.fp32_share_fwd(ppl) * fl
+        t_roof = (fl - f32) / flops.PEAK_FP64_VALU + f32 / flops.PEAK_FP32_VALU
+        peak = fl / t_roof
+    else:
+        peak = flops.PEAK_FP64_VALU
+    roof = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
+                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
+                kernel=("k_esp_fwd<L> x7 + k_esp_reduce" if a.mode == "fwd" else
+                        "k_esp_fwd_grad<L> x7 + k_esp_reduce_grad + k_dG<L> x7 + k_dG_reduce<L> "
+                        "x7 + k_dg_to_dp (one forward_grad + backward)"),
+                flops_per_step=fl, flops_fwd=f_fwd, flops_bwd=f_bwd)
 
     cpu = None
     if not a.no_cpu:
         log("CPU baseline (C++/OpenMP port)")
         cpu = cpu_baseline(w, a.cpu_seconds)
 
-    value = ms_step / (a.points / 1e4)
+    value = head["ms_per_1e4_pts"]
     out = {
         "metric": "ESP ms/1e4 pts (1634 fns) FP64 vs mixed, fwd+bwd; max err vs FP64 oracle",
-        "value": round(value, 4),
+        "value": value,
         "unit": "ms/1e4 pts",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": ms_step,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
-        "data": "synthetic (seeded config-1 molecule and vdW-surface points)",
+        "data": "synthetic (seeded config-1 molecule, density and vdW-surface points)",
         "config": {"workload": f"esp_{a.mode}_{a.precision}", "molecule": "config1 54 atoms",
-                   "n_functions": int(w.basis.n), "n_points": a.points,
-                   "n_pairs": info["pairs_per_L"], "tol": 1e-12, "parallelism": "single GPU"},
+                   "n_functions": int(w.basis.n), "n_points": n,
+                   "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU"},
+        "legs": legs,
+        "speedup_mixed_vs_fp64_fwdbwd": round(legs["fwdbwd_fp64"]["ms_per_step"]
+                                              / legs["fwdbwd_mixed"]["ms_per_step"], 3),
         "table_build_ms": round(t_build * 1e3, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
     return 0

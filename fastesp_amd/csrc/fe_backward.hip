@@ -180,10 +180,14 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * plane * sizeof(double));
   if (rc) return rc;
   double* partial = (double*)c->ws[FE_WS_PARTIAL];
-  if (c->ppl == 2)
+  if (precision == FE_PREC_MIXED) {
+    rc = fe_launch_fwd_grad_mixed(t, pl, d_pts, (int)n, partial, plane, st);
+    if (rc) return rc;
+  } else if (c->ppl == 2) {
     launch_fwd_grad_all<2>(t, pl, d_pts, (int)n, partial, plane, st);
-  else
+  } else {
     launch_fwd_grad_all<1>(t, pl, d_pts, (int)n, partial, plane, st);
+  }
   hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                      partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);

@@ -227,6 +227,10 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
     case 3:
       c->sort_points = value != 0;
       return FE_OK;
+    case 4:  // mixed-precision routing threshold, in units of 1e-12 a.u.
+      if (value < 0) return fe_set_error(FE_ERR_ARG, "mixed threshold must be >= 0");
+      c->mixed_tau = (double)value * 1e-12;
+      return FE_OK;
     default:
       return fe_set_error(FE_ERR_ARG, "unknown tuning key %d", key);
   }

@@ -65,6 +65,7 @@ struct fe_ctx {
   int target_wg = 2048; // workgroups per class launch
   bool tile_screen = false;
   bool sort_points = true;  // Morton-order the points before evaluation
+  double mixed_tau = 1e-4;  // FP32 routing threshold of the mixed path (a.u.)
 };
 
 struct fe_table {
@@ -120,3 +121,5 @@ int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hip
                    const double** d_sorted, const int32_t** d_perm);
 int fe_launch_fwd_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n,
                         double* partial, hipStream_t st);
+int fe_launch_fwd_grad_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                             double* partial, size_t plane, hipStream_t st);

@@ -213,6 +213,45 @@ def gen_L(L):
             expr = f"fma((double)a{m}, F[{m}], {expr})"
     o.append(f"  return {expr};")
     o.append("}")
+
+    # ---------------- mixed gradient: FP32 monomials/polynomials, FP64 F, p and sums
+    o.append(f"template <> __device__ __forceinline__ void fe_contract_grad_f32<{L}>("
+             "const float Rx, const float Ry, const float Rz, const double* F, "
+             "const float* __restrict__ g, const double p, const double Rxd, const double Ryd, "
+             "const double Rzd, double& I, double& Gx, double& Gy, double& Gz) {")
+    emit_monos(o, L, order, parent, axis, T="float")
+    lines = []
+    init = set()
+
+    def add32(var, coef_expr, mono_j, scale=1):
+        g = coef_expr if scale == 1 else f"({scale}.0f * {coef_expr})"
+        if var not in init:
+            init.add(var)
+            lines.append(f"  float {var} = " + (f"{g};" if mono_j == 0 else f"{g} * m{mono_j};"))
+        else:
+            lines.append(f"  {var} = " + (f"{var} + {g};" if mono_j == 0 else f"fmaf({g}, m{mono_j}, {var});"))
+
+    for k, (ax, ay, az, m) in enumerate(keys):
+        add32(f"a{m}", f"g[{k}]", aidx[k])
+        for d, al in enumerate((ax, ay, az)):
+            if al > 0:
+                lo = [ax, ay, az]
+                lo[d] -= 1
+                add32(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+    o.extend(lines)
+    i_expr, s_expr = "0.0", "0.0"
+    for m in ms:
+        i_expr = f"fma((double)a{m}, F[{m}], {i_expr})"
+        s_expr = f"fma((double)a{m}, F[{m + 1}], {s_expr})"
+    o.append(f"  I = {i_expr};")
+    o.append(f"  const double s1 = -2.0 * p * ({s_expr});")
+    for d in "xyz":
+        e = f"s1 * R{d}d"
+        for m in ms:
+            if f"d{d}{m}" in init:
+                e = f"fma((double)d{d}{m}, F[{m}], {e})"
+        o.append(f"  G{d} = {e};")
+    o.append("}")
     return o, keys, order, parent, axis, aidx
 
 
@@ -226,6 +265,7 @@ def main():
            "template <int L> __device__ __forceinline__ void fe_contract_grad(const double, const double, const double, const double*, const double* __restrict__, const double, double&, double&, double&, double&);",
            "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, const double, double* __restrict__);",
            "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, const float* __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, const float* __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, koff, moff = [], [], [0], [0]

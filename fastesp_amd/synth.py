@@ -35,6 +35,18 @@ class Workload:
     points: np.ndarray
     meta: dict = field(default_factory=dict)
 
+    def fingerprint(self) -> np.ndarray:
+        """Checksums of the inputs.  The density and the point rotations come out
+        of BLAS/LAPACK, whose last-bit rounding differs between hosts, so inputs
+        regenerated elsewhere are compared with a tolerance, not bit for bit."""
+        c, P, x = self.wfn.coords, self.density, self.points
+        e = np.concatenate([np.asarray(s.exps, float) for s in self.wfn.shells])
+        k = np.concatenate([np.asarray(s.coeffs, float) for s in self.wfn.shells])
+        i = np.arange(P.shape[0], dtype=float)
+        return np.array([c.sum(), (c * c).sum(), P.sum(), (P * P).sum(), np.abs(P).sum(),
+                         (P @ i).sum() / max(len(i), 1), x.sum(), (x * x).sum(), np.log(e).sum(),
+                         np.abs(k).sum(), float(len(x)), float(P.shape[0])])
+
     def input_hash(self) -> str:
         h = hashlib.sha256()
         h.update(np.ascontiguousarray(self.wfn.coords).tobytes())

@@ -77,13 +77,15 @@ def make_cfg0():
         out["npairs_" + name[2:]] = tab.n_pairs
     out["n_candidates"] = len(prims.p)
     out["sum_rule"] = prims.sum_rule
+    out["fp"] = w.fingerprint()
     np.savez_compressed(OUT / "cfg0.npz", **out)
 
 
 def make_cfg1_300():
     w = synth.config1(300)
     prims = engine.build_prim_pairs(w.basis, w.density)
-    out = dict(hash=w.input_hash(), n_candidates=len(prims.p), sum_rule=prims.sum_rule)
+    out = dict(hash=w.input_hash(), fp=w.fingerprint(), n_candidates=len(prims.p),
+               sum_rule=prims.sum_rule)
     for name, tol in (("V_tol0", 0.0), ("V_tol12", 1e-12)):
         tab = engine.build_table(w.wfn, w.basis, w.density, w.points, tol, prims=prims)
         out[name] = par_esp(tab, w.points)
@@ -98,7 +100,7 @@ def make_cfg1_30k():
     rng = np.random.default_rng(77)
     idx = np.sort(rng.choice(len(w.points), 256, replace=False))
     V = par_esp(tab, w.points[idx])
-    np.savez_compressed(OUT / "cfg1_30k.npz", hash=w.input_hash(), idx=idx, V=V,
+    np.savez_compressed(OUT / "cfg1_30k.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx, V=V,
                         npairs=tab.n_pairs, n_candidates=len(prims.p),
                         pairs_per_L=np.bincount(tab.pp_L, minlength=7))
 
@@ -135,7 +137,7 @@ def make_cfg2_sub():
     fld = evaluate.field(tab, pts[:64])
     dG = evaluate.grad_G(tab, pts, gV)
     dP = evaluate.grad_P(tab, w.basis, dG)
-    np.savez_compressed(OUT / "cfg2_sub.npz", hash=w.input_hash(), idx=idx, V=V, w=wts,
+    np.savez_compressed(OUT / "cfg2_sub.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx, V=V, w=wts,
                         target=target, gV=gV, field64=fld, dP=dP.astype(np.float64),
                         npairs=tab.n_pairs)
 
