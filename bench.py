@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0 = library default)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="skip the launch-shape autotuner (runs before the timed region)")
     return ap.parse_args()
 
 
@@ -152,7 +154,18 @@ def main() -> int:
     gold = ROOT / "tests" / "golden" / "cfg1_30k.npz"
     g = np.load(gold) if (gold.exists() and n == 30000) else None
 
+    tuned = {}
+
     def timed(mode, prec):
+        if a.tune:
+            from fastesp_amd import autotune
+            r = autotune.tune(tab, pts, op=mode, precision=PREC[prec])
+            tuned[f"{mode}_{prec}"] = dict(ppl_L=r["best"]["ppl_L"],
+                                           target_wg=r["best"]["target_wg"],
+                                           trials=len(r["all"]))
+            log(f"autotune {mode} {prec}: best ppl_L={r['best']['ppl_L']} "
+                f"wg={r['best']['target_wg']} {r['best']['ms']:.2f} ms "
+                f"(first {r['all'][0]['ms']:.2f} ms, {len(r['all'])} trials)")
         step = make_step(mode, prec)
         for _ in range(a.warmup):
             step()
@@ -225,6 +238,7 @@ def main() -> int:
                    "n_functions": int(w.basis.n), "n_points": n,
                    "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU"},
         "legs": legs,
+        "tuning": tuned,
         "speedup_mixed_vs_fp64_fwdbwd": round(legs["fwdbwd_fp64"]["ms_per_step"]
                                               / legs["fwdbwd_mixed"]["ms_per_step"], 3),
         "table_build_ms": round(t_build * 1e3, 2),

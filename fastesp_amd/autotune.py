@@ -1,0 +1,138 @@
+"""Autotuner for the evaluation launch shape.
+
+Knobs (``fe_set_tuning``):
+  keys 10..16  points per lane of class L = key-10 (1, 2 or 4): more points per
+               lane reuse each scalar-loaded pair record more often and hide
+               its load latency, at the cost of VGPRs (occupancy);
+  key 1        target workgroups per class launch (pair-chunk count = target /
+               point blocks): parallelism vs partial-row traffic.
+
+``tune(table, points, ...)`` times candidates with HIP events on the current
+stream (device-resident inputs): first the workgroup target, then the points
+per lane of every class by coordinate descent.  The best shape is applied to
+the context and recorded in a JSON cache keyed by the workload shape (pairs
+per class and point count, rounded to powers of two), so later runs of the same
+shape start tuned (``apply_cached``).
+"""
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+PPL_CHOICES = (1, 2, 4)
+WG_CHOICES = (2048, 4096, 8192)
+NCLS = _lib.FE_LMAX + 1
+CACHE = Path(os.environ.get("FASTESP_TUNE_CACHE",
+                            Path.home() / ".cache" / "fastesp_amd" / "tuning.json"))
+
+
+def shape_key(pairs_per_L, n_points: int, op: str, precision: int) -> str:
+    b = lambda x: int(2 ** round(np.log2(max(int(x), 1))))  # noqa: E731
+    return f"{op}:p{precision}:n{b(n_points)}:" + ",".join(str(b(x) if x else 0)
+                                                          for x in pairs_per_L)
+
+
+def _load() -> dict:
+    try:
+        return json.loads(CACHE.read_text())
+    except Exception:
+        return {}
+
+
+def _store(d: dict) -> None:
+    try:
+        CACHE.parent.mkdir(parents=True, exist_ok=True)
+        CACHE.write_text(json.dumps(d, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def apply(ctx: _lib.Context, ppl_L, target_wg: int) -> None:
+    for L, v in enumerate(ppl_L):
+        ctx.set_tuning(10 + L, int(v))
+    ctx.set_tuning(1, int(target_wg))
+
+
+def apply_cached(ctx, table, n_points: int, op: str = "fwdbwd",
+                 precision: int = _lib.FE_PREC_FP64) -> bool:
+    v = _load().get(shape_key(table.info()["pairs_per_L"], n_points, op, precision))
+    if v:
+        apply(ctx, v["ppl_L"], v["target_wg"])
+        return True
+    return False
+
+
+def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP64,
+         reps: int = 3, verbose: bool = False) -> dict:
+    """Time launch shapes on device-resident points (a torch float64 tensor)."""
+    import torch
+    ctx = table._ctx
+    lib = ctx.lib
+    n = int(points_dev.shape[0])
+    dev = points_dev.device
+    V = torch.empty(n, dtype=torch.float64, device=dev)
+    G = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    gV = torch.ones(n, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    present = [L for L, c in enumerate(table.info()["pairs_per_L"]) if c]
+
+    def run():
+        if op == "fwd":
+            _lib.check(lib.fe_esp_forward(table._h, points_dev.data_ptr(), n, precision,
+                                          _lib.FE_PART_ESP, _lib.FE_PTR_DEVICE, sh, V.data_ptr()),
+                       "fe_esp_forward")
+        else:
+            _lib.check(lib.fe_esp_forward_grad(table._h, points_dev.data_ptr(), n, precision,
+                                               _lib.FE_PTR_DEVICE, sh, V.data_ptr(),
+                                               G.data_ptr()), "fe_esp_forward_grad")
+            _lib.check(lib.fe_esp_backward(table._h, points_dev.data_ptr(), n, gV.data_ptr(),
+                                           precision, _lib.FE_PTR_DEVICE, sh, None, None),
+                       "fe_esp_backward")
+
+    def measure(ppl_L, wg):
+        apply(ctx, ppl_L, wg)
+        run()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        if verbose:
+            print(f"  tune {op} ppl={list(ppl_L)} wg={wg}: {ms:.3f} ms", flush=True)
+        return ms
+
+    log = []
+    ppl = [2] * NCLS
+    best_wg, best_ms = None, float("inf")
+    for wg in WG_CHOICES:
+        ms = measure(ppl, wg)
+        log.append(dict(ppl_L=list(ppl), target_wg=wg, ms=ms))
+        if ms < best_ms:
+            best_wg, best_ms = wg, ms
+    for L in present:
+        for v in PPL_CHOICES:
+            if v == ppl[L]:
+                continue
+            trial = list(ppl)
+            trial[L] = v
+            ms = measure(trial, best_wg)
+            log.append(dict(ppl_L=trial, target_wg=best_wg, ms=ms))
+            if ms < best_ms:
+                ppl, best_ms = trial, ms
+    apply(ctx, ppl, best_wg)
+    best = dict(ppl_L=ppl, target_wg=best_wg, ms=best_ms)
+    k = shape_key(table.info()["pairs_per_L"], n, op, precision)
+    d = _load()
+    d[k] = dict(best, trials=len(log))
+    _store(d)
+    return dict(best=best, all=log, key=k)

@@ -57,18 +57,36 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd_grad(const double* __restrict
   for (int ip = p0; ip < p1; ++ip) {
     const double* __restrict__ r = rec + (size_t)ip * S;
     const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
+    int near = 0;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-      const double Rx = cx[q] - Px, Ry = cy[q] - Py, Rz = cz[q] - Pz;
-      const double T = p * fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
+      Rx[q] = cx[q] - Px;
+      Ry[q] = cy[q] - Py;
+      Rz[q] = cz[q] - Pz;
+      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
+      near |= p * r2[q] < fe_tfar(L + 1);
+    }
+    const bool far = !__any(near);
+    const double hp = r[6], KF = K * r[7];
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
       double F[L + 2];
-      fe_boys<L + 1>(T, F, s_tab);
+      double w = K;
+      if (far) {
+        // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
+        const double ri = rsqrt(r2[q]);
+        fe_boys_far_scaled<L + 1>(hp * (ri * ri), F);
+        w = KF * ri;
+      } else {
+        fe_boys<L + 1>(p * r2[q], F, s_tab);
+      }
       double I, Gx, Gy, Gz;
-      fe_contract_grad<L>(Rx, Ry, Rz, F, r + 8, p, I, Gx, Gy, Gz);
-      av[q] = fma(K, I, av[q]);
-      ax[q] = fma(K, Gx, ax[q]);
-      ay[q] = fma(K, Gy, ay[q]);
-      az[q] = fma(K, Gz, az[q]);
+      fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
+      av[q] = fma(w, I, av[q]);
+      ax[q] = fma(w, Gx, ax[q]);
+      ay[q] = fma(w, Gy, ay[q]);
+      az[q] = fma(w, Gz, az[q]);
     }
   }
 #pragma unroll
@@ -84,7 +102,7 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd_grad(const double* __restrict
   }
 }
 
-__global__ __launch_bounds__(256) void k_esp_reduce_grad(const double* __restrict__ partial,
+__global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict__ partial,
                                                          const int n_rows, const int ld,
                                                          const size_t plane,
                                                          const double* __restrict__ pts,
@@ -94,16 +112,33 @@ __global__ __launch_bounds__(256) void k_esp_reduce_grad(const double* __restric
                                                          const int* __restrict__ perm,
                                                          double* __restrict__ V,
                                                          double* __restrict__ dVdC) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_pts) return;
-  double e = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-  for (int r = 0; r < n_rows; ++r) {
+  // two independent chains per quantity over the rows (fixed order)
+  double e = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, e1 = 0.0, gx1 = 0.0, gy1 = 0.0, gz1 = 0.0;
+  int r = 0;
+  for (; r + 1 < n_rows; r += 2) {
+    const size_t o = (size_t)r * ld + i, o1 = o + ld;
+    e += partial[o];
+    gx += partial[plane + o];
+    gy += partial[2 * plane + o];
+    gz += partial[3 * plane + o];
+    e1 += partial[o1];
+    gx1 += partial[plane + o1];
+    gy1 += partial[2 * plane + o1];
+    gz1 += partial[3 * plane + o1];
+  }
+  if (r < n_rows) {
     const size_t o = (size_t)r * ld + i;
     e += partial[o];
     gx += partial[plane + o];
     gy += partial[2 * plane + o];
     gz += partial[3 * plane + o];
   }
+  e += e1;
+  gx += gx1;
+  gy += gy1;
+  gz += gz1;
   const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
   double nuc = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
   for (int a = 0; a < n_atoms; ++a) {
@@ -125,25 +160,12 @@ __global__ __launch_bounds__(256) void k_esp_reduce_grad(const double* __restric
 }
 
 template <int L, int PPL>
-static void launch_fwd_grad_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                              double* partial, size_t plane, hipStream_t st) {
-  if (pl.chunks[L] == 0) return;
+static void launch_fwd_grad_LP(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                               double* partial, size_t plane, hipStream_t st) {
   const double* tab = t->ctx->d_boys + (size_t)(L + 1) * FE_BOYS_NR * FE_BOYS_COLS;
-  hipLaunchKernelGGL((k_esp_fwd_grad<L, PPL>), dim3(pl.n_pb, pl.chunks[L]), dim3(FE_WG), 0, st,
-                     t->d_rec[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.row0[L],
+  hipLaunchKernelGGL((k_esp_fwd_grad<L, PPL>), dim3(pl.n_pb[L], pl.chunks[L]), dim3(FE_WG), 0,
+                     st, t->d_rec[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.row0[L],
                      pl.ld, plane);
-}
-
-template <int PPL>
-static void launch_fwd_grad_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                                double* partial, size_t plane, hipStream_t st) {
-  launch_fwd_grad_L<0, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<1, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<2, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<3, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<4, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<5, PPL>(t, pl, pts, n, partial, plane, st);
-  launch_fwd_grad_L<6, PPL>(t, pl, pts, n, partial, plane, st);
 }
 
 extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, int precision,
@@ -175,7 +197,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
     rc = fe_sort_points(c, t, d_pts, (int)n, st, &d_pts, &perm);
     if (rc) return rc;
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl, c->target_wg);
+  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
   rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * plane * sizeof(double));
   if (rc) return rc;
@@ -183,12 +205,11 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   if (precision == FE_PREC_MIXED) {
     rc = fe_launch_fwd_grad_mixed(t, pl, d_pts, (int)n, partial, plane, st);
     if (rc) return rc;
-  } else if (c->ppl == 2) {
-    launch_fwd_grad_all<2>(t, pl, d_pts, (int)n, partial, plane, st);
   } else {
-    launch_fwd_grad_all<1>(t, pl, d_pts, (int)n, partial, plane, st);
+    const int ni = (int)n;
+    FE_FOR_EACH_CLASS(pl, launch_fwd_grad_LP, t, pl, d_pts, ni, partial, plane, st);
   }
-  hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
                      partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);
   FE_HIPCHK(hipGetLastError());
@@ -229,18 +250,35 @@ __global__ __launch_bounds__(256) void k_dG(const double* __restrict__ rec, cons
   const int ip = min(pair, n_pairs - 1);
   const double* r = rec + (size_t)ip * S;
   const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+  const double hp = r[6], cF = r[7];
   double acc[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) acc[k] = 0.0;
   const int q0 = blockIdx.y * ppc;
   const int q1 = min(n_pts, q0 + ppc);
-  for (int q = q0; q < q1; ++q) {
-    const double* __restrict__ c = pw + 4 * (size_t)q;
-    const double Rx = c[0] - Px, Ry = c[1] - Py, Rz = c[2] - Pz, w = c[3];
-    const double T = p * fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
+  auto point = [&](const double x, const double y, const double z, double w) {
+    const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
+    const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
     double F[L + 1];
-    fe_boys<L>(T, F, s_tab);
+    if (!__any(p * r2 < fe_tfar(L))) {
+      const double ri = rsqrt(r2);
+      fe_boys_far_scaled<L>(hp * (ri * ri), F);
+      w *= cF * ri;
+    } else {
+      fe_boys<L>(p * r2, F, s_tab);
+    }
     fe_dg_accum<L>(Rx, Ry, Rz, F, w, acc);
+  };
+  // two wave-uniform points per iteration so their scalar loads overlap; the
+  // odd tail point repeats the last point with zero weight
+  for (int q = q0; q < q1; q += 2) {
+    const double* __restrict__ c0 = pw + 4 * (size_t)q;
+    const double* __restrict__ c1 = pw + 4 * (size_t)min(q + 1, q1 - 1);
+    const double x0 = c0[0], y0 = c0[1], z0 = c0[2], w0 = c0[3];
+    const double x1 = c1[0], y1 = c1[1], z1 = c1[2];
+    const double w1 = (q + 1 < q1) ? c1[3] : 0.0;
+    point(x0, y0, z0, w0);
+    point(x1, y1, z1, w1);
   }
   if (pair < n_pairs) {
     double* o = partial + (size_t)blockIdx.y * NT * n_pairs + pair;
@@ -290,8 +328,10 @@ static DGPlan plan_dG(const fe_table* t, int64_t n, int target_wg) {
     const int NT = fe_host::NTERM[L];
     const int64_t nblk = (nL + 255) / 256;
     int64_t ch = std::max<int64_t>(1, target_wg / nblk);
-    ch = std::min<int64_t>(ch, std::max<int64_t>(1, (32ll << 20) / (nL * NT)));
-    ch = std::min<int64_t>(ch, std::max<int64_t>(1, n / 8));
+    // bound the partial buffer (written and re-read once) and keep >= 128
+    // points per chunk so the reduction stays small next to the kernel
+    ch = std::min<int64_t>(ch, std::max<int64_t>(1, (16ll << 20) / (nL * NT)));
+    ch = std::min<int64_t>(ch, std::max<int64_t>(1, n / 64));
     ch = std::min<int64_t>(ch, 65535);
     const int64_t ppc = (n + ch - 1) / ch;
     ch = (n + ppc - 1) / ppc;

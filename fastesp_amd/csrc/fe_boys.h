@@ -41,6 +41,25 @@ __device__ __forceinline__ double fe_exp_small(const double x) {
   return fma(e, x, 1.0);
 }
 
+// Far field.  For T >= fe_tfar(M) the incomplete-gamma tail is below 2^-54 of
+// F_m for every m <= M (thresholds from mpmath at 40 digits), so
+//   F_m(T) = F_0(T) prod_{j=1..m} (2j-1)/(2T),   F_0(T) = sqrt(pi/T)/2
+// exactly to double precision.  A pair of class L then contributes
+//   K F_0 sum_m F'_m a_m(R),  F'_m = prod_{j<=m} (2j-1) x,  x = 1/(2 p R^2),
+// with K F_0 = K sqrt(pi/p)/2 * |R|^-1: no table, no exp, no division per point
+// (1/(2p) and sqrt(pi/p)/2 are stored per pair).
+__host__ __device__ constexpr double fe_tfar(int M) {
+  return M <= 0 ? 35.5 : M == 1 ? 39.5 : M == 2 ? 43.0 : M == 3 ? 46.0 : M == 4 ? 49.0
+       : M == 5 ? 51.5 : M == 6 ? 54.0 : M == 7 ? 56.5 : 59.0;
+}
+
+template <int M>
+__device__ __forceinline__ void fe_boys_far_scaled(const double x, double* F) {
+  F[0] = 1.0;
+#pragma unroll
+  for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((2 * m - 1) * x);
+}
+
 // F[0..M] at T; `tab` is the (FE_BOYS_NR x FE_BOYS_COLS) table of top order M
 // (normally staged in LDS).
 template <int M>

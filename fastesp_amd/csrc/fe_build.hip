@@ -248,7 +248,10 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, 
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)n * S) return;
   const int r = (int)(e / S), c = (int)(e - (int64_t)r * S);
-  dst[e] = src[(size_t)idx[r] * S + c];
+  const double* s = src + (size_t)idx[r] * S;
+  // slots 6/7 of a candidate hold reff and the screen flags; the compacted
+  // record keeps the per-pair far-field constants there instead
+  dst[e] = c == 6 ? 0.5 / s[3] : c == 7 ? 0.88622692545275801365 / sqrt(s[3]) : s[c];
 }
 
 // ---------------------------------------------------------------------------

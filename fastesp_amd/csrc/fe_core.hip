@@ -4,7 +4,7 @@
 // L = l_A + l_B (0..FE_LMAX) are stored contiguously, one 64-byte aligned
 // record of FeL<L>::STRIDE doubles per primitive pair:
 //
-//     [0] Px [1] Py [2] Pz [3] p [4] K [5] K*bound [6] reff [7] spare
+//     [0] Px [1] Py [2] Pz [3] p [4] K [5] K*bound [6] 1/(2p) [7] sqrt(pi/p)/2
 //     [8 .. 8+NT)  g_k in canonical key order, zero padded to a multiple of 8
 //
 // A wavefront evaluates 64 points (x points-per-lane) against the same pair,
@@ -193,8 +193,7 @@ extern "C" int fe_init(int device, fe_ctx** out) {
     delete c;
     return rc;
   }
-  c->ppl = 1;
-  c->target_wg = 8 * c->n_cu;
+  c->target_wg = 16 * c->n_cu;
   *out = c;
   return FE_OK;
 }
@@ -213,9 +212,16 @@ extern "C" int fe_finalize(fe_ctx* c) {
 extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_set_tuning: null context");
   switch (key) {
-    case 0:
-      if (value != 1 && value != 2) return fe_set_error(FE_ERR_ARG, "points per lane must be 1 or 2");
-      c->ppl = (int)value;
+    case 0:  // points per lane of every class
+      if (value != 1 && value != 2 && value != 4)
+        return fe_set_error(FE_ERR_ARG, "points per lane must be 1, 2 or 4");
+      for (int L = 0; L <= FE_LMAX; ++L) c->ppl_L[L] = (int)value;
+      return FE_OK;
+    case 10: case 11: case 12: case 13: case 14: case 15: case 16:  // per class L = key-10
+      if (key - 10 > FE_LMAX) return fe_set_error(FE_ERR_ARG, "class out of range");
+      if (value != 1 && value != 2 && value != 4)
+        return fe_set_error(FE_ERR_ARG, "points per lane must be 1, 2 or 4");
+      c->ppl_L[key - 10] = (int)value;
       return FE_OK;
     case 1:
       if (value < 1 || value > (1 << 24)) return fe_set_error(FE_ERR_ARG, "bad target workgroups");
@@ -358,7 +364,8 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
     r[3] = pp_exp[i];
     r[4] = pp_K[i];
     r[5] = INFINITY;  // K*bound unknown for uploaded tables
-    r[6] = 0.0;
+    r[6] = 0.5 / pp_exp[i];                        // 1/(2p)
+    r[7] = 0.88622692545275801365 / sqrt(pp_exp[i]);  // sqrt(pi/p)/2
     for (int j = 0; j < nt; ++j) r[8 + j] = tm_g[off + j];
     orig[L].push_back(i);
   }
@@ -443,7 +450,7 @@ extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx,
     pp_exp[i] = r[3];
     pp_K[i] = r[4];
     if (pp_kbound) pp_kbound[i] = r[5];
-    if (pp_reff) pp_reff[i] = r[6];
+    if (pp_reff) pp_reff[i] = 1.6 * sqrt((L + 2.0) / r[3]);  // fastesp/esp.py:254
     const int nt = fe_host::NTERM[L], nm = fe_host::NMONO[L];
     pp_nt[i] = nt;
     pp_off[i] = toff;
@@ -497,14 +504,35 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd(const double* __restrict__ re
   for (int ip = p0; ip < p1; ++ip) {
     const double* __restrict__ r = rec + (size_t)ip * S;
     const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
+    int near = 0;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-      const double Rx = cx[q] - Px, Ry = cy[q] - Py, Rz = cz[q] - Pz;
-      const double T = p * fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
-      double F[L + 1];
-      fe_boys<L>(T, F, s_tab);
-      const double s = fe_contract<L>(Rx, Ry, Rz, F, r + 8);
-      acc[q] = fma(K, s, acc[q]);
+      Rx[q] = cx[q] - Px;
+      Ry[q] = cy[q] - Py;
+      Rz[q] = cz[q] - Pz;
+      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
+      near |= p * r2[q] < fe_tfar(L);
+    }
+    if (!__any(near)) {
+      // whole wavefront in the far field: finite multipole, no Boys table
+      const double hp = r[6], KF = K * r[7];
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        const double ri = rsqrt(r2[q]);
+        double F[L + 1];
+        fe_boys_far_scaled<L>(hp * (ri * ri), F);
+        const double s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
+        acc[q] = fma(KF * ri, s, acc[q]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        double F[L + 1];
+        fe_boys<L>(p * r2[q], F, s_tab);
+        const double s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
+        acc[q] = fma(K, s, acc[q]);
+      }
     }
   }
 #pragma unroll
@@ -515,18 +543,27 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd(const double* __restrict__ re
 }
 
 // V = [sum_A Z_A/|C-R_A|] - [sum of partial rows], rows added in a fixed order.
-__global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ partial,
+__global__ __launch_bounds__(64) void k_esp_reduce(const double* __restrict__ partial,
                                                     const int n_rows, const int ld,
                                                     const double* __restrict__ pts, const int n_pts,
                                                     const double* __restrict__ atoms,
                                                     const int n_atoms, const int parts,
                                                     const int* __restrict__ perm,
                                                     double* __restrict__ V) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_pts) return;
   const int o = perm ? perm[i] : i;
-  double e = 0.0;
-  for (int r = 0; r < n_rows; ++r) e += partial[(size_t)r * ld + i];
+  // four independent chains over the rows (fixed order: bitwise reproducible)
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+  int r = 0;
+  for (; r + 3 < n_rows; r += 4) {
+    e0 += partial[(size_t)r * ld + i];
+    e1 += partial[(size_t)(r + 1) * ld + i];
+    e2 += partial[(size_t)(r + 2) * ld + i];
+    e3 += partial[(size_t)(r + 3) * ld + i];
+  }
+  for (; r < n_rows; ++r) e0 += partial[(size_t)r * ld + i];
+  const double e = (e0 + e1) + (e2 + e3);
   double nuc = 0.0;
   if (parts & FE_PART_NUCLEAR) {
     const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
@@ -554,40 +591,50 @@ __global__ __launch_bounds__(256) void k_boys_eval(const double* __restrict__ T,
 // launch helpers
 // ---------------------------------------------------------------------------
 template <int L, int PPL>
-static void launch_fwd_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                         double* partial, hipStream_t st) {
-  if (pl.chunks[L] == 0) return;
-  dim3 grid(pl.n_pb, pl.chunks[L]);
+static void launch_fwd_LP(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                          double* partial, hipStream_t st) {
+  dim3 grid(pl.n_pb[L], pl.chunks[L]);
   const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
   hipLaunchKernelGGL((k_esp_fwd<L, PPL>), grid, dim3(FE_WG), 0, st, t->d_rec[L], (int)t->n_L[L],
                      pl.ppc[L], pts, n, tab, partial, pl.row0[L], pl.ld);
 }
 
-template <int PPL>
-static void launch_fwd_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                           double* partial, hipStream_t st) {
-  launch_fwd_L<0, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<1, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<2, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<3, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<4, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<5, PPL>(t, pl, pts, n, partial, st);
-  launch_fwd_L<6, PPL>(t, pl, pts, n, partial, st);
+template <int L>
+static void launch_fwd_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                         double* partial, hipStream_t st) {
+  if (pl.chunks[L] == 0) return;
+  switch (pl.ppl[L]) {
+    case 4: launch_fwd_LP<L, 4>(t, pl, pts, n, partial, st); break;
+    case 2: launch_fwd_LP<L, 2>(t, pl, pts, n, partial, st); break;
+    default: launch_fwd_LP<L, 1>(t, pl, pts, n, partial, st); break;
+  }
 }
 
-FwdPlan fe_plan(const fe_table* t, int64_t n, int ppl, int target_wg) {
+static void launch_fwd_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
+                           double* partial, hipStream_t st) {
+  launch_fwd_L<0>(t, pl, pts, n, partial, st);
+  launch_fwd_L<1>(t, pl, pts, n, partial, st);
+  launch_fwd_L<2>(t, pl, pts, n, partial, st);
+  launch_fwd_L<3>(t, pl, pts, n, partial, st);
+  launch_fwd_L<4>(t, pl, pts, n, partial, st);
+  launch_fwd_L<5>(t, pl, pts, n, partial, st);
+  launch_fwd_L<6>(t, pl, pts, n, partial, st);
+}
+
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
   FwdPlan pl{};
-  const int per_block = FE_WG * ppl;
-  pl.n_pb = (int)((n + per_block - 1) / per_block);
-  pl.ld = pl.n_pb * per_block;
+  pl.ld = (int)((n + 1023) / 1024 * 1024);
   int rows = 0;
   for (int L = 0; L <= FE_LMAX; ++L) {
     const int64_t nL = t->n_L[L];
+    pl.ppl[L] = (ppl[L] == 4 || ppl[L] == 2) ? ppl[L] : 1;
+    const int per_block = FE_WG * pl.ppl[L];
+    pl.n_pb[L] = (int)((n + per_block - 1) / per_block);
     if (nL == 0) {
       pl.chunks[L] = 0;
       continue;
     }
-    int64_t c = std::max<int64_t>(1, target_wg / std::max(pl.n_pb, 1));
+    int64_t c = std::max<int64_t>(1, target_wg / std::max(pl.n_pb[L], 1));
     c = std::min<int64_t>(c, nL);
     c = std::min<int64_t>(c, 65535);
     const int64_t ppc = (nL + c - 1) / c;
@@ -644,7 +691,7 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     rc = fe_sort_points(c, t, d_pts, (int)n, st, &d_pts, &perm);
     if (rc) return rc;
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl, c->target_wg);
+  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
   double* partial = nullptr;
   if ((parts & FE_PART_ELECTRON) && pl.rows > 0) {
     rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)pl.rows * pl.ld * sizeof(double));
@@ -653,15 +700,13 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     if (precision == FE_PREC_MIXED) {
       rc = fe_launch_fwd_mixed(t, pl, d_pts, (int)n, partial, st);
       if (rc) return rc;
-    } else if (c->ppl == 2) {
-      launch_fwd_all<2>(t, pl, d_pts, (int)n, partial, st);
     } else {
-      launch_fwd_all<1>(t, pl, d_pts, (int)n, partial, st);
+      launch_fwd_all(t, pl, d_pts, (int)n, partial, st);
     }
     FE_HIPCHK(hipGetLastError());
   }
   const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
-  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, partial,
+  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, partial,
                      nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, perm, d_V);
   FE_HIPCHK(hipGetLastError());
   if (ptr_kind == FE_PTR_HOST) {

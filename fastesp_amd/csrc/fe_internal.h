@@ -61,8 +61,8 @@ struct fe_ctx {
   LTabDev ltab{};
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
-  int ppl = 1;          // points per lane
-  int target_wg = 2048; // workgroups per class launch
+  int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 2};  // points per lane, per class
+  int target_wg = 4096; // workgroups per class launch
   bool tile_screen = false;
   bool sort_points = true;  // Morton-order the points before evaluation
   double mixed_tau = 1e-4;  // FP32 routing threshold of the mixed path (a.u.)
@@ -102,9 +102,10 @@ struct fe_table {
 };
 
 struct FwdPlan {
-  int n_pb = 0;   // point blocks
-  int ld = 0;     // leading dimension of a partial row
+  int ld = 0;     // leading dimension of a partial row (>= n, multiple of 1024)
   int rows = 0;   // total partial rows
+  int n_pb[FE_LMAX + 1] = {};   // point blocks of the class launch
+  int ppl[FE_LMAX + 1] = {};    // points per lane of the class launch (1, 2, 4)
   int chunks[FE_LMAX + 1] = {};
   int ppc[FE_LMAX + 1] = {};
   int row0[FE_LMAX + 1] = {};
@@ -114,7 +115,29 @@ int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);
 int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz);
 int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec);
 int fe_table_index(fe_table* t);  // flat term offsets of the records (from orig order)
-FwdPlan fe_plan(const fe_table* t, int64_t n, int ppl, int target_wg);
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg);
+
+// dispatch a templated class launcher on the plan's points-per-lane of class L
+#define FE_DISPATCH_PPL(pl, L, FN, ...)                  \
+  do {                                                   \
+    if ((pl).chunks[L] == 0) break;                      \
+    switch ((pl).ppl[L]) {                               \
+      case 4: FN<L, 4>(__VA_ARGS__); break;              \
+      case 2: FN<L, 2>(__VA_ARGS__); break;              \
+      default: FN<L, 1>(__VA_ARGS__); break;             \
+    }                                                    \
+  } while (0)
+
+#define FE_FOR_EACH_CLASS(pl, FN, ...)                   \
+  do {                                                   \
+    FE_DISPATCH_PPL(pl, 0, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 1, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 2, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 3, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 4, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 5, FN, __VA_ARGS__);             \
+    FE_DISPATCH_PPL(pl, 6, FN, __VA_ARGS__);             \
+  } while (0)
 int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipStream_t st,
                     const double** d_pts);
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,

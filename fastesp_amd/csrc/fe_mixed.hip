@@ -77,16 +77,28 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd_mixed(
       big |= kb * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
     }
     const bool fp64 = __any(big);
+    int near = 0;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) near |= p * r2[q] < fe_tfar(L);
+    const bool far = !__any(near);
+    const double hp = r[6], KF = K * r[7];
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       double F[L + 1];
-      fe_boys<L>(p * r2[q], F, s_tab);
+      double w = K;
+      if (far) {
+        const double ri = rsqrt(r2[q]);
+        fe_boys_far_scaled<L>(hp * (ri * ri), F);
+        w = KF * ri;
+      } else {
+        fe_boys<L>(p * r2[q], F, s_tab);
+      }
       double s;
       if (fp64)
         s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
       else
         s = fe_contract_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g);
-      fe_kahan_add(acc[q], cmp[q], K * s);
+      fe_kahan_add(acc[q], cmp[q], w * s);
     }
   }
 #pragma unroll
@@ -136,20 +148,32 @@ __global__ __launch_bounds__(FE_WG) void k_esp_fwd_grad_mixed(
       big |= kb * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
     }
     const bool fp64 = __any(big);
+    int near = 0;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) near |= p * r2[q] < fe_tfar(L + 1);
+    const bool far = !__any(near);
+    const double hp = r[6], KF = K * r[7];
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       double F[L + 2];
-      fe_boys<L + 1>(p * r2[q], F, s_tab);
+      double w = K;
+      if (far) {
+        const double ri = rsqrt(r2[q]);
+        fe_boys_far_scaled<L + 1>(hp * (ri * ri), F);
+        w = KF * ri;
+      } else {
+        fe_boys<L + 1>(p * r2[q], F, s_tab);
+      }
       double I, Gx, Gy, Gz;
       if (fp64)
         fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
       else
         fe_contract_grad_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g, p, Rx[q], Ry[q],
                                 Rz[q], I, Gx, Gy, Gz);
-      fe_kahan_add(av[q], cv[q], K * I);
-      ax[q] = fma(K, Gx, ax[q]);
-      ay[q] = fma(K, Gy, ay[q]);
-      az[q] = fma(K, Gz, az[q]);
+      fe_kahan_add(av[q], cv[q], w * I);
+      ax[q] = fma(w, Gx, ax[q]);
+      ay[q] = fma(w, Gy, ay[q]);
+      az[q] = fma(w, Gz, az[q]);
     }
   }
 #pragma unroll
@@ -195,9 +219,8 @@ int fe_ensure_g32(fe_table* t, hipStream_t st) {
 template <int L, int PPL>
 static void launch_mixed_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
                            double* partial, float tau, hipStream_t st) {
-  if (!pl.chunks[L]) return;
   const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
-  hipLaunchKernelGGL((k_esp_fwd_mixed<L, PPL>), dim3(pl.n_pb, pl.chunks[L]), dim3(FE_WG), 0, st,
+  hipLaunchKernelGGL((k_esp_fwd_mixed<L, PPL>), dim3(pl.n_pb[L], pl.chunks[L]), dim3(FE_WG), 0, st,
                      t->d_rec[L], t->d_rec32[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial,
                      pl.row0[L], pl.ld, tau);
 }
@@ -205,35 +228,10 @@ static void launch_mixed_L(const fe_table* t, const FwdPlan& pl, const double* p
 template <int L, int PPL>
 static void launch_mixed_grad_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
                                 double* partial, size_t plane, float tau, hipStream_t st) {
-  if (!pl.chunks[L]) return;
   const double* tab = t->ctx->d_boys + (size_t)(L + 1) * FE_BOYS_NR * FE_BOYS_COLS;
-  hipLaunchKernelGGL((k_esp_fwd_grad_mixed<L, PPL>), dim3(pl.n_pb, pl.chunks[L]), dim3(FE_WG), 0,
-                     st, t->d_rec[L], t->d_rec32[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab,
+  hipLaunchKernelGGL((k_esp_fwd_grad_mixed<L, PPL>), dim3(pl.n_pb[L], pl.chunks[L]), dim3(FE_WG),
+                     0, st, t->d_rec[L], t->d_rec32[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab,
                      partial, pl.row0[L], pl.ld, plane, tau);
-}
-
-template <int PPL>
-static void launch_mixed_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                             double* partial, float tau, hipStream_t st) {
-  launch_mixed_L<0, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<1, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<2, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<3, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<4, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<5, PPL>(t, pl, pts, n, partial, tau, st);
-  launch_mixed_L<6, PPL>(t, pl, pts, n, partial, tau, st);
-}
-
-template <int PPL>
-static void launch_mixed_grad_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                                  double* partial, size_t plane, float tau, hipStream_t st) {
-  launch_mixed_grad_L<0, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<1, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<2, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<3, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<4, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<5, PPL>(t, pl, pts, n, partial, plane, tau, st);
-  launch_mixed_grad_L<6, PPL>(t, pl, pts, n, partial, plane, tau, st);
 }
 
 int fe_launch_fwd_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,
@@ -241,10 +239,7 @@ int fe_launch_fwd_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n
   int rc = fe_ensure_g32(t, st);
   if (rc) return rc;
   const float tau = (float)t->ctx->mixed_tau;
-  if (t->ctx->ppl == 2)
-    launch_mixed_all<2>(t, pl, pts, n, partial, tau, st);
-  else
-    launch_mixed_all<1>(t, pl, pts, n, partial, tau, st);
+  FE_FOR_EACH_CLASS(pl, launch_mixed_L, t, pl, pts, n, partial, tau, st);
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }
@@ -254,10 +249,7 @@ int fe_launch_fwd_grad_mixed(fe_table* t, const FwdPlan& pl, const double* pts, 
   int rc = fe_ensure_g32(t, st);
   if (rc) return rc;
   const float tau = (float)t->ctx->mixed_tau;
-  if (t->ctx->ppl == 2)
-    launch_mixed_grad_all<2>(t, pl, pts, n, partial, plane, tau, st);
-  else
-    launch_mixed_grad_all<1>(t, pl, pts, n, partial, plane, tau, st);
+  FE_FOR_EACH_CLASS(pl, launch_mixed_grad_L, t, pl, pts, n, partial, plane, tau, st);
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }
