@@ -48,6 +48,13 @@ def parse():
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0 = library default)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the launch-shape autotuner (runs before the timed region)")
+    ap.add_argument("--ppl-classes", default="",
+                    help="explicit points per lane per class, e.g. 4,2,2,2,2,2,1 (implies --no-tune)")
+    ap.add_argument("--wg", type=int, default=0, help="explicit target workgroups per launch")
+    ap.add_argument("--legs", default="fwd_fp64,fwdbwd_fp64,fwdbwd_mixed",
+                    help="comma list of mode_precision legs to time")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -108,6 +115,12 @@ def main() -> int:
     ctx = _lib.Context.get(0)
     if a.ppl:
         ctx.set_tuning(0, a.ppl)
+    if a.ppl_classes:
+        a.tune = False
+        for L, v in enumerate(int(x) for x in a.ppl_classes.split(",")):
+            ctx.set_tuning(10 + L, v)
+    if a.wg:
+        ctx.set_tuning(1, a.wg)
     # --- table build on the GPU (reported, not part of the step) ----------
     build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)  # warm-up
     torch.cuda.synchronize()
@@ -188,11 +201,19 @@ def main() -> int:
         return res
 
     legs = {}
-    for mode, prec in (("fwd", "fp64"), ("fwdbwd", "fp64"), ("fwdbwd", "mixed")):
-        legs[f"{mode}_{prec}"] = timed(mode, prec)
+    for leg in a.legs.split(","):
+        mode, prec = leg.split("_")
+        legs[leg] = timed(mode, prec)
     key = f"{a.mode}_{a.precision}"
     if key not in legs:
         legs[key] = timed(a.mode, a.precision)
+    if a.copy_probe:
+        # a plain copy of known size, so HBM byte counters can be calibrated
+        src = torch.empty(a.copy_probe << 17, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {a.copy_probe} MiB read + {a.copy_probe} MiB written")
     head = legs[key]
     ms_step = head["ms_per_step"]
 
@@ -208,8 +229,16 @@ def main() -> int:
         peak = fl / t_roof
     else:
         peak = flops.PEAK_FP64_VALU
+    # HBM bytes per step from the committed rocprofv3 --pmc summary of the FP64
+    # forward+backward step (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    traffic = None
+    tr = ROOT / "profiles" / "esp_cfg1_30k_fwdbwd_fp64_hbm_traffic.md"
+    if a.mode == "fwdbwd" and n == 30000 and tr.exists():
+        for line in tr.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_step"):
+                traffic = int(float(line.split("=")[1]))
     roof = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
-                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
+                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic,
                 kernel=("k_esp_fwd<L> x7 + k_esp_reduce" if a.mode == "fwd" else
                         "k_esp_fwd_grad<L> x7 + k_esp_reduce_grad + k_dG<L> x7 + k_dG_reduce<L> "
                         "x7 + k_dg_to_dp (one forward_grad + backward)"),
@@ -239,8 +268,9 @@ def main() -> int:
                    "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU"},
         "legs": legs,
         "tuning": tuned,
-        "speedup_mixed_vs_fp64_fwdbwd": round(legs["fwdbwd_fp64"]["ms_per_step"]
-                                              / legs["fwdbwd_mixed"]["ms_per_step"], 3),
+        "speedup_mixed_vs_fp64_fwdbwd": (
+            round(legs["fwdbwd_fp64"]["ms_per_step"] / legs["fwdbwd_mixed"]["ms_per_step"], 3)
+            if "fwdbwd_fp64" in legs and "fwdbwd_mixed" in legs else None),
         "table_build_ms": round(t_build * 1e3, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
