@@ -2,7 +2,7 @@
 // The reference has no gradients; this is the hand-written backward behind
 // the torch.autograd.Function of fastesp_amd.autograd.
 //
-//  fe_esp_forward_grad : V and dV/dC in one pass (k_esp_fwd_grad<L>): per
+//  fe_esp_forward_grad : V and dV/dC in one pass (k_eval<L, PPL, GRAD=true>): per
 //      (point, pair) the Boys vector up to L+1 and
 //      dI/dC = K sum_k g_k [ d(R^a)/dR F_m - 2p R R^a F_{m+1} ],
 //      nuclear part dV/dC = -sum_A Z_A (C-R_A)/|C-R_A|^3.
@@ -24,83 +24,6 @@
 #include "fe_lclass.inc"
 #include "fe_internal.h"
 #include "fe_hermite.h"
-
-// ---------------------------------------------------------------------------
-// forward + field
-// ---------------------------------------------------------------------------
-template <int L, int PPL>
-__global__ __launch_bounds__(FE_WG) void k_esp_fwd_grad(const double* __restrict__ rec,
-                                                        const int n_pairs, const int ppc,
-                                                        const double* __restrict__ pts,
-                                                        const int n_pts,
-                                                        const double* __restrict__ boys_tab,
-                                                        double* __restrict__ partial,
-                                                        const int row0, const int ld,
-                                                        const size_t plane) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
-  __syncthreads();
-  constexpr int S = FeL<L>::STRIDE;
-  const int chunk = blockIdx.y;
-  const int p0 = chunk * ppc;
-  const int p1 = min(n_pairs, p0 + ppc);
-  double cx[PPL], cy[PPL], cz[PPL], av[PPL], ax[PPL], ay[PPL], az[PPL];
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
-    const int ii = min(i, n_pts - 1);
-    cx[q] = pts[3 * ii];
-    cy[q] = pts[3 * ii + 1];
-    cz[q] = pts[3 * ii + 2];
-    av[q] = ax[q] = ay[q] = az[q] = 0.0;
-  }
-  for (int ip = p0; ip < p1; ++ip) {
-    const double* __restrict__ r = rec + (size_t)ip * S;
-    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
-    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
-    int near = 0;
-#pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      Rx[q] = cx[q] - Px;
-      Ry[q] = cy[q] - Py;
-      Rz[q] = cz[q] - Pz;
-      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
-      near |= p * r2[q] < fe_tfar(L + 1);
-    }
-    const bool far = !__any(near);
-    const double hp = r[6], KF = K * r[7];
-#pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      double F[L + 2];
-      double w = K;
-      if (far) {
-        // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
-        const double ri = rsqrt(r2[q]);
-        fe_boys_far_scaled<L + 1>(hp * (ri * ri), F);
-        w = KF * ri;
-      } else {
-        fe_boys<L + 1>(p * r2[q], F, s_tab);
-      }
-      double I, Gx, Gy, Gz;
-      fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
-      av[q] = fma(w, I, av[q]);
-      ax[q] = fma(w, Gx, ax[q]);
-      ay[q] = fma(w, Gy, ay[q]);
-      az[q] = fma(w, Gz, az[q]);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
-    if (i < n_pts) {
-      const size_t o = (size_t)(row0 + chunk) * ld + i;
-      partial[o] = av[q];
-      partial[plane + o] = ax[q];
-      partial[2 * plane + o] = ay[q];
-      partial[3 * plane + o] = az[q];
-    }
-  }
-}
 
 __global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict__ partial,
                                                          const int n_rows, const int ld,
@@ -159,15 +82,6 @@ __global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict
   dVdC[3 * o + 2] = nz - gz;
 }
 
-template <int L, int PPL>
-static void launch_fwd_grad_LP(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                               double* partial, size_t plane, hipStream_t st) {
-  const double* tab = t->ctx->d_boys + (size_t)(L + 1) * FE_BOYS_NR * FE_BOYS_COLS;
-  hipLaunchKernelGGL((k_esp_fwd_grad<L, PPL>), dim3(pl.n_pb[L], pl.chunks[L]), dim3(FE_WG), 0,
-                     st, t->d_rec[L], (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.row0[L],
-                     pl.ld, plane);
-}
-
 extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, int precision,
                                    int ptr_kind, void* stream, double* V, double* dVdC) {
   if (!t || (n > 0 && (!pts || !V || !dVdC)))
@@ -202,13 +116,8 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * plane * sizeof(double));
   if (rc) return rc;
   double* partial = (double*)c->ws[FE_WS_PARTIAL];
-  if (precision == FE_PREC_MIXED) {
-    rc = fe_launch_fwd_grad_mixed(t, pl, d_pts, (int)n, partial, plane, st);
-    if (rc) return rc;
-  } else {
-    const int ni = (int)n;
-    FE_FOR_EACH_CLASS(pl, launch_fwd_grad_LP, t, pl, d_pts, ni, partial, plane, st);
-  }
+  rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, plane, true, precision == FE_PREC_MIXED, st);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
                      partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);

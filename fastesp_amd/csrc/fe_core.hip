@@ -476,72 +476,6 @@ extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx,
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int L, int PPL>
-__global__ __launch_bounds__(FE_WG) void k_esp_fwd(const double* __restrict__ rec, const int n_pairs,
-                                                   const int ppc, const double* __restrict__ pts,
-                                                   const int n_pts,
-                                                   const double* __restrict__ boys_tab,
-                                                   double* __restrict__ partial, const int row0,
-                                                   const int ld) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
-  __syncthreads();
-
-  constexpr int S = FeL<L>::STRIDE;
-  const int chunk = blockIdx.y;
-  const int p0 = chunk * ppc;
-  const int p1 = min(n_pairs, p0 + ppc);
-  double cx[PPL], cy[PPL], cz[PPL], acc[PPL];
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
-    const int ii = min(i, n_pts - 1);
-    cx[q] = pts[3 * ii + 0];
-    cy[q] = pts[3 * ii + 1];
-    cz[q] = pts[3 * ii + 2];
-    acc[q] = 0.0;
-  }
-  for (int ip = p0; ip < p1; ++ip) {
-    const double* __restrict__ r = rec + (size_t)ip * S;
-    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
-    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
-    int near = 0;
-#pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      Rx[q] = cx[q] - Px;
-      Ry[q] = cy[q] - Py;
-      Rz[q] = cz[q] - Pz;
-      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
-      near |= p * r2[q] < fe_tfar(L);
-    }
-    if (!__any(near)) {
-      // whole wavefront in the far field: finite multipole, no Boys table
-      const double hp = r[6], KF = K * r[7];
-#pragma unroll
-      for (int q = 0; q < PPL; ++q) {
-        const double ri = rsqrt(r2[q]);
-        double F[L + 1];
-        fe_boys_far_scaled<L>(hp * (ri * ri), F);
-        const double s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
-        acc[q] = fma(KF * ri, s, acc[q]);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < PPL; ++q) {
-        double F[L + 1];
-        fe_boys<L>(p * r2[q], F, s_tab);
-        const double s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
-        acc[q] = fma(K, s, acc[q]);
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    const int i = (blockIdx.x * PPL + q) * FE_WG + threadIdx.x;
-    if (i < n_pts) partial[(size_t)(row0 + chunk) * ld + i] = acc[q];
-  }
-}
-
 // V = [sum_A Z_A/|C-R_A|] - [sum of partial rows], rows added in a fixed order.
 __global__ __launch_bounds__(64) void k_esp_reduce(const double* __restrict__ partial,
                                                     const int n_rows, const int ld,
@@ -590,37 +524,6 @@ __global__ __launch_bounds__(256) void k_boys_eval(const double* __restrict__ T,
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-template <int L, int PPL>
-static void launch_fwd_LP(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                          double* partial, hipStream_t st) {
-  dim3 grid(pl.n_pb[L], pl.chunks[L]);
-  const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
-  hipLaunchKernelGGL((k_esp_fwd<L, PPL>), grid, dim3(FE_WG), 0, st, t->d_rec[L], (int)t->n_L[L],
-                     pl.ppc[L], pts, n, tab, partial, pl.row0[L], pl.ld);
-}
-
-template <int L>
-static void launch_fwd_L(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                         double* partial, hipStream_t st) {
-  if (pl.chunks[L] == 0) return;
-  switch (pl.ppl[L]) {
-    case 4: launch_fwd_LP<L, 4>(t, pl, pts, n, partial, st); break;
-    case 2: launch_fwd_LP<L, 2>(t, pl, pts, n, partial, st); break;
-    default: launch_fwd_LP<L, 1>(t, pl, pts, n, partial, st); break;
-  }
-}
-
-static void launch_fwd_all(const fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                           double* partial, hipStream_t st) {
-  launch_fwd_L<0>(t, pl, pts, n, partial, st);
-  launch_fwd_L<1>(t, pl, pts, n, partial, st);
-  launch_fwd_L<2>(t, pl, pts, n, partial, st);
-  launch_fwd_L<3>(t, pl, pts, n, partial, st);
-  launch_fwd_L<4>(t, pl, pts, n, partial, st);
-  launch_fwd_L<5>(t, pl, pts, n, partial, st);
-  launch_fwd_L<6>(t, pl, pts, n, partial, st);
-}
-
 FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
   FwdPlan pl{};
   pl.ld = (int)((n + 1023) / 1024 * 1024);
@@ -697,13 +600,8 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)pl.rows * pl.ld * sizeof(double));
     if (rc) return rc;
     partial = (double*)c->ws[FE_WS_PARTIAL];
-    if (precision == FE_PREC_MIXED) {
-      rc = fe_launch_fwd_mixed(t, pl, d_pts, (int)n, partial, st);
-      if (rc) return rc;
-    } else {
-      launch_fwd_all(t, pl, d_pts, (int)n, partial, st);
-    }
-    FE_HIPCHK(hipGetLastError());
+    rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, 0, false, precision == FE_PREC_MIXED, st);
+    if (rc) return rc;
   }
   const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
   hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, partial,

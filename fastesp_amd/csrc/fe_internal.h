@@ -142,7 +142,19 @@ int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipSt
                     const double** d_pts);
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
                    const double** d_sorted, const int32_t** d_perm);
-int fe_launch_fwd_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                        double* partial, hipStream_t st);
-int fe_launch_fwd_grad_mixed(fe_table* t, const FwdPlan& pl, const double* pts, int n,
-                             double* partial, size_t plane, hipStream_t st);
+// one work item of the batched (persistent) evaluation: pairs [p0, p1) of one
+// class against the points [base, min(base + PPL*FE_WG, end)) of one conformer;
+// partial sums go to partial[out + ...] (planes `plane` apart for the field)
+struct FeBatchItem {
+  int64_t out;
+  int p0, p1, base, end;
+  int pad0, pad1;
+};
+
+int fe_ensure_g32(fe_table* t, hipStream_t st);
+int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,
+                   size_t plane, bool grad, bool mixed, hipStream_t st);
+void fe_launch_batch_eval(int L, int ppl, const FeBatchItem* items, int n_items, int grid,
+                          const double* rec, const float* g32, const double* pts,
+                          const double* boys, double* partial, size_t plane, bool grad,
+                          bool mixed, float tau, hipStream_t st);

@@ -78,8 +78,9 @@ int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hip
   const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
   const double scale = 1023.0 / ext;
   size_t temp_bytes = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                     (int32_t*)nullptr, (int32_t*)nullptr, n, 0, 30, st);
+  FE_HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (uint32_t*)nullptr,
+                                               (uint32_t*)nullptr, (int32_t*)nullptr,
+                                               (int32_t*)nullptr, n, 0, 30, st));
   const size_t al = 256;
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
   const size_t b_key = up((size_t)n * 4), b_pts = up((size_t)n * 3 * sizeof(double));

@@ -1,0 +1,127 @@
+// The per-(point tile, pair range) evaluation shared by every evaluation kernel
+// (single table and batched; value or value + field; FP64 or mixed).
+//
+// A wavefront holds PPL points per lane and walks the pairs [p0, p1) of class
+// L.  The pair record is wave-uniform (scalar loads); per point the kernel
+// forms R = C - P and T = p R^2, takes the far-field path when every lane of
+// the wavefront has T >= T_far(M) (M = L, or L+1 for the field), else the
+// tabulated Boys function, and contracts with the generated per-L code.
+//
+// MIXED: the contraction runs in FP32 (pair-local R) for pairs whose screening
+// estimate K*bound/max(|R|, 0.05) is below tau on every lane; R, T, Boys and
+// the point sums stay FP64, the value is accumulated with Neumaier compensation.
+#pragma once
+
+#include "fe_boys.h"
+#include "fe_lclass.inc"
+
+#define FE_S32(NT) ((((NT) + 15) / 16) * 16)
+
+__device__ __forceinline__ void fe_kahan_add(double& s, double& c, const double y) {
+  const double t = s + y;
+  c += (fabs(s) >= fabs(y)) ? (s - t) + y : (y - t) + s;
+  s = t;
+}
+
+template <int L, int PPL, bool GRAD, bool MIXED>
+__device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
+                                             const float* __restrict__ g32, const int p0,
+                                             const int p1, const double* cx, const double* cy,
+                                             const double* cz, const double* s_tab,
+                                             const float tau, double* av, double* cv, double* ax,
+                                             double* ay, double* az) {
+  constexpr int S = FeL<L>::STRIDE, S32 = FE_S32(FeL<L>::NT);
+  constexpr int M = GRAD ? L + 1 : L;
+  for (int ip = p0; ip < p1; ++ip) {
+    const double* __restrict__ r = rec + (size_t)ip * S;
+    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
+    int near = 0, big = 0;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      Rx[q] = cx[q] - Px;
+      Ry[q] = cy[q] - Py;
+      Rz[q] = cz[q] - Pz;
+      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
+      near |= p * r2[q] < fe_tfar(M);
+      if constexpr (MIXED) big |= (float)r[5] * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
+    }
+    const bool far = !__any(near);
+    bool fp64 = true;
+    if constexpr (MIXED) fp64 = __any(big);
+    const double hp = r[6], KF = K * r[7];
+    const float* __restrict__ g = g32 + (size_t)ip * S32;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      double F[M + 1];
+      double w = K;
+      if (far) {
+        // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
+        const double ri = rsqrt(r2[q]);
+        fe_boys_far_scaled<M>(hp * (ri * ri), F);
+        w = KF * ri;
+      } else {
+        fe_boys<M>(p * r2[q], F, s_tab);
+      }
+      if constexpr (GRAD) {
+        double I, Gx, Gy, Gz;
+        if (fp64)
+          fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
+        else
+          fe_contract_grad_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g, p, Rx[q],
+                                  Ry[q], Rz[q], I, Gx, Gy, Gz);
+        if constexpr (MIXED)
+          fe_kahan_add(av[q], cv[q], w * I);
+        else
+          av[q] = fma(w, I, av[q]);
+        ax[q] = fma(w, Gx, ax[q]);
+        ay[q] = fma(w, Gy, ay[q]);
+        az[q] = fma(w, Gz, az[q]);
+      } else {
+        double s;
+        if (fp64)
+          s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
+        else
+          s = fe_contract_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g);
+        if constexpr (MIXED)
+          fe_kahan_add(av[q], cv[q], w * s);
+        else
+          av[q] = fma(w, s, av[q]);
+      }
+    }
+  }
+}
+
+// Load PPL points per lane of the block starting at `base` (clamped), run the
+// tile, and write the partial sums (value plane; three more planes with GRAD)
+// for the points below `end`.
+template <int L, int PPL, bool GRAD, bool MIXED>
+__device__ __forceinline__ void fe_eval_block(const double* __restrict__ rec,
+                                              const float* __restrict__ g32, const int p0,
+                                              const int p1, const double* __restrict__ pts,
+                                              const int base, const int end,
+                                              const double* s_tab, const float tau,
+                                              double* __restrict__ out, const size_t plane) {
+  double cx[PPL], cy[PPL], cz[PPL], av[PPL], cv[PPL], ax[PPL], ay[PPL], az[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int i = min(base + q * FE_WG + (int)threadIdx.x, end - 1);
+    cx[q] = pts[3 * (size_t)i];
+    cy[q] = pts[3 * (size_t)i + 1];
+    cz[q] = pts[3 * (size_t)i + 2];
+    av[q] = cv[q] = ax[q] = ay[q] = az[q] = 0.0;
+  }
+  fe_eval_tile<L, PPL, GRAD, MIXED>(rec, g32, p0, p1, cx, cy, cz, s_tab, tau, av, cv, ax, ay, az);
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int o = q * FE_WG + (int)threadIdx.x;
+    if (base + o < end) {
+      out[o] = av[q] + cv[q];
+      if constexpr (GRAD) {
+        out[plane + o] = ax[q];
+        out[2 * plane + o] = ay[q];
+        out[3 * plane + o] = az[q];
+      }
+    }
+  }
+}

@@ -142,8 +142,24 @@ def make_cfg2_sub():
                         npairs=tab.n_pairs)
 
 
+def make_cfg3():
+    """Config 3 (150 atoms, ~4500 functions, 200000 points): the NumPy oracle is
+    too slow to screen ~1e6 pairs against 2e5 points, so this fixture uses the
+    C++/OpenMP restatement (oracle/cpu_port.cpp, NumPy screen rule), which
+    matches the NumPy oracle to 5e-14 a.u. on config 0 (test_cpu_port.py)."""
+    from oracle.cpu_port import CpuTable
+    w = synth.config3()
+    ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, rule=0)
+    rng = np.random.default_rng(303)
+    idx = np.sort(rng.choice(len(w.points), 256, replace=False))
+    V = ct.esp(w.points[idx])
+    np.savez_compressed(OUT / "cfg3.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx, V=V,
+                        npairs=ct.n_pairs, n_candidates=ct.n_candidates, sum_rule=ct.sum_rule,
+                        n_func=w.basis.n)
+
+
 JOBS = dict(boys=make_boys, cfg0=make_cfg0, cfg1_300=make_cfg1_300, cfg1_30k=make_cfg1_30k,
-            grad_toy=make_grad_toy, cfg2_sub=make_cfg2_sub)
+            grad_toy=make_grad_toy, cfg2_sub=make_cfg2_sub, cfg3=make_cfg3)
 
 
 def main():
