@@ -25,6 +25,13 @@ FE_PART_ELECTRON = 2
 FE_PART_ESP = 3
 FE_LMAX = 6
 FE_BOYS_MMAX = 8
+FE_BATCH_FIELD = 1
+FE_BATCH_DLDP = 2
+FE_BOUT_V, FE_BOUT_FIELD, FE_BOUT_DLDP, FE_BOUT_LOSS, FE_BOUT_GV = range(5)
+FE_BT_ITEMS_PER_WG = 1
+FE_BT_DG_ITEMS_PER_WG = 2
+FE_BT_CLASS_MASK = 3
+FE_BT_PPL0 = 10
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -51,6 +58,20 @@ SIGNATURES = {
     "fe_esp_backward": [_vp, _vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, _vp, _vp],
     "fe_table_export": [_vp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip,
                         _dp, _dp, _dp],
+    "fe_batch_create": [_vp, C.c_int, C.POINTER(_vp), _vp, C.POINTER(C.c_int64), C.c_int, _vp,
+                        C.POINTER(_vp)],
+    "fe_batch_free": [_vp],
+    "fe_batch_set_points": [_vp, _vp, C.c_int, _vp],
+    "fe_batch_set_loss": [_vp, _vp, _vp, C.c_int, _vp],
+    "fe_batch_set_tuning": [_vp, C.c_int, C.c_int64],
+    "fe_batch_forward": [_vp, C.c_int, C.c_int, _vp],
+    "fe_batch_loss": [_vp, _vp],
+    "fe_batch_backward": [_vp, _vp, C.c_int, _vp],
+    "fe_batch_step": [_vp, C.c_int, C.c_int, _vp],
+    "fe_batch_capture": [_vp, C.c_int, C.c_int],
+    "fe_batch_launch": [_vp, _vp],
+    "fe_batch_read": [_vp, C.c_int, C.c_int, _vp, _vp],
+    "fe_batch_info": [_vp, C.POINTER(C.c_int64), C.c_int],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 

@@ -136,3 +136,86 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
     d[k] = dict(best, trials=len(log))
     _store(d)
     return dict(best=best, all=log, key=k)
+
+
+# ---------------------------------------------------------------------------
+# batched conformers
+# ---------------------------------------------------------------------------
+ITEMS_CHOICES = (2, 4, 8, 16, 32)
+DG_ITEMS_CHOICES = (2, 4, 8, 16)
+
+
+def tune_batch(batch, precision="mixed", field: bool = True, dldp: bool = True, reps: int = 2,
+               verbose: bool = False) -> dict:
+    """Tile shape of a batch (``fe_batch_set_tuning``), timed with HIP events.
+
+    1. points per lane of each class, timing that class's persistent launch
+       alone (class mask), since the per-class launches are independent;
+    2. forward work items per resident workgroup (item size: load balance
+       against partial-row traffic), on the whole forward;
+    3. dL/dG work items per resident workgroup, on the backward.
+    The batch keeps the best shape; the graph must be captured afterwards.
+    """
+    import torch
+    from .batch import _prec
+    dev = torch.device("cuda", batch._ctx.device)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    prec = _prec(precision)
+    present = [L for L in range(NCLS) if any(t.info()["pairs_per_L"][L] for t in batch.tables)]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    fwd = lambda: batch.forward(prec, field=field, stream=sh)  # noqa: E731
+    log = []
+    ppl = {}
+    for L in present:
+        batch.set_tuning(_lib.FE_BT_CLASS_MASK, 1 << L)
+        best = (float("inf"), 2)
+        for v in PPL_CHOICES:
+            batch.set_tuning(_lib.FE_BT_PPL0 + L, v)
+            ms = timed(fwd)
+            log.append(dict(cls=L, ppl=v, ms=ms))
+            if verbose:
+                print(f"  tune batch class {L} ppl={v}: {ms:.3f} ms", flush=True)
+            best = min(best, (ms, v))
+        ppl[L] = best[1]
+        batch.set_tuning(_lib.FE_BT_PPL0 + L, best[1])
+    batch.set_tuning(_lib.FE_BT_CLASS_MASK, (1 << NCLS) - 1)
+    best_k = (float("inf"), 8)
+    for k in ITEMS_CHOICES:
+        batch.set_tuning(_lib.FE_BT_ITEMS_PER_WG, k)
+        ms = timed(fwd)
+        log.append(dict(items_per_wg=k, ms=ms))
+        if verbose:
+            print(f"  tune batch items/WG={k}: {ms:.3f} ms", flush=True)
+        best_k = min(best_k, (ms, k))
+    batch.set_tuning(_lib.FE_BT_ITEMS_PER_WG, best_k[1])
+    best_kd = (None, None)
+    if dldp:
+        batch.forward(prec, field=field, stream=sh)
+        batch.loss(stream=sh)
+        best_kd = (float("inf"), 8)
+        for k in DG_ITEMS_CHOICES:
+            batch.set_tuning(_lib.FE_BT_DG_ITEMS_PER_WG, k)
+            ms = timed(lambda: batch.backward(stream=sh))
+            log.append(dict(dg_items_per_wg=k, ms=ms))
+            if verbose:
+                print(f"  tune batch dG items/WG={k}: {ms:.3f} ms", flush=True)
+            best_kd = min(best_kd, (ms, k))
+        batch.set_tuning(_lib.FE_BT_DG_ITEMS_PER_WG, best_kd[1])
+    return dict(ppl_L=[ppl.get(L, 2) for L in range(NCLS)], items_per_wg=best_k[1],
+                fwd_ms=best_k[0], dg_items_per_wg=best_kd[1], bwd_ms=best_kd[0],
+                trials=len(log), all=log)

@@ -1,0 +1,175 @@
+// Adjoint device code shared by the single-table and the batched backward.
+//
+// fe_dg_block<L>: one lane per pair of class L; the points (x, y, z, gV) of
+//   [q0, q1) are wave-uniform scalar loads, so there is no cross-lane
+//   reduction.  Writes out[k * stride + pair] = -K sum_i gV_i R_i^a F_m(T_i).
+// fe_dg_to_dp_sp: dL/dG -> dL/dP for one shell pair (64 lanes):
+//   d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t], d blk_{mu nu} = c_i c_j
+//   sum_t d etab_t Ex Ey Ez; both triangles of P receive d blk because the
+//   block is folded P_ij + P_ji^T (fastesp/esp.py:185-187).
+#pragma once
+
+#include "fe_internal.h"
+#include "fe_boys.h"
+#include "fe_hermite.h"
+#include "fe_lclass.inc"
+
+struct DPArgs {
+  const int* sh_l; const int* sh_fn0; const int* sh_nfn; const int* sh_prim0; const int* sh_nprim;
+  const double* sh_c; const double* pexp; const double* pcoef;
+  const int* fn_lx; const int* fn_ly; const int* fn_lz;
+  const int* sp_i; const int* sp_j; const int* sp_slot;
+  const int* slot2kept[FE_LMAX + 1];
+  const double* dG[FE_LMAX + 1];
+  int n_func;
+  double* dP;
+  LTabDev lt;
+};
+
+template <int L>
+__device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, const int n_pairs,
+                                            const int pair, const double* __restrict__ pw,
+                                            const int q0, const int q1, const double* s_tab,
+                                            double* __restrict__ out, const int stride) {
+  constexpr int S = FeL<L>::STRIDE, NT = FeL<L>::NT;
+  const int ip = min(pair, n_pairs - 1);
+  const double* r = rec + (size_t)ip * S;
+  const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+  const double hp = r[6], cF = r[7];
+  double acc[NT];
+#pragma unroll
+  for (int k = 0; k < NT; ++k) acc[k] = 0.0;
+  auto point = [&](const double x, const double y, const double z, double w) {
+    const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
+    const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
+    double F[L + 1];
+    if (!__any(p * r2 < fe_tfar(L))) {
+      const double ri = rsqrt(r2);
+      fe_boys_far_scaled<L>(hp * (ri * ri), F);
+      w *= cF * ri;
+    } else {
+      fe_boys<L>(p * r2, F, s_tab);
+    }
+    fe_dg_accum<L>(Rx, Ry, Rz, F, w, acc);
+  };
+  // two wave-uniform points per iteration so their scalar loads overlap; the
+  // odd tail point repeats the last point with zero weight
+  for (int q = q0; q < q1; q += 2) {
+    const double* __restrict__ c0 = pw + 4 * (size_t)q;
+    const double* __restrict__ c1 = pw + 4 * (size_t)min(q + 1, q1 - 1);
+    const double x0 = c0[0], y0 = c0[1], z0 = c0[2], w0 = c0[3];
+    const double x1 = c1[0], y1 = c1[1], z1 = c1[2];
+    const double w1 = (q + 1 < q1) ? c1[3] : 0.0;
+    point(x0, y0, z0, w0);
+    point(x1, y1, z1, w1);
+  }
+  if (pair < n_pairs) {
+#pragma unroll
+    for (int k = 0; k < NT; ++k) out[(size_t)k * stride + pair] = -K * acc[k];
+  }
+}
+
+// One shell pair `sp` of the table described by `a`; 64 lanes.  The caller
+// synchronises the workgroup before reusing the shared arrays for another
+// shell pair.
+__device__ __forceinline__ void fe_dg_to_dp_sp(const DPArgs& a, const int sp) {
+  __shared__ int s_ci[10][3], s_cj[10][3];
+  __shared__ double s_E[3][4 * 4 * 7];
+  __shared__ double s_det[84];
+  const int tid = threadIdx.x;
+  const int si = a.sp_i[sp], sj = a.sp_j[sp];
+  const int li = a.sh_l[si], lj = a.sh_l[sj];
+  const int L = li + lj, T = L + 1;
+  const int ni = a.sh_nfn[si], nj = a.sh_nfn[sj];
+  const int fi = a.sh_fn0[si], fj = a.sh_fn0[sj];
+  const int npi = a.sh_nprim[si], npj = a.sh_nprim[sj];
+  const int pi0 = a.sh_prim0[si], pj0 = a.sh_prim0[sj];
+  const int slot0 = a.sp_slot[sp];
+  const int NT = a.lt.key_off[L + 1] - a.lt.key_off[L];
+  const int nT = a.lt.mono_off[L + 1] - a.lt.mono_off[L];
+  const int* tl = a.lt.tlist + 3 * a.lt.mono_off[L];
+  const int* cp = a.lt.m0t_colptr + a.lt.mono_off[L];
+  const int k0 = a.lt.key_off[L];
+  const int* s2k = a.slot2kept[L];
+  const double* dGL = a.dG[L];
+  if (tid < ni) {
+    s_ci[tid][0] = a.fn_lx[fi + tid];
+    s_ci[tid][1] = a.fn_ly[fi + tid];
+    s_ci[tid][2] = a.fn_lz[fi + tid];
+  }
+  if (tid < nj) {
+    s_cj[tid][0] = a.fn_lx[fj + tid];
+    s_cj[tid][1] = a.fn_ly[fj + tid];
+    s_cj[tid][2] = a.fn_lz[fj + tid];
+  }
+  __syncthreads();
+  const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
+  const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
+  double acc0 = 0.0, acc1 = 0.0;  // entries e = tid and tid + 64 of the ni x nj block
+  for (int ii = 0; ii < npi; ++ii)
+    for (int jj = 0; jj < npj; ++jj) {
+      const int kept = s2k[slot0 + ii * npj + jj];
+      if (kept < 0) continue;
+      const double al = a.pexp[pi0 + ii], be = a.pexp[pj0 + jj];
+      const double p = al + be;
+      const double cc = a.pcoef[pi0 + ii] * a.pcoef[pj0 + jj];
+      const double Px = (al * Ax + be * Bx) / p;
+      const double Py = (al * Ay + be * By) / p;
+      const double Pz = (al * Az + be * Bz) / p;
+      if (tid < 3) {
+        const double xa = (tid == 0 ? Px - Ax : tid == 1 ? Py - Ay : Pz - Az);
+        const double xb = (tid == 0 ? Px - Bx : tid == 1 ? Py - By : Pz - Bz);
+        fe_e1d_table(s_E[tid], li, lj, xa, xb, p);
+      }
+      const double tp = 2.0 * p, cf = 2.0 * M_PI / p;
+      const double* g = dGL + (size_t)kept * NT;
+      for (int t = tid; t < nT; t += 64) {
+        double s = 0.0;
+        for (int q = cp[t]; q < cp[t + 1]; ++q) {
+          const int k = a.lt.m0t_k[q];
+          double pw = 1.0;
+          for (int m = 0; m < a.lt.key_m[k0 + k]; ++m) pw *= tp;
+          s += a.lt.m0t_val[q] * pw * g[k];
+        }
+        s_det[t] = cf * s;
+      }
+      __syncthreads();
+      for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
+        const int mi = e / nj, mj = e - mi * nj;
+        const int ex0 = (s_ci[mi][0] * (lj + 1) + s_cj[mj][0]) * T;
+        const int ey0 = (s_ci[mi][1] * (lj + 1) + s_cj[mj][1]) * T;
+        const int ez0 = (s_ci[mi][2] * (lj + 1) + s_cj[mj][2]) * T;
+        double s = 0.0;
+        for (int t = 0; t < nT; ++t) {
+          const int tx = tl[3 * t], ty = tl[3 * t + 1], tz = tl[3 * t + 2];
+          s += s_det[t] * s_E[0][ex0 + tx] * s_E[1][ey0 + ty] * s_E[2][ez0 + tz];
+        }
+        if (slot == 0) acc0 = fma(cc, s, acc0);
+        else acc1 = fma(cc, s, acc1);
+      }
+      __syncthreads();
+    }
+  for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
+    const int mi = e / nj, mj = e - mi * nj;
+    const double v = slot == 0 ? acc0 : acc1;
+    a.dP[(size_t)(fi + mi) * a.n_func + (fj + mj)] = v;
+    if (si != sj) a.dP[(size_t)(fj + mj) * a.n_func + (fi + mi)] = v;
+  }
+}
+
+// DPArgs of a table whose dG of class L starts at dG + dg_off[L]
+inline DPArgs fe_dp_args(const fe_table* t, double* dG, const size_t* dg_off, double* dP) {
+  DPArgs pa{};
+  pa.sh_l = t->b.l; pa.sh_fn0 = t->b.fn0; pa.sh_nfn = t->b.nfn; pa.sh_prim0 = t->b.prim0;
+  pa.sh_nprim = t->b.nprim; pa.sh_c = t->b.c; pa.pexp = t->b.pexp; pa.pcoef = t->b.pcoef;
+  pa.fn_lx = t->b.fn_lx; pa.fn_ly = t->b.fn_ly; pa.fn_lz = t->b.fn_lz;
+  pa.sp_i = t->b.sp_i; pa.sp_j = t->b.sp_j; pa.sp_slot = t->b.sp_slot;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    pa.slot2kept[L] = t->d_slot2kept[L];
+    pa.dG[L] = dG + dg_off[L];
+  }
+  pa.n_func = t->n_func;
+  pa.dP = dP;
+  pa.lt = t->ctx->ltab;
+  return pa;
+}

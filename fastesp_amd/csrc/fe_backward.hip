@@ -23,7 +23,8 @@
 #include "fe_boys.h"
 #include "fe_lclass.inc"
 #include "fe_internal.h"
-#include "fe_hermite.h"
+#include "fe_adjoint.h"
+#include "fe_reduce.h"
 
 __global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict__ partial,
                                                          const int n_rows, const int ld,
@@ -37,44 +38,9 @@ __global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict
                                                          double* __restrict__ dVdC) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_pts) return;
-  // two independent chains per quantity over the rows (fixed order)
-  double e = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, e1 = 0.0, gx1 = 0.0, gy1 = 0.0, gz1 = 0.0;
-  int r = 0;
-  for (; r + 1 < n_rows; r += 2) {
-    const size_t o = (size_t)r * ld + i, o1 = o + ld;
-    e += partial[o];
-    gx += partial[plane + o];
-    gy += partial[2 * plane + o];
-    gz += partial[3 * plane + o];
-    e1 += partial[o1];
-    gx1 += partial[plane + o1];
-    gy1 += partial[2 * plane + o1];
-    gz1 += partial[3 * plane + o1];
-  }
-  if (r < n_rows) {
-    const size_t o = (size_t)r * ld + i;
-    e += partial[o];
-    gx += partial[plane + o];
-    gy += partial[2 * plane + o];
-    gz += partial[3 * plane + o];
-  }
-  e += e1;
-  gx += gx1;
-  gy += gy1;
-  gz += gz1;
-  const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-  double nuc = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
-  for (int a = 0; a < n_atoms; ++a) {
-    const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double ir = 1.0 / sqrt(r2);
-    const double zr = atoms[4 * a + 3] * ir;
-    nuc += zr;
-    const double w = zr * ir * ir;
-    nx -= w * dx;
-    ny -= w * dy;
-    nz -= w * dz;
-  }
+  double e, gx, gy, gz, nuc, nx, ny, nz;
+  fe_sum_rows4(partial + i, n_rows, ld, plane, e, gx, gy, gz);
+  fe_nuclear_grad(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], atoms, n_atoms, nuc, nx, ny, nz);
   const int o = perm ? perm[i] : i;
   V[o] = nuc - e;
   dVdC[3 * o] = nx - gx;
@@ -154,46 +120,9 @@ __global__ __launch_bounds__(256) void k_dG(const double* __restrict__ rec, cons
   __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
   for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += 256) s_tab[i] = boys_tab[i];
   __syncthreads();
-  constexpr int S = FeL<L>::STRIDE, NT = FeL<L>::NT;
-  const int pair = blockIdx.x * 256 + threadIdx.x;
-  const int ip = min(pair, n_pairs - 1);
-  const double* r = rec + (size_t)ip * S;
-  const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
-  const double hp = r[6], cF = r[7];
-  double acc[NT];
-#pragma unroll
-  for (int k = 0; k < NT; ++k) acc[k] = 0.0;
   const int q0 = blockIdx.y * ppc;
-  const int q1 = min(n_pts, q0 + ppc);
-  auto point = [&](const double x, const double y, const double z, double w) {
-    const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
-    const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
-    double F[L + 1];
-    if (!__any(p * r2 < fe_tfar(L))) {
-      const double ri = rsqrt(r2);
-      fe_boys_far_scaled<L>(hp * (ri * ri), F);
-      w *= cF * ri;
-    } else {
-      fe_boys<L>(p * r2, F, s_tab);
-    }
-    fe_dg_accum<L>(Rx, Ry, Rz, F, w, acc);
-  };
-  // two wave-uniform points per iteration so their scalar loads overlap; the
-  // odd tail point repeats the last point with zero weight
-  for (int q = q0; q < q1; q += 2) {
-    const double* __restrict__ c0 = pw + 4 * (size_t)q;
-    const double* __restrict__ c1 = pw + 4 * (size_t)min(q + 1, q1 - 1);
-    const double x0 = c0[0], y0 = c0[1], z0 = c0[2], w0 = c0[3];
-    const double x1 = c1[0], y1 = c1[1], z1 = c1[2];
-    const double w1 = (q + 1 < q1) ? c1[3] : 0.0;
-    point(x0, y0, z0, w0);
-    point(x1, y1, z1, w1);
-  }
-  if (pair < n_pairs) {
-    double* o = partial + (size_t)blockIdx.y * NT * n_pairs + pair;
-#pragma unroll
-    for (int k = 0; k < NT; ++k) o[(size_t)k * n_pairs] = -K * acc[k];
-  }
+  fe_dg_block<L>(rec, n_pairs, blockIdx.x * 256 + threadIdx.x, pw, q0, min(n_pts, q0 + ppc),
+                 s_tab, partial + (size_t)blockIdx.y * FeL<L>::NT * n_pairs, n_pairs);
 }
 
 __global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, const int64_t n,
@@ -269,104 +198,7 @@ static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, in
 // ---------------------------------------------------------------------------
 // dL/dG -> dL/dP, one workgroup (64 lanes) per shell pair
 // ---------------------------------------------------------------------------
-struct DPArgs {
-  const int* sh_l; const int* sh_fn0; const int* sh_nfn; const int* sh_prim0; const int* sh_nprim;
-  const double* sh_c; const double* pexp; const double* pcoef;
-  const int* fn_lx; const int* fn_ly; const int* fn_lz;
-  const int* sp_i; const int* sp_j; const int* sp_slot;
-  const int* slot2kept[FE_LMAX + 1];
-  const double* dG[FE_LMAX + 1];
-  int n_func;
-  double* dP;
-  LTabDev lt;
-};
-
-__global__ __launch_bounds__(64) void k_dg_to_dp(const DPArgs a) {
-  __shared__ int s_ci[10][3], s_cj[10][3];
-  __shared__ double s_E[3][4 * 4 * 7];
-  __shared__ double s_det[84];
-  const int sp = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int si = a.sp_i[sp], sj = a.sp_j[sp];
-  const int li = a.sh_l[si], lj = a.sh_l[sj];
-  const int L = li + lj, T = L + 1;
-  const int ni = a.sh_nfn[si], nj = a.sh_nfn[sj];
-  const int fi = a.sh_fn0[si], fj = a.sh_fn0[sj];
-  const int npi = a.sh_nprim[si], npj = a.sh_nprim[sj];
-  const int pi0 = a.sh_prim0[si], pj0 = a.sh_prim0[sj];
-  const int slot0 = a.sp_slot[sp];
-  const int NT = a.lt.key_off[L + 1] - a.lt.key_off[L];
-  const int nT = a.lt.mono_off[L + 1] - a.lt.mono_off[L];
-  const int* tl = a.lt.tlist + 3 * a.lt.mono_off[L];
-  const int* cp = a.lt.m0t_colptr + a.lt.mono_off[L];
-  const int k0 = a.lt.key_off[L];
-  const int* s2k = a.slot2kept[L];
-  const double* dGL = a.dG[L];
-  if (tid < ni) {
-    s_ci[tid][0] = a.fn_lx[fi + tid];
-    s_ci[tid][1] = a.fn_ly[fi + tid];
-    s_ci[tid][2] = a.fn_lz[fi + tid];
-  }
-  if (tid < nj) {
-    s_cj[tid][0] = a.fn_lx[fj + tid];
-    s_cj[tid][1] = a.fn_ly[fj + tid];
-    s_cj[tid][2] = a.fn_lz[fj + tid];
-  }
-  __syncthreads();
-  const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
-  const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
-  double acc0 = 0.0, acc1 = 0.0;  // entries e = tid and tid + 64 of the ni x nj block
-  for (int ii = 0; ii < npi; ++ii)
-    for (int jj = 0; jj < npj; ++jj) {
-      const int kept = s2k[slot0 + ii * npj + jj];
-      if (kept < 0) continue;
-      const double al = a.pexp[pi0 + ii], be = a.pexp[pj0 + jj];
-      const double p = al + be;
-      const double cc = a.pcoef[pi0 + ii] * a.pcoef[pj0 + jj];
-      const double Px = (al * Ax + be * Bx) / p;
-      const double Py = (al * Ay + be * By) / p;
-      const double Pz = (al * Az + be * Bz) / p;
-      if (tid < 3) {
-        const double xa = (tid == 0 ? Px - Ax : tid == 1 ? Py - Ay : Pz - Az);
-        const double xb = (tid == 0 ? Px - Bx : tid == 1 ? Py - By : Pz - Bz);
-        fe_e1d_table(s_E[tid], li, lj, xa, xb, p);
-      }
-      // d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t]
-      const double tp = 2.0 * p, cf = 2.0 * M_PI / p;
-      const double* g = dGL + (size_t)kept * NT;
-      for (int t = tid; t < nT; t += 64) {
-        double s = 0.0;
-        for (int q = cp[t]; q < cp[t + 1]; ++q) {
-          const int k = a.lt.m0t_k[q];
-          double pw = 1.0;
-          for (int m = 0; m < a.lt.key_m[k0 + k]; ++m) pw *= tp;
-          s += a.lt.m0t_val[q] * pw * g[k];
-        }
-        s_det[t] = cf * s;
-      }
-      __syncthreads();
-      for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
-        const int mi = e / nj, mj = e - mi * nj;
-        const int ex0 = (s_ci[mi][0] * (lj + 1) + s_cj[mj][0]) * T;
-        const int ey0 = (s_ci[mi][1] * (lj + 1) + s_cj[mj][1]) * T;
-        const int ez0 = (s_ci[mi][2] * (lj + 1) + s_cj[mj][2]) * T;
-        double s = 0.0;
-        for (int t = 0; t < nT; ++t) {
-          const int tx = tl[3 * t], ty = tl[3 * t + 1], tz = tl[3 * t + 2];
-          s += s_det[t] * s_E[0][ex0 + tx] * s_E[1][ey0 + ty] * s_E[2][ez0 + tz];
-        }
-        if (slot == 0) acc0 = fma(cc, s, acc0);
-        else acc1 = fma(cc, s, acc1);
-      }
-      __syncthreads();
-    }
-  for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
-    const int mi = e / nj, mj = e - mi * nj;
-    const double v = slot == 0 ? acc0 : acc1;
-    a.dP[(size_t)(fi + mi) * a.n_func + (fj + mj)] = v;
-    if (si != sj) a.dP[(size_t)(fj + mj) * a.n_func + (fi + mi)] = v;
-  }
-}
+__global__ __launch_bounds__(64) void k_dg_to_dp(const DPArgs a) { fe_dg_to_dp_sp(a, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const double* gV,
@@ -446,18 +278,7 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
       if (rc) return rc;
       d_dP = (double*)c->ws[FE_WS_OUT];
     }
-    DPArgs pa{};
-    pa.sh_l = t->b.l; pa.sh_fn0 = t->b.fn0; pa.sh_nfn = t->b.nfn; pa.sh_prim0 = t->b.prim0;
-    pa.sh_nprim = t->b.nprim; pa.sh_c = t->b.c; pa.pexp = t->b.pexp; pa.pcoef = t->b.pcoef;
-    pa.fn_lx = t->b.fn_lx; pa.fn_ly = t->b.fn_ly; pa.fn_lz = t->b.fn_lz;
-    pa.sp_i = t->b.sp_i; pa.sp_j = t->b.sp_j; pa.sp_slot = t->b.sp_slot;
-    for (int L = 0; L <= FE_LMAX; ++L) {
-      pa.slot2kept[L] = t->d_slot2kept[L];
-      pa.dG[L] = dG + d.dg_off[L];
-    }
-    pa.n_func = t->n_func;
-    pa.dP = d_dP;
-    pa.lt = c->ltab;
+    const DPArgs pa = fe_dp_args(t, dG, d.dg_off, d_dP);
     if (t->n_sp > 0) hipLaunchKernelGGL(k_dg_to_dp, dim3(t->n_sp), dim3(64), 0, st, pa);
     FE_HIPCHK(hipGetLastError());
   }

@@ -35,6 +35,7 @@
 #include "fe_boys.h"
 #include "fe_lclass.inc"
 #include "fe_internal.h"
+#include "fe_reduce.h"
 
 #define FE_VERSION 100
 
@@ -487,25 +488,10 @@ __global__ __launch_bounds__(64) void k_esp_reduce(const double* __restrict__ pa
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i >= n_pts) return;
   const int o = perm ? perm[i] : i;
-  // four independent chains over the rows (fixed order: bitwise reproducible)
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
-  int r = 0;
-  for (; r + 3 < n_rows; r += 4) {
-    e0 += partial[(size_t)r * ld + i];
-    e1 += partial[(size_t)(r + 1) * ld + i];
-    e2 += partial[(size_t)(r + 2) * ld + i];
-    e3 += partial[(size_t)(r + 3) * ld + i];
-  }
-  for (; r < n_rows; ++r) e0 += partial[(size_t)r * ld + i];
-  const double e = (e0 + e1) + (e2 + e3);
-  double nuc = 0.0;
-  if (parts & FE_PART_NUCLEAR) {
-    const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    for (int a = 0; a < n_atoms; ++a) {
-      const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
-      nuc += atoms[4 * a + 3] / sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
-    }
-  }
+  const double e = fe_sum_rows(partial + i, n_rows, ld);
+  const double nuc = (parts & FE_PART_NUCLEAR)
+                         ? fe_nuclear(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], atoms, n_atoms)
+                         : 0.0;
   V[o] = (parts & FE_PART_ELECTRON) ? nuc - e : nuc;
 }
 

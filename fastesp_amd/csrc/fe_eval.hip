@@ -26,22 +26,30 @@ __global__ __launch_bounds__(FE_WG) void k_eval(const double* __restrict__ rec,
                                      partial + (size_t)(row0 + chunk) * ld + base, plane);
 }
 
+// Persistent: each workgroup takes the next item of the queue until it is
+// drained (every workgroup reaches the exit test after at most n_items + 1
+// increments of the counter).
 template <int L, int PPL, bool GRAD, bool MIXED>
 __global__ __launch_bounds__(FE_WG) void k_batch_eval(const FeBatchItem* __restrict__ items,
-                                                      const int n_items,
-                                                      const double* __restrict__ rec,
-                                                      const float* __restrict__ g32,
+                                                      const int n_items, int* __restrict__ counter,
+                                                      const double* const* __restrict__ recs,
+                                                      const float* const* __restrict__ g32s,
                                                       const double* __restrict__ pts,
                                                       const double* __restrict__ boys_tab,
                                                       double* __restrict__ partial,
                                                       const size_t plane, const float tau) {
   __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
+  __shared__ int s_it;
   for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
-  __syncthreads();
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_it = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_it);
+    if (it >= n_items) break;
     const FeBatchItem b = items[it];
-    fe_eval_block<L, PPL, GRAD, MIXED>(rec, g32, b.p0, b.p1, pts, b.base, b.end, s_tab, tau,
-                                       partial + b.out, plane);
+    fe_eval_block<L, PPL, GRAD, MIXED>(recs[b.conf], MIXED ? g32s[b.conf] : nullptr, b.p0, b.p1,
+                                       pts, b.base, b.end, s_tab, tau, partial + b.out, plane);
   }
 }
 
@@ -78,48 +86,49 @@ int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, dou
   return FE_OK;
 }
 
-template <int L, int PPL>
-static void launch_batch_LP(const FeBatchItem* items, int n_items, int grid, const double* rec,
-                            const float* g32, const double* pts, const double* boys,
-                            double* partial, size_t plane, bool grad, bool mixed, float tau,
-                            hipStream_t st) {
-  const int M = grad ? L + 1 : L;
-  const double* tab = boys + (size_t)M * FE_BOYS_NR * FE_BOYS_COLS;
-#define FE_K(G, X)                                                                            \
-  hipLaunchKernelGGL((k_batch_eval<L, PPL, G, X>), dim3(grid), dim3(FE_WG), 0, st, items,     \
-                     n_items, rec, g32, pts, tab, partial, plane, tau)
-  if (grad) {
-    if (mixed) FE_K(true, true); else FE_K(true, false);
-  } else {
-    if (mixed) FE_K(false, true); else FE_K(false, false);
+template <int L, int PPL, bool G, bool X>
+static int batch_LPGX(const FeBatchEvalArgs* a, hipStream_t st) {
+  if (!a) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_eval<L, PPL, G, X>, FE_WG, 0) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    return nb;
   }
-#undef FE_K
+  const int M = G ? L + 1 : L;
+  const double* tab = a->boys + (size_t)M * FE_BOYS_NR * FE_BOYS_COLS;
+  hipLaunchKernelGGL((k_batch_eval<L, PPL, G, X>), dim3(a->grid), dim3(FE_WG), 0, st, a->items,
+                     a->n_items, a->counter, a->recs, a->g32s, a->pts, tab, a->partial, a->plane,
+                     a->tau);
+  return 0;
+}
+
+template <int L, int PPL>
+static int batch_LP(bool grad, bool mixed, const FeBatchEvalArgs* a, hipStream_t st) {
+  if (grad) return mixed ? batch_LPGX<L, PPL, true, true>(a, st) : batch_LPGX<L, PPL, true, false>(a, st);
+  return mixed ? batch_LPGX<L, PPL, false, true>(a, st) : batch_LPGX<L, PPL, false, false>(a, st);
 }
 
 template <int L>
-static void launch_batch_L(int ppl, const FeBatchItem* items, int n_items, int grid,
-                           const double* rec, const float* g32, const double* pts,
-                           const double* boys, double* partial, size_t plane, bool grad,
-                           bool mixed, float tau, hipStream_t st) {
-  if (ppl == 2)
-    launch_batch_LP<L, 2>(items, n_items, grid, rec, g32, pts, boys, partial, plane, grad, mixed,
-                          tau, st);
-  else
-    launch_batch_LP<L, 1>(items, n_items, grid, rec, g32, pts, boys, partial, plane, grad, mixed,
-                          tau, st);
+static int batch_L(int ppl, bool grad, bool mixed, const FeBatchEvalArgs* a, hipStream_t st) {
+  switch (ppl) {
+    case 4: return batch_LP<L, 4>(grad, mixed, a, st);
+    case 2: return batch_LP<L, 2>(grad, mixed, a, st);
+    default: return batch_LP<L, 1>(grad, mixed, a, st);
+  }
 }
 
-void fe_launch_batch_eval(int L, int ppl, const FeBatchItem* items, int n_items, int grid,
-                          const double* rec, const float* g32, const double* pts,
-                          const double* boys, double* partial, size_t plane, bool grad,
-                          bool mixed, float tau, hipStream_t st) {
+int fe_batch_eval(int L, int ppl, bool grad, bool mixed, const FeBatchEvalArgs* a,
+                  hipStream_t st) {
   switch (L) {
-#define FE_C(l)                                                                               \
-  case l:                                                                                     \
-    launch_batch_L<l>(ppl, items, n_items, grid, rec, g32, pts, boys, partial, plane, grad,   \
-                      mixed, tau, st);                                                        \
-    break;
-    FE_C(0) FE_C(1) FE_C(2) FE_C(3) FE_C(4) FE_C(5) FE_C(6)
-#undef FE_C
+    case 0: return batch_L<0>(ppl, grad, mixed, a, st);
+    case 1: return batch_L<1>(ppl, grad, mixed, a, st);
+    case 2: return batch_L<2>(ppl, grad, mixed, a, st);
+    case 3: return batch_L<3>(ppl, grad, mixed, a, st);
+    case 4: return batch_L<4>(ppl, grad, mixed, a, st);
+    case 5: return batch_L<5>(ppl, grad, mixed, a, st);
+    default: return batch_L<6>(ppl, grad, mixed, a, st);
   }
 }

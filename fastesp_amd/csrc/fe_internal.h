@@ -142,19 +142,32 @@ int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipSt
                     const double** d_pts);
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
                    const double** d_sorted, const int32_t** d_perm);
-// one work item of the batched (persistent) evaluation: pairs [p0, p1) of one
-// class against the points [base, min(base + PPL*FE_WG, end)) of one conformer;
-// partial sums go to partial[out + ...] (planes `plane` apart for the field)
+// one work item of the batched evaluation: pairs [p0, p1) of class L of
+// conformer `conf` against the sorted points [base, min(base + PPL*FE_WG, end));
+// the partial sums go to partial[out + ...] (planes `plane` apart for the field)
 struct FeBatchItem {
   int64_t out;
-  int p0, p1, base, end;
-  int pad0, pad1;
+  int conf, p0, p1, base, end, pad;
+};
+
+struct FeBatchEvalArgs {
+  const FeBatchItem* items;
+  int n_items;
+  int* counter;                 // work-queue head, zero before the launch
+  const double* const* recs;    // per conformer: class-L records
+  const float* const* g32s;     // per conformer: FP32 coefficients (mixed)
+  const double* pts;            // sorted points of all conformers
+  const double* boys;           // Boys grids of the context (all orders)
+  double* partial;
+  size_t plane;
+  float tau;
+  int grid;
 };
 
 int fe_ensure_g32(fe_table* t, hipStream_t st);
 int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,
                    size_t plane, bool grad, bool mixed, hipStream_t st);
-void fe_launch_batch_eval(int L, int ppl, const FeBatchItem* items, int n_items, int grid,
-                          const double* rec, const float* g32, const double* pts,
-                          const double* boys, double* partial, size_t plane, bool grad,
-                          bool mixed, float tau, hipStream_t st);
+// launches the batched evaluation of class L (a != nullptr), or returns its
+// resident workgroups per CU (a == nullptr)
+int fe_batch_eval(int L, int ppl, bool grad, bool mixed, const FeBatchEvalArgs* a,
+                  hipStream_t st);

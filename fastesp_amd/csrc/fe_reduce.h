@@ -1,0 +1,83 @@
+// Per-point combination of the partial rows and the nuclear term, shared by
+// the single-table and the batched reductions.  Rows are added in a fixed
+// order with independent chains, so a result is bitwise reproducible for a
+// given launch shape.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// sum_r p[r * ld], four chains
+__device__ __forceinline__ double fe_sum_rows(const double* __restrict__ p, const int n_rows,
+                                              const size_t ld) {
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+  int r = 0;
+  for (; r + 3 < n_rows; r += 4) {
+    e0 += p[(size_t)r * ld];
+    e1 += p[(size_t)(r + 1) * ld];
+    e2 += p[(size_t)(r + 2) * ld];
+    e3 += p[(size_t)(r + 3) * ld];
+  }
+  for (; r < n_rows; ++r) e0 += p[(size_t)r * ld];
+  return (e0 + e1) + (e2 + e3);
+}
+
+// value and the three field planes (`plane` apart), two chains each
+__device__ __forceinline__ void fe_sum_rows4(const double* __restrict__ p, const int n_rows,
+                                             const size_t ld, const size_t plane, double& e,
+                                             double& gx, double& gy, double& gz) {
+  double e1 = 0.0, gx1 = 0.0, gy1 = 0.0, gz1 = 0.0;
+  e = gx = gy = gz = 0.0;
+  int r = 0;
+  for (; r + 1 < n_rows; r += 2) {
+    const size_t o = (size_t)r * ld, o1 = o + ld;
+    e += p[o];
+    gx += p[plane + o];
+    gy += p[2 * plane + o];
+    gz += p[3 * plane + o];
+    e1 += p[o1];
+    gx1 += p[plane + o1];
+    gy1 += p[2 * plane + o1];
+    gz1 += p[3 * plane + o1];
+  }
+  if (r < n_rows) {
+    const size_t o = (size_t)r * ld;
+    e += p[o];
+    gx += p[plane + o];
+    gy += p[2 * plane + o];
+    gz += p[3 * plane + o];
+  }
+  e += e1;
+  gx += gx1;
+  gy += gy1;
+  gz += gz1;
+}
+
+// sum_A Z_A / |C - R_A|; atoms as (x, y, z, Z)
+__device__ __forceinline__ double fe_nuclear(const double x, const double y, const double z,
+                                             const double* __restrict__ atoms, const int n_atoms) {
+  double nuc = 0.0;
+  for (int a = 0; a < n_atoms; ++a) {
+    const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
+    nuc += atoms[4 * a + 3] / sqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
+  }
+  return nuc;
+}
+
+// nuclear potential and its gradient -sum_A Z_A (C - R_A) / |C - R_A|^3
+__device__ __forceinline__ void fe_nuclear_grad(const double x, const double y, const double z,
+                                                const double* __restrict__ atoms,
+                                                const int n_atoms, double& nuc, double& nx,
+                                                double& ny, double& nz) {
+  nuc = nx = ny = nz = 0.0;
+  for (int a = 0; a < n_atoms; ++a) {
+    const double dx = x - atoms[4 * a], dy = y - atoms[4 * a + 1], dz = z - atoms[4 * a + 2];
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double ir = 1.0 / sqrt(r2);
+    const double zr = atoms[4 * a + 3] * ir;
+    nuc += zr;
+    const double w = zr * ir * ir;
+    nx -= w * dx;
+    ny -= w * dy;
+    nz -= w * dz;
+  }
+}

@@ -169,6 +169,55 @@ int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
  * key 2: use per-tile screening (0/1) -- reserved. */
 int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
 
+/* ---- batched conformers ------------------------------------------------
+ * No counterpart in the reference (it evaluates one table per esp_gpu
+ * process, fastesp/cli.py:143-163).  A batch references n_conf tables of one
+ * context (they must outlive it) and owns the concatenated points (conformer
+ * c holds rows [sum_{c'<c} n_pts[c'], ...) of pts, caller order), the outputs
+ * and a hipGraph of one step.  Outputs are concatenated the same way; dL/dP
+ * of conformer c is its n_func x n_func block after those of 0..c-1. */
+typedef struct fe_batch fe_batch;
+
+#define FE_BATCH_FIELD 1   /* forward also computes dV/dC                 */
+#define FE_BATCH_DLDP 2    /* step runs loss + backward to dL/dP          */
+
+#define FE_BOUT_V 0        /* V, n_tot                                    */
+#define FE_BOUT_FIELD 1    /* dV/dC, n_tot x 3                            */
+#define FE_BOUT_DLDP 2     /* dL/dP blocks, sum_c n_func_c^2              */
+#define FE_BOUT_LOSS 3     /* sum_i w_i (V_i - T_i)^2 per conformer       */
+#define FE_BOUT_GV 4       /* dL/dV, n_tot                                */
+
+#define FE_BT_ITEMS_PER_WG 1    /* forward work items per resident workgroup */
+#define FE_BT_DG_ITEMS_PER_WG 2 /* dL/dG work items per resident workgroup   */
+#define FE_BT_CLASS_MASK 3      /* bit L: run class L (timing only)          */
+#define FE_BT_PPL0 10           /* + L: points per lane of class L (1, 2, 4) */
+
+int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables, const double* pts,
+                    const int64_t* n_pts, int ptr_kind, void* stream, fe_batch** out);
+int fe_batch_free(fe_batch* b);
+/* new coordinates (same counts); the Morton order is recomputed */
+int fe_batch_set_points(fe_batch* b, const double* pts, int ptr_kind, void* stream);
+/* loss weights w and targets T (n_tot each; NULL keeps; default w = 1, T = 0) */
+int fe_batch_set_loss(fe_batch* b, const double* w, const double* target, int ptr_kind,
+                      void* stream);
+int fe_batch_set_tuning(fe_batch* b, int key, int64_t value);
+/* V (and dV/dC with FE_BATCH_FIELD) of every conformer */
+int fe_batch_forward(fe_batch* b, int precision, int flags, void* stream);
+/* loss per conformer and gV = 2 w (V - T) from the last forward */
+int fe_batch_loss(fe_batch* b, void* stream);
+/* dL/dP of every conformer for gV (n_tot values; NULL: the loss gradient) */
+int fe_batch_backward(fe_batch* b, const double* gV, int ptr_kind, void* stream);
+/* forward + loss (+ backward with FE_BATCH_DLDP), eager */
+int fe_batch_step(fe_batch* b, int precision, int flags, void* stream);
+/* record fe_batch_step(precision, flags) as a hipGraph; fe_batch_launch replays
+ * it on any stream.  Tuning changes drop the graph. */
+int fe_batch_capture(fe_batch* b, int precision, int flags);
+int fe_batch_launch(fe_batch* b, void* stream);
+int fe_batch_read(fe_batch* b, int what, int ptr_kind, void* stream, double* dst);
+/* [n_conf, n_points, n_pairs, forward items, dG items, partial bytes,
+ *  dL/dP doubles, shell pairs, captured, items/WG, dG items/WG, class mask] */
+int fe_batch_info(const fe_batch* b, int64_t* out, int n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,8 @@ def test_header_declares_the_api():
     names = declared()
     for must in ("fe_init", "fe_finalize", "fe_last_error", "fe_table_build", "fe_table_from_pairs",
                  "fe_table_export", "fe_table_free", "fe_esp_forward", "fe_esp_forward_grad",
-                 "fe_esp_backward", "fe_boys_eval"):
+                 "fe_esp_backward", "fe_boys_eval", "fe_batch_create", "fe_batch_capture",
+                 "fe_batch_launch", "fe_batch_step", "fe_batch_free"):
         assert must in names
 
 
@@ -50,6 +51,10 @@ def test_errors_are_reported_not_fatal():
     assert b"null" in lib.fe_last_error()
     assert lib.fe_table_free(None) == -1
     assert lib.fe_esp_forward(None, None, 0, 0, 3, 0, None, None) == -1
+    assert lib.fe_batch_create(None, 1, None, None, None, 0, None, None) == -1
+    assert lib.fe_batch_free(None) == 0
+    assert lib.fe_batch_launch(None, None) == -1
+    assert lib.fe_batch_step(None, 0, 3, None) == -1
 
 
 def test_no_device_is_a_clean_error():
