@@ -55,11 +55,72 @@ def parse():
                     help="comma list of mode_precision legs to time")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--batch", type=int, default=256,
+                    help="conformers of the batched leg (0 = skip it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--batch-precisions", default="mixed,fp64")
     return ap.parse_args()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# phase ids of fe_profile
+PHASE_NAMES = {0: "sort", 1: "reduce", 2: "pack_gV", 3: "dG_to_dP"}
+PHASE_NAMES.update({8 + L: f"eval_L{L}" for L in range(7)})
+PHASE_NAMES.update({16 + L: f"dG_L{L}" for L in range(7)})
+PHASE_NAMES.update({24 + L: f"dG_reduce_L{L}" for L in range(7)})
+
+
+def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, flops,
+                             traffic_file):
+    """Roofline of the launch that takes the most time in the timed region.
+
+    phases: {phase id: (total ms, launches)} from fe_profile over the timed
+    steps.  Candidates are the per-class evaluation and dL/dG launches; the
+    achieved rate is that launch's algorithmic FLOPs (flops.kernel_flops) over
+    its own average event-timed duration."""
+    best = None
+    for pid, (ms, cnt) in phases.items():
+        if 8 <= pid < 15 or 16 <= pid < 23:
+            if best is None or ms > best[1]:
+                best = (pid, ms, cnt)
+    if best is None:
+        return None
+    pid, ms, cnt = best
+    grad = mode == "fwdbwd"
+    fp32 = False
+    if pid < 16:
+        L = pid - 8
+        kind = "fwd_grad" if grad else "fwd"
+        fp32 = prec == "mixed" and bool(fp32_L[L])
+        name = f"k_eval<{L}, {ppl_L[L]}, {str(grad).lower()}, {str(fp32).lower()}>"
+    else:
+        L = pid - 16
+        kind = "dg"
+        name = f"k_dG<{L}>"
+    avg_ms = ms / cnt
+    fl = flops.kernel_flops(kind, L, pairs_per_L[L], n)
+    achieved = fl / (avg_ms * 1e-3)
+    if fp32:
+        f_all = {"fwd": flops.f_fwd, "fwd_grad": flops.f_eval_grad}[kind](L)
+        share32 = flops.f32_part(L, grad) / f_all
+        peak = 1.0 / ((1 - share32) / flops.PEAK_FP64_VALU + share32 / flops.PEAK_FP32_VALU)
+    else:
+        peak = flops.PEAK_FP64_VALU
+    traffic = None
+    if traffic_file.exists():
+        kv = dict(line.split("=", 1) for line in traffic_file.read_text().splitlines()
+                  if "=" in line and not line.startswith(("#", "`", " ")))
+        if kv.get("kernel", "").strip() == name:
+            traffic = int(float(kv.get("traffic_bytes_per_launch", "nan")))
+    return dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
+                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic, kernel=name,
+                launches=cnt, avg_launch_ms=round(avg_ms, 4), flops_per_launch=fl,
+                flop_model=f"{kind}: N x pairs(L={L}) x {flops.__name__}."
+                           + {"fwd": "f_fwd", "fwd_grad": "f_eval_grad", "dg": "f_dg"}[kind])
 
 
 def cpu_baseline(w, budget_s: float) -> dict:
@@ -100,6 +161,84 @@ def cpu_baseline(w, budget_s: float) -> dict:
     return out
 
 
+def batch_leg(a, ws, torch, ctx, stream) -> dict:
+    """Batched conformers: every table evaluated forward (+ field) + loss +
+    dL/dP per step, recorded as one hipGraph and replayed; tile shape from the
+    batch autotuner.  Errors are per conformer against the batched FP64 forward."""
+    from fastesp_amd import autotune, flops
+    from fastesp_amd.batch import Batch
+    from fastesp_amd.builder import build_table_hip
+    sh = stream.cuda_stream
+    t0 = time.perf_counter()
+    tabs = [build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-12) for w in ws]
+    t_build = time.perf_counter() - t0
+    infos = [t.info() for t in tabs]
+    b = Batch(tabs, [w.points for w in ws])
+    n_tot = int(b.offsets[-1])
+    fl = sum(flops.forward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
+             + flops.backward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
+             for i, w in zip(infos, ws))
+    b.forward("fp64")
+    V64 = b.values()
+    rng = np.random.default_rng(4004)
+    wts = [rng.uniform(0.5, 1.0, len(v)) for v in V64]
+    T = [v + rng.normal(0.0, 1e-3, len(v)) for v in V64]
+    b.set_loss(wts, T)
+    log(f"batch: {len(ws)} conformers, {n_tot} points, {sum(i['n_pairs'] for i in infos)} pairs, "
+        f"tables built in {t_build:.1f} s")
+    out = dict(n_conformers=len(ws), n_points=n_tot,
+               n_pairs=int(sum(i["n_pairs"] for i in infos)),
+               atoms_range=[int(min(w.meta["n_atoms"] for w in ws)),
+                            int(max(w.meta["n_atoms"] for w in ws))],
+               functions_range=[int(min(w.basis.n for w in ws)), int(max(w.basis.n for w in ws))],
+               points_range=[int(min(len(w.points) for w in ws)),
+                             int(max(len(w.points) for w in ws))],
+               table_build_s=round(t_build, 2), flops_per_step=fl, legs={})
+    tuned = None
+    for prec in a.batch_precisions.split(","):
+        if tuned is None:
+            t1 = time.perf_counter()
+            tuned = autotune.tune_batch(b, prec, field=True, dldp=True)
+            tuned["seconds"] = round(time.perf_counter() - t1, 1)
+            log(f"batch autotune ({prec}): ppl_L={tuned['ppl_L']} fp32={tuned['mixed_L']} items/WG="
+                f"{tuned['items_per_wg']} dG items/WG={tuned['dg_items_per_wg']} "
+                f"({tuned['trials']} trials, {tuned['seconds']} s)")
+        b.capture(prec, field=True, dldp=True)
+        for _ in range(a.batch_warmup):
+            b.launch(sh)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.batch_steps):
+            b.launch(sh)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.batch_steps
+        V = b.values()
+        errs = np.array([np.abs(v - r).max() if len(v) else 0.0 for v, r in zip(V, V64)])
+        # per-launch breakdown from one eager step (launches inside a graph are not timed)
+        ctx.profile_start()
+        b.step(prec, field=True, dldp=True, stream=sh)
+        torch.cuda.synchronize()
+        ph = ctx.profile_stop()
+        leg = dict(ms_per_step=round(ms, 3), points_per_s=round(n_tot / (ms * 1e-3), 1),
+                   ms_per_1e4_pts=round(ms / (n_tot / 1e4), 5),
+                   achieved_tflops=round(fl / (ms * 1e-3) / 1e12, 2),
+                   max_abs_dV_vs_fp64_au=float(errs.max()),
+                   median_conformer_max_dV_au=float(np.median(errs)),
+                   eager_step_phases_ms={PHASE_NAMES.get(k, str(k)): round(v[0], 3)
+                                         for k, v in sorted(ph.items())})
+        out["legs"][prec] = leg
+        log(f"batch {prec}: {ms:.1f} ms/step = {leg['points_per_s']:.3e} points/s, "
+            f"max |dV| vs FP64 {errs.max():.2e}")
+    out["tuning"] = {k: v for k, v in tuned.items() if k != "all"} if tuned else None
+    b.close()
+    for t in tabs:
+        t.close()
+    return out
+
+
 def main() -> int:
     a = parse()
     if a.gpus != 1:
@@ -110,6 +249,12 @@ def main() -> int:
     from fastesp_amd import _lib, flops, synth
     from fastesp_amd.table import build_table
 
+    batch_ws = None
+    if a.batch:
+        # before the GPU is touched: the generator forks worker processes
+        t0 = time.perf_counter()
+        batch_ws = synth.conformer_set(a.batch)
+        log(f"generated {a.batch} conformers in {time.perf_counter() - t0:.1f} s")
     log(f"generating workload (config 1, {a.points} points)")
     w = synth.config1(a.points)
     ctx = _lib.Context.get(0)
@@ -175,6 +320,7 @@ def main() -> int:
             r = autotune.tune(tab, pts, op=mode, precision=PREC[prec])
             tuned[f"{mode}_{prec}"] = dict(ppl_L=r["best"]["ppl_L"],
                                            target_wg=r["best"]["target_wg"],
+                                           fp32_classes=r["best"]["mixed_L"],
                                            trials=len(r["all"]))
             log(f"autotune {mode} {prec}: best ppl_L={r['best']['ppl_L']} "
                 f"wg={r['best']['target_wg']} {r['best']['ms']:.2f} ms "
@@ -185,13 +331,18 @@ def main() -> int:
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
+        ctx.profile_start()  # per-launch HIP events inside the library, same stream
         e0.record(stream)
         for _ in range(a.steps):
             step()
         e1.record(stream)
         torch.cuda.synchronize()
+        phases = ctx.profile_stop()
         ms = e0.elapsed_time(e1) / a.steps
-        res = dict(ms_per_step=round(ms, 4), ms_per_1e4_pts=round(ms / (n / 1e4), 4))
+        res = dict(ms_per_step=round(ms, 4), ms_per_1e4_pts=round(ms / (n / 1e4), 4),
+                   phases_ms_per_step={PHASE_NAMES.get(k, str(k)): round(v[0] / a.steps, 4)
+                                       for k, v in sorted(phases.items())})
+        res["_phases"] = phases
         if g is not None:
             d = V.cpu().numpy()[g["idx"]] - g["V"]
             res["max_abs_dV_au"] = float(np.abs(d).max())
@@ -217,32 +368,35 @@ def main() -> int:
     head = legs[key]
     ms_step = head["ms_per_step"]
 
-    # --- roofline of the step's kernels (VALU, algorithmic FLOPs) -----------
+    # --- roofline: the dominant launch of the headline leg (VALU) ------------
     ppl = info["pairs_per_L"]
+    fp32_used = [0, 0, 0, 1, 1, 1, 1]  # library default under mixed precision
+    if key in tuned:
+        ppl_used = tuned[key]["ppl_L"]
+        fp32_used = tuned[key]["fp32_classes"] or fp32_used
+    elif a.ppl_classes:
+        ppl_used = [int(x) for x in a.ppl_classes.split(",")]
+    else:
+        ppl_used = [a.ppl or 2] * 7
+    traffic_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_hbm_traffic.md"
+    roof = dominant_kernel_roofline(head["_phases"], ppl, n, a.mode, a.precision, ppl_used,
+                                    fp32_used, flops, traffic_file)
+    # whole step, for reference: all launches of one step against the blended peak
     f_fwd = flops.forward_flops(ppl, n, info["n_atoms"])
     f_bwd = flops.backward_flops(ppl, n, info["n_atoms"]) if a.mode == "fwdbwd" else 0.0
     fl = f_fwd + f_bwd
-    achieved = fl / (ms_step * 1e-3)
     if a.precision == "mixed":
         f32 = flops.fp32_share_fwd(ppl) * fl
-        t_roof = (fl - f32) / flops.PEAK_FP64_VALU + f32 / flops.PEAK_FP32_VALU
-        peak = fl / t_roof
+        peak_step = fl / ((fl - f32) / flops.PEAK_FP64_VALU + f32 / flops.PEAK_FP32_VALU)
     else:
-        peak = flops.PEAK_FP64_VALU
-    # HBM bytes per step from the committed rocprofv3 --pmc summary of the FP64
-    # forward+backward step (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-    traffic = None
-    tr = ROOT / "profiles" / "esp_cfg1_30k_fwdbwd_fp64_hbm_traffic.md"
-    if a.mode == "fwdbwd" and n == 30000 and tr.exists():
-        for line in tr.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_step"):
-                traffic = int(float(line.split("=")[1]))
-    roof = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
-                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic,
-                kernel=("k_esp_fwd<L> x7 + k_esp_reduce" if a.mode == "fwd" else
-                        "k_esp_fwd_grad<L> x7 + k_esp_reduce_grad + k_dG<L> x7 + k_dG_reduce<L> "
-                        "x7 + k_dg_to_dp (one forward_grad + backward)"),
-                flops_per_step=fl, flops_fwd=f_fwd, flops_bwd=f_bwd)
+        peak_step = flops.PEAK_FP64_VALU
+    roof_step = dict(achieved=round(fl / (ms_step * 1e-3) / 1e12, 3),
+                     peak=round(peak_step / 1e12, 2), unit="TFLOP/s",
+                     frac=round(fl / (ms_step * 1e-3) / peak_step, 4), flops_per_step=fl)
+    for leg in legs.values():
+        leg.pop("_phases", None)
+
+    batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None
 
     cpu = None
     if not a.no_cpu:
@@ -273,6 +427,8 @@ def main() -> int:
             if "fwdbwd_fp64" in legs and "fwdbwd_mixed" in legs else None),
         "table_build_ms": round(t_build * 1e3, 2),
         "roofline": roof,
+        "roofline_step": roof_step,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -32,6 +32,7 @@ FE_BT_ITEMS_PER_WG = 1
 FE_BT_DG_ITEMS_PER_WG = 2
 FE_BT_CLASS_MASK = 3
 FE_BT_PPL0 = 10
+FE_BT_MIXED0 = 20
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -72,6 +73,7 @@ SIGNATURES = {
     "fe_batch_launch": [_vp, _vp],
     "fe_batch_read": [_vp, C.c_int, C.c_int, _vp, _vp],
     "fe_batch_info": [_vp, C.POINTER(C.c_int64), C.c_int],
+    "fe_profile": [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.c_int],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 
@@ -139,6 +141,16 @@ class Context:
 
     def set_tuning(self, key: int, value: int) -> None:
         check(self.lib.fe_set_tuning(self.handle, key, int(value)), "fe_set_tuning")
+
+    def profile_start(self) -> None:
+        check(self.lib.fe_profile(self.handle, 1, None, None, 0), "fe_profile")
+
+    def profile_stop(self) -> dict:
+        """{phase id: (total ms, launches)} recorded since profile_start()."""
+        ms = (C.c_double * 32)()
+        cnt = (C.c_int64 * 32)()
+        check(self.lib.fe_profile(self.handle, 0, ms, cnt, 32), "fe_profile")
+        return {i: (ms[i], int(cnt[i])) for i in range(32) if cnt[i]}
 
     def boys(self, T, M: int) -> np.ndarray:
         T = np.ascontiguousarray(T, np.float64).ravel()

@@ -4,6 +4,9 @@ Knobs (``fe_set_tuning``):
   keys 10..16  points per lane of class L = key-10 (1, 2 or 4): more points per
                lane reuse each scalar-loaded pair record more often and hide
                its load latency, at the cost of VGPRs (occupancy);
+  keys 20..26  under mixed precision, whether class L = key-20 takes the FP32
+               contraction or the FP64 kernel (FP32 only pays where the
+               contraction outweighs the conversion and routing work);
   key 1        target workgroups per class launch (pair-chunk count = target /
                point blocks): parallelism vs partial-row traffic.
 
@@ -52,17 +55,23 @@ def _store(d: dict) -> None:
         pass
 
 
-def apply(ctx: _lib.Context, ppl_L, target_wg: int) -> None:
+DEFAULT_MIXED_L = (0, 0, 0, 1, 1, 1, 1)
+
+
+def apply(ctx: _lib.Context, ppl_L, target_wg: int, mixed_L=None) -> None:
     for L, v in enumerate(ppl_L):
         ctx.set_tuning(10 + L, int(v))
     ctx.set_tuning(1, int(target_wg))
+    if mixed_L is not None:
+        for L, m in enumerate(mixed_L):
+            ctx.set_tuning(20 + L, int(m))
 
 
 def apply_cached(ctx, table, n_points: int, op: str = "fwdbwd",
                  precision: int = _lib.FE_PREC_FP64) -> bool:
     v = _load().get(shape_key(table.info()["pairs_per_L"], n_points, op, precision))
     if v:
-        apply(ctx, v["ppl_L"], v["target_wg"])
+        apply(ctx, v["ppl_L"], v["target_wg"], v.get("mixed_L"))
         return True
     return False
 
@@ -95,8 +104,8 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
                                            precision, _lib.FE_PTR_DEVICE, sh, None, None),
                        "fe_esp_backward")
 
-    def measure(ppl_L, wg):
-        apply(ctx, ppl_L, wg)
+    def measure(ppl_L, wg, mixed_L=None):
+        apply(ctx, ppl_L, wg, mixed_L)
         run()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -113,24 +122,27 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
 
     log = []
     ppl = [2] * NCLS
+    mixed = precision == _lib.FE_PREC_MIXED
+    mix = list(DEFAULT_MIXED_L)
     best_wg, best_ms = None, float("inf")
     for wg in WG_CHOICES:
-        ms = measure(ppl, wg)
+        ms = measure(ppl, wg, mix)
         log.append(dict(ppl_L=list(ppl), target_wg=wg, ms=ms))
         if ms < best_ms:
             best_wg, best_ms = wg, ms
     for L in present:
-        for v in PPL_CHOICES:
-            if v == ppl[L]:
-                continue
-            trial = list(ppl)
-            trial[L] = v
-            ms = measure(trial, best_wg)
-            log.append(dict(ppl_L=trial, target_wg=best_wg, ms=ms))
-            if ms < best_ms:
-                ppl, best_ms = trial, ms
-    apply(ctx, ppl, best_wg)
-    best = dict(ppl_L=ppl, target_wg=best_wg, ms=best_ms)
+        for m in ((0, 1) if mixed else (mix[L],)):
+            for v in PPL_CHOICES:
+                if v == ppl[L] and m == mix[L]:
+                    continue
+                trial, tmix = list(ppl), list(mix)
+                trial[L], tmix[L] = v, m
+                ms = measure(trial, best_wg, tmix)
+                log.append(dict(ppl_L=trial, mixed_L=tmix, target_wg=best_wg, ms=ms))
+                if ms < best_ms:
+                    ppl, mix, best_ms = trial, tmix, ms
+    apply(ctx, ppl, best_wg, mix)
+    best = dict(ppl_L=ppl, target_wg=best_wg, ms=best_ms, mixed_L=mix if mixed else None)
     k = shape_key(table.info()["pairs_per_L"], n, op, precision)
     d = _load()
     d[k] = dict(best, trials=len(log))
@@ -141,8 +153,8 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
 # ---------------------------------------------------------------------------
 # batched conformers
 # ---------------------------------------------------------------------------
-ITEMS_CHOICES = (2, 4, 8, 16, 32)
-DG_ITEMS_CHOICES = (2, 4, 8, 16)
+ITEMS_CHOICES = (4, 8, 16, 32, 64)
+DG_ITEMS_CHOICES = (4, 8, 16, 32, 64)
 
 
 def tune_batch(batch, precision="mixed", field: bool = True, dldp: bool = True, reps: int = 2,
@@ -180,19 +192,22 @@ def tune_batch(batch, precision="mixed", field: bool = True, dldp: bool = True, 
 
     fwd = lambda: batch.forward(prec, field=field, stream=sh)  # noqa: E731
     log = []
-    ppl = {}
+    ppl, mix = {}, {}
     for L in present:
         batch.set_tuning(_lib.FE_BT_CLASS_MASK, 1 << L)
-        best = (float("inf"), 2)
-        for v in PPL_CHOICES:
-            batch.set_tuning(_lib.FE_BT_PPL0 + L, v)
-            ms = timed(fwd)
-            log.append(dict(cls=L, ppl=v, ms=ms))
-            if verbose:
-                print(f"  tune batch class {L} ppl={v}: {ms:.3f} ms", flush=True)
-            best = min(best, (ms, v))
-        ppl[L] = best[1]
+        best = (float("inf"), 2, DEFAULT_MIXED_L[L])
+        for m in ((0, 1) if prec == _lib.FE_PREC_MIXED else (DEFAULT_MIXED_L[L],)):
+            batch.set_tuning(_lib.FE_BT_MIXED0 + L, m)
+            for v in PPL_CHOICES:
+                batch.set_tuning(_lib.FE_BT_PPL0 + L, v)
+                ms = timed(fwd)
+                log.append(dict(cls=L, ppl=v, mixed=m, ms=ms))
+                if verbose:
+                    print(f"  tune batch class {L} ppl={v} fp32={m}: {ms:.3f} ms", flush=True)
+                best = min(best, (ms, v, m))
+        ppl[L], mix[L] = best[1], best[2]
         batch.set_tuning(_lib.FE_BT_PPL0 + L, best[1])
+        batch.set_tuning(_lib.FE_BT_MIXED0 + L, best[2])
     batch.set_tuning(_lib.FE_BT_CLASS_MASK, (1 << NCLS) - 1)
     best_k = (float("inf"), 8)
     for k in ITEMS_CHOICES:
@@ -216,6 +231,8 @@ def tune_batch(batch, precision="mixed", field: bool = True, dldp: bool = True, 
                 print(f"  tune batch dG items/WG={k}: {ms:.3f} ms", flush=True)
             best_kd = min(best_kd, (ms, k))
         batch.set_tuning(_lib.FE_BT_DG_ITEMS_PER_WG, best_kd[1])
-    return dict(ppl_L=[ppl.get(L, 2) for L in range(NCLS)], items_per_wg=best_k[1],
+    return dict(ppl_L=[ppl.get(L, 2) for L in range(NCLS)],
+                mixed_L=[mix.get(L, DEFAULT_MIXED_L[L]) for L in range(NCLS)]
+                if prec == _lib.FE_PREC_MIXED else None, items_per_wg=best_k[1],
                 fwd_ms=best_k[0], dg_items_per_wg=best_kd[1], bwd_ms=best_kd[0],
                 trials=len(log), all=log)

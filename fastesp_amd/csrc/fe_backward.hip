@@ -74,8 +74,10 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   }
   const int32_t* perm = nullptr;
   if (c->sort_points && n >= 2 * FE_WG) {
+    const int ps = fe_prof_begin(c, FE_PH_SORT, st);
     rc = fe_sort_points(c, t, d_pts, (int)n, st, &d_pts, &perm);
     if (rc) return rc;
+    fe_prof_end(c, ps, st);
   }
   FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
@@ -84,9 +86,11 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   double* partial = (double*)c->ws[FE_WS_PARTIAL];
   rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, plane, true, precision == FE_PREC_MIXED, st);
   if (rc) return rc;
+  const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
   hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
                      partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);
+  fe_prof_end(c, pr, st);
   FE_HIPCHK(hipGetLastError());
   if (ptr_kind == FE_PTR_HOST) {
     FE_HIPCHK(hipMemcpyAsync(V, d_V, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -187,12 +191,16 @@ static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, in
   const int64_t nL = t->n_L[L];
   if (!nL) return;
   const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
+  const int pg = fe_prof_begin(t->ctx, FE_PH_DG0 + L, st);
   hipLaunchKernelGGL(k_dG<L>, dim3((unsigned)((nL + 255) / 256), d.chunks[L]), dim3(256), 0, st,
                      t->d_rec[L], (int)nL, pw, n, d.ppc[L], tab, part + d.part_off[L]);
+  fe_prof_end(t->ctx, pg, st);
   const int64_t ne = nL * FeL<L>::NT;
+  const int pr = fe_prof_begin(t->ctx, FE_PH_DGRED0 + L, st);
   hipLaunchKernelGGL(k_dG_reduce<L>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st,
                      part + d.part_off[L], (int)nL, d.chunks[L], dG + d.dg_off[L],
                      t->d_flatoff[L], flat);
+  fe_prof_end(t->ctx, pr, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -234,9 +242,12 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     FE_HIPCHK(hipMemcpyAsync(tmp, gV, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
     d_gV = tmp;
   }
-  if (n > 0)
+  if (n > 0) {
+    const int pp = fe_prof_begin(c, FE_PH_PACK, st);
     hipLaunchKernelGGL(k_pack_pw, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_pts,
                        d_gV, (int)n, pw);
+    fe_prof_end(c, pp, st);
+  }
   rc = fe_ctx_reserve(c, FE_WS_AUX1, std::max<size_t>(d.part_total, 1) * sizeof(double));
   if (rc) return rc;
   rc = fe_ctx_reserve(c, FE_WS_AUX2, std::max<size_t>(d.dg_total, 1) * sizeof(double));
@@ -279,7 +290,9 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
       d_dP = (double*)c->ws[FE_WS_OUT];
     }
     const DPArgs pa = fe_dp_args(t, dG, d.dg_off, d_dP);
+    const int pd = fe_prof_begin(c, FE_PH_DP, st);
     if (t->n_sp > 0) hipLaunchKernelGGL(k_dg_to_dp, dim3(t->n_sp), dim3(64), 0, st, pa);
+    fe_prof_end(c, pd, st);
     FE_HIPCHK(hipGetLastError());
   }
   if (ptr_kind == FE_PTR_HOST) {

@@ -111,13 +111,20 @@ struct fe_batch {
   std::vector<int64_t> dp_off;  // n_conf + 1
   // tuning
   int ppl[FE_LMAX + 1] = {2, 2, 2, 2, 2, 1, 1};
+  bool mixed_L[FE_LMAX + 1] = {false, false, false, true, true, true, true};
   int k_eval = 8, k_dg = 8, mask = 0x7f;
   // graph
   hipStream_t cs = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_prec = 0, g_flags = 0;
+  bool capturing = false;
 };
+
+// phase events only for eager launches (never recorded into the graph)
+static int prof_begin(fe_batch* b, int phase, hipStream_t st) {
+  return b->capturing ? -1 : fe_prof_begin(b->ctx, phase, st);
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -471,7 +478,7 @@ static int plan_fwd(fe_batch* b, bool grad, bool mixed, hipStream_t st) {
   int resident[FE_LMAX + 1];
   double total[FE_LMAX + 1] = {};
   for (int L = 0; L <= FE_LMAX; ++L) {
-    const int occ = fe_batch_eval(L, b->ppl[L], grad, mixed, nullptr, nullptr);
+    const int occ = fe_batch_eval(L, b->ppl[L], grad, mixed && b->mixed_L[L], nullptr, nullptr);
     if (occ < 1) return fe_set_error(FE_ERR_HIP, "batch: class %d kernel cannot be resident", L);
     resident[L] = occ * c->n_cu;
     const int blk = FE_WG * b->ppl[L];
@@ -715,9 +722,12 @@ static void launch_forward(fe_batch* b, hipStream_t st) {
     a.plane = b->plane;
     a.tau = (float)c->mixed_tau;
     a.grid = b->grid[L];
-    fe_batch_eval(L, b->ppl[L], grad, mixed, &a, st);
+    const int ph = prof_begin(b, FE_PH_EVAL0 + L, st);
+    fe_batch_eval(L, b->ppl[L], grad, mixed && b->mixed_L[L], &a, st);
+    fe_prof_end(c, ph, st);
   }
   if (b->n_tot) {
+    const int pr = prof_begin(b, FE_PH_REDUCE, st);
     const dim3 g((unsigned)((b->n_tot + 63) / 64));
     if (grad)
       hipLaunchKernelGGL(k_batch_reduce<true>, g, dim3(64), 0, st, b->partial, b->plane, b->conf,
@@ -725,6 +735,7 @@ static void launch_forward(fe_batch* b, hipStream_t st) {
     else
       hipLaunchKernelGGL(k_batch_reduce<false>, g, dim3(64), 0, st, b->partial, b->plane, b->conf,
                          b->pt_conf, b->pts, b->perm, (int)b->n_tot, b->V, b->field);
+    fe_prof_end(c, pr, st);
   }
   b->has_field = grad;
 }
@@ -736,21 +747,30 @@ static void launch_loss(fe_batch* b, hipStream_t st) {
 
 template <int L>
 static void launch_dG_L(fe_batch* b, hipStream_t st) {
-  if (b->n_dgitems[L] && (b->mask & (1 << L)))
+  if (b->n_dgitems[L] && (b->mask & (1 << L))) {
+    const int pg = prof_begin(b, FE_PH_DG0 + L, st);
     hipLaunchKernelGGL(k_dG_batch<L>, dim3(b->dggrid[L]), dim3(256), 0, st, b->dgitems[L],
                        b->n_dgitems[L], b->counters + CNT_DG + L, b->recs[L], b->pw,
                        b->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS, b->dg_part);
-  if (b->n_elem[L])
+    fe_prof_end(b->ctx, pg, st);
+  }
+  if (b->n_elem[L]) {
+    const int pr = prof_begin(b, FE_PH_DGRED0 + L, st);
     hipLaunchKernelGGL(k_dG_reduce_batch<L>, dim3((unsigned)((b->n_elem[L] + 255) / 256)),
                        dim3(256), 0, st, b->segs[L], b->n_segs[L], b->n_elem[L], b->dg_part,
                        b->dG);
+    fe_prof_end(b->ctx, pr, st);
+  }
 }
 
 static void launch_backward(fe_batch* b, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, b->counters + CNT_DG, FE_LMAX + 1);
-  if (b->n_tot)
+  if (b->n_tot) {
+    const int pp = prof_begin(b, FE_PH_PACK, st);
     hipLaunchKernelGGL(k_pack_pw_perm, dim3((unsigned)((b->n_tot + 255) / 256)), dim3(256), 0, st,
                        b->pts, b->gV, b->perm, (int)b->n_tot, b->pw);
+    fe_prof_end(b->ctx, pp, st);
+  }
   launch_dG_L<0>(b, st);
   launch_dG_L<1>(b, st);
   launch_dG_L<2>(b, st);
@@ -760,8 +780,10 @@ static void launch_backward(fe_batch* b, hipStream_t st) {
   launch_dG_L<6>(b, st);
   if (b->n_sp_tot) {
     const int64_t grid = std::min<int64_t>(b->n_sp_tot, (int64_t)b->ctx->n_cu * 32);
+    const int pd = prof_begin(b, FE_PH_DP, st);
     hipLaunchKernelGGL(k_dg_to_dp_batch, dim3((unsigned)grid), dim3(64), 0, st, b->dpargs,
                        b->sp_off, b->n_conf, b->n_sp_tot);
+    fe_prof_end(b->ctx, pd, st);
   }
 }
 
@@ -838,7 +860,9 @@ extern "C" int fe_batch_capture(fe_batch* b, int precision, int flags) {
   FE_HIPCHK(hipStreamSynchronize(b->cs));
   drop_graph(b);
   FE_HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+  b->capturing = true;
   launch_step(b, flags, b->cs);
+  b->capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->cs, &g);
   if (e != hipSuccess || !g) {
@@ -930,6 +954,11 @@ extern "C" int fe_batch_read(fe_batch* b, int what, int ptr_kind, void* stream, 
 
 extern "C" int fe_batch_set_tuning(fe_batch* b, int key, int64_t value) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_set_tuning: null batch");
+  if (key >= 20 && key <= 20 + FE_LMAX) {
+    if (b->mixed_L[key - 20] != (value != 0)) drop_fplan(b);
+    b->mixed_L[key - 20] = value != 0;
+    return FE_OK;
+  }
   if (key >= 10 && key <= 10 + FE_LMAX) {
     if (value != 1 && value != 2 && value != 4)
       return fe_set_error(FE_ERR_ARG, "fe_batch_set_tuning: points per lane must be 1, 2 or 4");

@@ -254,6 +254,15 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, 
   dst[e] = c == 6 ? 0.5 / s[3] : c == 7 ? 0.88622692545275801365 / sqrt(s[3]) : s[c];
 }
 
+// (Px, Py, Pz, p) of the records idx[0..n) of a candidate buffer, packed
+__global__ __launch_bounds__(256) void k_head(const double* __restrict__ src, const int S,
+                                              const int* __restrict__ idx, const int n,
+                                              double* __restrict__ head) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= 4 * n) return;
+  head[e] = src[(size_t)idx[e >> 2] * S + (e & 3)];
+}
+
 // ---------------------------------------------------------------------------
 namespace {
 struct HostShell {
@@ -501,10 +510,22 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     if (!ncand[L]) continue;
     std::vector<int> idx, s2k(ncand[L], -1);
     for (int64_t q = 0; q < ncand[L]; ++q)
-      if (hk[L][q]) {
-        s2k[q] = (int)idx.size();
-        idx.push_back((int)q);
+      if (hk[L][q]) idx.push_back((int)q);
+    // original (reference-order) position of every kept record, for export:
+    // candidate slot -> global candidate index via its shell pair (slots of
+    // one class are contiguous per shell pair in shell-pair order)
+    std::vector<int32_t> orig(idx.size());
+    {
+      std::vector<int> sps;
+      for (int s = 0; s < nsp; ++s)
+        if (sh[sp_i[s]].l + sh[sp_j[s]].l == L) sps.push_back(s);
+      size_t cur = 0;
+      for (size_t k = 0; k < idx.size(); ++k) {
+        while (cur + 1 < sps.size() && sp_slot[sps[cur + 1]] <= idx[k]) ++cur;
+        const int s = sps[cur];
+        orig[k] = (int32_t)(sp_global[s] + (idx[k] - sp_slot[s]));
       }
+    }
     t->n_L[L] = (int64_t)idx.size();
     total += (int64_t)idx.size();
     if (hipMalloc(&t->d_slot2kept[L], sizeof(int) * ncand[L]) != hipSuccess) {
@@ -526,24 +547,49 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     }
     t->dev_bytes += (size_t)idx.size() * S * sizeof(double);
     (void)hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+    if (c->pair_order && idx.size() > 1) {
+      // record headers of the kept candidates -> host order -> permuted gather list
+      const int nk = (int)idx.size();
+      std::vector<double> head((size_t)nk * 4);
+      double* d_head = nullptr;
+      if (hipMalloc(&d_head, head.size() * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d_idx);
+        free_cand();
+        return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
+      }
+      hipLaunchKernelGGL(k_head, dim3((unsigned)((4 * nk + 255) / 256)), dim3(256), 0, st,
+                         cand[L], S, d_idx, nk, d_head);
+      const hipError_t he =
+          hipMemcpyAsync(head.data(), d_head, head.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st);
+      const hipError_t hs = hipStreamSynchronize(st);
+      (void)hipFree(d_head);
+      if (he != hipSuccess || hs != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(d_idx);
+        free_cand();
+        return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: record header copy failed"));
+      }
+      const std::vector<int> ord = fe_pair_order(head.data(), 4, nk);
+      std::vector<int> idx2(nk);
+      std::vector<int32_t> orig2(nk);
+      for (int k = 0; k < nk; ++k) {
+        idx2[k] = idx[ord[k]];
+        orig2[k] = orig[ord[k]];
+      }
+      idx.swap(idx2);
+      orig.swap(orig2);
+      (void)hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+    }
+    for (size_t k = 0; k < idx.size(); ++k) s2k[idx[k]] = (int)k;
+    (void)hipMemcpy(t->d_slot2kept[L], s2k.data(), sizeof(int) * ncand[L], hipMemcpyHostToDevice);
     const int64_t ne = (int64_t)idx.size() * S;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, cand[L], S,
                        d_idx, (int)idx.size(), t->d_rec[L]);
     (void)hipStreamSynchronize(st);
     (void)hipFree(d_idx);
-    // original (reference-order) position of every kept record, for export
-    t->orig[L].resize(idx.size());
-    // map candidate slot -> global candidate index via its shell pair
-    // (slots of one class are contiguous per shell pair in shell-pair order)
-    std::vector<int> sps;
-    for (int s = 0; s < nsp; ++s)
-      if (sh[sp_i[s]].l + sh[sp_j[s]].l == L) sps.push_back(s);
-    size_t cur = 0;
-    for (size_t k = 0; k < idx.size(); ++k) {
-      while (cur + 1 < sps.size() && sp_slot[sps[cur + 1]] <= idx[k]) ++cur;
-      const int s = sps[cur];
-      t->orig[L][k] = (int32_t)(sp_global[s] + (idx[k] - sp_slot[s]));
-    }
+    t->orig[L] = std::move(orig);
   }
   t->n_pairs = (int)total;
   int64_t nterms = 0, nvalid = 0;

@@ -161,6 +161,60 @@ int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes) {
   return FE_OK;
 }
 
+int fe_prof_begin(fe_ctx* c, int phase, hipStream_t st) {
+  if (!c->prof_on) return -1;
+  if (c->prof_n == c->prof.size()) {
+    FeProfEv e{};
+    if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1;
+    }
+    c->prof.push_back(e);
+  }
+  FeProfEv& e = c->prof[c->prof_n];
+  e.phase = phase;
+  if (hipEventRecord(e.a, st) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return (int)c->prof_n++;
+}
+
+void fe_prof_end(fe_ctx* c, int slot, hipStream_t st) {
+  if (slot < 0) return;
+  if (hipEventRecord(c->prof[slot].b, st) != hipSuccess) {
+    (void)hipGetLastError();
+    c->prof[slot].phase = -1;
+  }
+}
+
+extern "C" int fe_profile(fe_ctx* c, int op, double* ms, int64_t* count, int n_phase) {
+  if (!c) return fe_set_error(FE_ERR_ARG, "fe_profile: null context");
+  FE_HIPCHK(hipSetDevice(c->device));
+  if (op == 1) {
+    c->prof_n = 0;
+    c->prof_on = true;
+    return FE_OK;
+  }
+  if (op != 0) return fe_set_error(FE_ERR_ARG, "fe_profile: op must be 0 (stop) or 1 (start)");
+  c->prof_on = false;
+  if (ms)
+    for (int i = 0; i < n_phase; ++i) ms[i] = 0.0;
+  if (count)
+    for (int i = 0; i < n_phase; ++i) count[i] = 0;
+  for (size_t k = 0; k < c->prof_n; ++k) {
+    const FeProfEv& e = c->prof[k];
+    if (e.phase < 0 || e.phase >= n_phase) continue;
+    FE_HIPCHK(hipEventSynchronize(e.b));
+    float t = 0.f;
+    FE_HIPCHK(hipEventElapsedTime(&t, e.a, e.b));
+    if (ms) ms[e.phase] += t;
+    if (count) count[e.phase] += 1;
+  }
+  c->prof_n = 0;
+  return FE_OK;
+}
+
 extern "C" int fe_init(int device, fe_ctx** out) {
   if (!out) return fe_set_error(FE_ERR_ARG, "fe_init: null output pointer");
   *out = nullptr;
@@ -206,6 +260,10 @@ extern "C" int fe_finalize(fe_ctx* c) {
     if (c->ws[s]) (void)hipFree(c->ws[s]);
   if (c->d_boys) (void)hipFree(c->d_boys);
   if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
+  for (FeProfEv& e : c->prof) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
   delete c;
   return FE_OK;
 }
@@ -228,11 +286,14 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       if (value < 1 || value > (1 << 24)) return fe_set_error(FE_ERR_ARG, "bad target workgroups");
       c->target_wg = (int)value;
       return FE_OK;
-    case 2:
-      c->tile_screen = value != 0;
-      return FE_OK;
     case 3:
       c->sort_points = value != 0;
+      return FE_OK;
+    case 5:
+      c->pair_order = value != 0;
+      return FE_OK;
+    case 20: case 21: case 22: case 23: case 24: case 25: case 26:  // FP32 contraction of class L
+      c->mixed_L[key - 20] = value != 0;
       return FE_OK;
     case 4:  // mixed-precision routing threshold, in units of 1e-12 a.u.
       if (value < 0) return fe_set_error(FE_ERR_ARG, "mixed threshold must be >= 0");
@@ -265,6 +326,41 @@ int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec) {
     t->dev_bytes += b;
   }
   return FE_OK;
+}
+
+std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n) {
+  std::vector<int> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int)i;
+  if (n < 2) return order;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double lpmin = INFINITY, lpmax = -INFINITY;
+  for (int64_t i = 0; i < n; ++i) {
+    const double* r = rec4 + i * stride;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], r[a]);
+      hi[a] = std::max(hi[a], r[a]);
+    }
+    const double lp = std::log(r[3]);
+    lpmin = std::min(lpmin, lp);
+    lpmax = std::max(lpmax, lp);
+  }
+  const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-12});
+  const double lpw = std::max(lpmax - lpmin, 1e-12);
+  std::vector<uint64_t> key(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* r = rec4 + i * stride;
+    uint64_t code = 0;
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a)
+      q[a] = (uint32_t)std::min(1023.0, std::max(0.0, (r[a] - lo[a]) / ext * 1023.0));
+    for (int b = 0; b < 10; ++b)
+      for (int a = 0; a < 3; ++a) code |= (uint64_t)((q[a] >> b) & 1u) << (3 * b + a);
+    const uint64_t bucket =
+        (uint64_t)std::min(15.0, std::floor((std::log(r[3]) - lpmin) / lpw * 16.0));
+    key[i] = (bucket << 30) | code;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+  return order;
 }
 
 int fe_table_index(fe_table* t) {
@@ -370,6 +466,21 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
     for (int j = 0; j < nt; ++j) r[8 + j] = tm_g[off + j];
     orig[L].push_back(i);
   }
+  if (c->pair_order)
+    for (int L = 0; L <= FE_LMAX; ++L) {
+      const int S = fe_host::STRIDE[L];
+      const int64_t nL = (int64_t)orig[L].size();
+      const std::vector<int> ord = fe_pair_order(rec[L].data(), S, nL);
+      std::vector<double> r2(rec[L].size());
+      std::vector<int32_t> o2(nL);
+      for (int64_t k = 0; k < nL; ++k) {
+        std::copy(rec[L].begin() + (size_t)ord[k] * S, rec[L].begin() + (size_t)(ord[k] + 1) * S,
+                  r2.begin() + (size_t)k * S);
+        o2[k] = orig[L][ord[k]];
+      }
+      rec[L].swap(r2);
+      orig[L].swap(o2);
+    }
   FE_HIPCHK(hipSetDevice(c->device));
   fe_table* t = new fe_table();
   t->ctx = c;
@@ -577,8 +688,10 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
   }
   const int32_t* perm = nullptr;
   if (c->sort_points && n >= 2 * FE_WG && (parts & FE_PART_ELECTRON)) {
+    const int ps = fe_prof_begin(c, FE_PH_SORT, st);
     rc = fe_sort_points(c, t, d_pts, (int)n, st, &d_pts, &perm);
     if (rc) return rc;
+    fe_prof_end(c, ps, st);
   }
   FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
   double* partial = nullptr;
@@ -590,8 +703,10 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     if (rc) return rc;
   }
   const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
+  const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
   hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, partial,
                      nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, perm, d_V);
+  fe_prof_end(c, pr, st);
   FE_HIPCHK(hipGetLastError());
   if (ptr_kind == FE_PTR_HOST) {
     FE_HIPCHK(hipMemcpyAsync(V, d_V, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));

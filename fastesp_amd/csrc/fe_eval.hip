@@ -61,7 +61,9 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
   const int M = grad ? L + 1 : L;
   const double* tab = t->ctx->d_boys + (size_t)M * FE_BOYS_NR * FE_BOYS_COLS;
   const dim3 grid(pl.n_pb[L], pl.chunks[L]);
+  mixed = mixed && t->ctx->mixed_L[L];
   const float* g32 = t->d_rec32[L];
+  const int ph = fe_prof_begin(t->ctx, FE_PH_EVAL0 + L, st);
 #define FE_K(G, X)                                                                            \
   hipLaunchKernelGGL((k_eval<L, PPL, G, X>), grid, dim3(FE_WG), 0, st, t->d_rec[L], g32,      \
                      (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.row0[L], pl.ld,      \
@@ -72,6 +74,7 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
     if (mixed) FE_K(false, true); else FE_K(false, false);
   }
 #undef FE_K
+  fe_prof_end(t->ctx, ph, st);
 }
 
 int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,

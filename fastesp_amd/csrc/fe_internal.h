@@ -53,6 +53,23 @@ struct LTabDev {
   const double* m0t_val;
 };
 
+// phases timed by fe_profile (HIP events on the launch stream)
+enum {
+  FE_PH_SORT = 0,
+  FE_PH_REDUCE = 1,
+  FE_PH_PACK = 2,
+  FE_PH_DP = 3,
+  FE_PH_EVAL0 = 8,    // + L: evaluation launch of class L
+  FE_PH_DG0 = 16,     // + L: dL/dG launch of class L
+  FE_PH_DGRED0 = 24,  // + L: dL/dG chunk reduction of class L
+  FE_PH_N = 32
+};
+
+struct FeProfEv {
+  int phase;
+  hipEvent_t a, b;
+};
+
 struct fe_ctx {
   int device = 0;
   int n_cu = 256;
@@ -63,10 +80,21 @@ struct fe_ctx {
   size_t ws_bytes[FE_WS_SLOTS] = {};
   int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 2};  // points per lane, per class
   int target_wg = 4096; // workgroups per class launch
-  bool tile_screen = false;
   bool sort_points = true;  // Morton-order the points before evaluation
+  bool pair_order = true;   // order each class by exponent bucket, then Morton code of P
   double mixed_tau = 1e-4;  // FP32 routing threshold of the mixed path (a.u.)
+  // classes that take the FP32 contraction under FE_PREC_MIXED (the others run
+  // the FP64 kernel: for small L the conversion and routing cost more than the
+  // FP32 contraction saves)
+  bool mixed_L[FE_LMAX + 1] = {false, false, false, true, true, true, true};
+  bool prof_on = false;     // record phase events (fe_profile)
+  std::vector<FeProfEv> prof;
+  size_t prof_n = 0;
 };
+
+// start / end of a timed phase on stream st (no-ops unless profiling)
+int fe_prof_begin(fe_ctx* c, int phase, hipStream_t st);
+void fe_prof_end(fe_ctx* c, int slot, hipStream_t st);
 
 struct fe_table {
   fe_ctx* ctx = nullptr;
@@ -115,6 +143,12 @@ int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);
 int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz);
 int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec);
 int fe_table_index(fe_table* t);  // flat term offsets of the records (from orig order)
+// Record order within a class: 16 buckets of log p over the class's range, then
+// the 30-bit Morton code of P in the bounding box of the records.  Pairs whose
+// near-field radii are alike then share a wavefront in the pair-per-lane
+// adjoint, whose far-field test is per wavefront.  rec4 holds (Px, Py, Pz, p)
+// at a stride of `stride` doubles; returns a stable permutation.
+std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n);
 FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg);
 
 // dispatch a templated class launcher on the plan's points-per-lane of class L

@@ -35,6 +35,29 @@ def f_bwd(L: int) -> int:
     return 19 + boys_cost(L + 1) + (NMONO[L] - 1) + 14 * NTERM[L]
 
 
+def f_eval_grad(L: int) -> int:
+    """Evaluation launch that also forms the field (value + dV/dC in one pass):
+    the forward plus the field's 9/term and the Boys-shift 3/term of F_bwd."""
+    return f_fwd(L) + 12 * NTERM[L]
+
+
+def f_dg(L: int) -> int:
+    """dL/dG launch: the rest of F_bwd (its own R, T, Boys and monomials, 2/term)."""
+    return f_bwd(L) - 12 * NTERM[L]
+
+
+def f32_part(L: int, grad: bool) -> int:
+    """FP32 share of a mixed evaluation launch per (point, pair): monomials and
+    the per-term products of the value (3/term) and, with the field, 9/term more."""
+    return (NMONO[L] - 1) + (12 if grad else 3) * NTERM[L]
+
+
+def kernel_flops(kind: str, L: int, n_pairs_L: int, n_points: int) -> float:
+    """Algorithmic FLOPs of one class launch: kind 'fwd', 'fwd_grad' or 'dg'."""
+    f = {"fwd": f_fwd, "fwd_grad": f_eval_grad, "dg": f_dg}[kind](L)
+    return float(n_points) * float(n_pairs_L) * f
+
+
 def forward_flops(pairs_per_L, n_points: int, n_atoms: int) -> float:
     return float(n_points) * (sum(n * f_fwd(L) for L, n in enumerate(pairs_per_L))
                               + 11 * n_atoms)
@@ -57,3 +80,4 @@ def table_bytes(pairs_per_L) -> int:
 
 
 assert [f_fwd(L) for L in range(7)] == [34, 49, 79, 131, 215, 341, 522]
+assert all(f_eval_grad(L) + f_dg(L) == f_fwd(L) + f_bwd(L) for L in range(7))

@@ -133,8 +133,8 @@ def build_wavefunction(rng, symbols, coords, recipe, extra_d_atoms=()) -> Wavefu
                         shells=shells)
 
 
-def random_density(basis: Basis, nelec: int, rng, S: np.ndarray | None = None):
-    """P = 2 C C^T, C Gaussian random with C^T S C = 1 (n_occ = nelec/2)."""
+def random_orbitals(basis: Basis, nelec: int, rng, S: np.ndarray | None = None):
+    """C (n_fn x n_occ) Gaussian random with C^T S C = 1 (n_occ = nelec/2)."""
     if nelec % 2:
         raise ValueError("odd electron count")
     nocc = nelec // 2
@@ -143,9 +143,18 @@ def random_density(basis: Basis, nelec: int, rng, S: np.ndarray | None = None):
     C = rng.standard_normal((basis.n, nocc))
     M = C.T @ S @ C
     w, v = np.linalg.eigh(M)
-    C = C @ (v @ np.diag(w ** -0.5) @ v.T)
+    return C @ (v @ np.diag(w ** -0.5) @ v.T), S
+
+
+def density_from_orbitals(C: np.ndarray) -> np.ndarray:
     P = 2.0 * C @ C.T
-    return 0.5 * (P + P.T), S
+    return 0.5 * (P + P.T)
+
+
+def random_density(basis: Basis, nelec: int, rng, S: np.ndarray | None = None):
+    """P = 2 C C^T, C Gaussian random with C^T S C = 1 (n_occ = nelec/2)."""
+    C, S = random_orbitals(basis, nelec, rng, S)
+    return density_from_orbitals(C), S
 
 
 # ---------------------------------------------------------------------------
@@ -203,7 +212,7 @@ def config0(n_points: int = 1000, seed: int = 1000) -> Workload:
 
 
 def _tz_molecule(rng, n_atoms: int, n_s: int, n_heavy: int, box: float,
-                 extra_d_atoms=()):
+                 extra_d_atoms=(), orbitals: bool = False):
     coords = place_atoms(rng, n_atoms, box, 1.8)
 
     def draw():
@@ -217,9 +226,9 @@ def _tz_molecule(rng, n_atoms: int, n_s: int, n_heavy: int, box: float,
     extra = tuple(s_atoms[:len(extra_d_atoms)]) if extra_d_atoms else ()
     wfn = build_wavefunction(rng, syms, coords, _recipe_tz, extra)
     basis = Basis(wfn)
-    P, _ = random_density(basis, ne, rng)
+    C, _ = random_orbitals(basis, ne, rng)
     wfn.nelec_override = ne
-    return wfn, basis, P, ne
+    return wfn, basis, (C if orbitals else density_from_orbitals(C)), ne
 
 
 def config1(n_points: int = 30000, seed: int = 1001) -> Workload:
@@ -245,15 +254,40 @@ def config3(n_points: int = 200000, seed: int = 1003) -> Workload:
 
 
 def conformer(index: int, seed: int = 1004, n_atoms: int | None = None,
-              n_points: int | None = None) -> Workload:
-    """One member of the batched set: 10-80 atoms, 500-20000 points, TZ recipe."""
+              n_points: int | None = None, orbitals: bool = False) -> Workload:
+    """One member of the batched set: 10-80 atoms, 500-20000 points, TZ recipe.
+    With ``orbitals`` the ``density`` field holds the occupied factor C
+    (P = 2 C C^T, ``density_from_orbitals``) instead of P."""
     rng = np.random.default_rng([seed, index])
     na = int(rng.integers(10, 81)) if n_atoms is None else n_atoms
     npts = int(rng.integers(500, 20001)) if n_points is None else n_points
     n_s = 2 if na >= 20 else 0
     n_heavy = max(1, int(round(0.48 * na)) - n_s)
     box = 20.0 * (na / 54.0) ** (1.0 / 3.0)
-    wfn, basis, P, ne = _tz_molecule(rng, na, n_s, n_heavy, box)
+    wfn, basis, P, ne = _tz_molecule(rng, na, n_s, n_heavy, box, orbitals=orbitals)
     pts = vdw_surface_points(wfn.symbols, wfn.coords, npts, 1.4, rng)
     return Workload(f"conf{index}", seed, wfn, basis, P, pts,
                     dict(n_atoms=na, n_func=basis.n, n_prim=basis.n_prim, nelec=ne))
+
+
+def _conformer_orbitals(args):
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        return conformer(*args, orbitals=True)
+
+
+def conformer_set(n: int, seed: int = 1004, workers: int = 0) -> list:
+    """conformer(0..n-1) generated in worker processes (fork: call this before
+    the process touches the GPU).  Workers return the occupied factor C, a few
+    MB instead of the n_fn^2 density; P = 2 C C^T is formed here."""
+    import multiprocessing as mp
+    import os
+    if workers <= 0:
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    if workers == 1 or n < 4:
+        return [conformer(i, seed) for i in range(n)]
+    with mp.get_context("fork").Pool(workers) as pool:
+        out = pool.map(_conformer_orbitals, [(i, seed) for i in range(n)], chunksize=1)
+    for w in out:
+        w.density = density_from_orbitals(w.density)
+    return out

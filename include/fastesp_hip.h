@@ -164,10 +164,26 @@ int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const double* gV,
 int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
                  void* stream, double* F);
 
-/* Tuning knobs of the evaluation kernels (autotuner output).
- * key 0: points per lane (1 or 2); key 1: target workgroups per launch;
- * key 2: use per-tile screening (0/1) -- reserved. */
+/* Tuning knobs of the single-table evaluation (autotuner output).
+ * key 0: points per lane of every class (1, 2 or 4); 10 + L: of class L only;
+ * key 1: target workgroups per class launch (sets the pair-chunk count);
+ * key 3: Morton-order the points before evaluation (0/1, default 1);
+ * key 4: FP32 routing threshold of FE_PREC_MIXED in units of 1e-12 a.u.
+ *        (default 1e8, i.e. 1e-4 a.u.; 0 sends every pair to FP64);
+ * key 5: order the pairs of each class by exponent bucket, then Morton code
+ *        of P, at table creation (0/1, default 1; export keeps reference order);
+ * key 20 + L: class L takes the FP32 contraction under FE_PREC_MIXED (0/1;
+ *        default 1 for L >= 3, else the class runs the FP64 kernel). */
 int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
+
+/* Phase timing with HIP events on the launch stream (no counterpart in the
+ * reference, whose esp_gpu times whole runs, fastesp/native/esp_gpu.cu:299-313).
+ * op 1 starts recording; op 0 stops and returns, per phase id < n_phase, the
+ * summed milliseconds and the number of launches.  Phase ids: 0 point sort,
+ * 1 per-point reduction, 2 gV packing, 3 dL/dP, 8+L evaluation of class L,
+ * 16+L dL/dG of class L, 24+L dL/dG chunk reduction of class L.  Launches
+ * recorded into a hipGraph are not timed. */
+int fe_profile(fe_ctx* ctx, int op, double* ms, int64_t* count, int n_phase);
 
 /* ---- batched conformers ------------------------------------------------
  * No counterpart in the reference (it evaluates one table per esp_gpu
@@ -191,6 +207,7 @@ typedef struct fe_batch fe_batch;
 #define FE_BT_DG_ITEMS_PER_WG 2 /* dL/dG work items per resident workgroup   */
 #define FE_BT_CLASS_MASK 3      /* bit L: run class L (timing only)          */
 #define FE_BT_PPL0 10           /* + L: points per lane of class L (1, 2, 4) */
+#define FE_BT_MIXED0 20         /* + L: class L uses FP32 under FE_PREC_MIXED */
 
 int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables, const double* pts,
                     const int64_t* n_pts, int ptr_kind, void* stream, fe_batch** out);

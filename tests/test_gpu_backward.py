@@ -113,3 +113,23 @@ def test_config2_subset_matches_oracle():
     say("cfg2 subset backward")
     _, dP = esp_backward(t, pts, g["gV"], want_dG=False, n_func=w.basis.n)
     assert np.abs(dP - g["dP"]).max() < 1e-9 * max(1.0, np.abs(g["dP"]).max())
+
+
+def test_phase_profile_counts_every_class_launch():
+    from fastesp_amd import _lib as L_
+    from fastesp_amd.builder import build_table_hip as bth
+    ctx = L_.Context.get(0)
+    w = synth.config0(1500)
+    t = bth(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    present = [L for L, c in enumerate(t.info()["pairs_per_L"]) if c]
+    say(f"profile phases, classes {present}")
+    ctx.profile_start()
+    V, _ = esp_with_grad(t, w.points)
+    esp_backward(t, w.points, np.ones_like(V), want_dG=False)
+    ph = ctx.profile_stop()
+    for L in present:
+        assert ph[8 + L][1] == 1 and ph[8 + L][0] > 0
+        assert ph[16 + L][1] == 1 and ph[24 + L][1] == 1
+    assert ph[1][1] == 1 and ph[2][1] == 1 and ph[3][1] == 1
+    ctx.profile_start()
+    assert ctx.profile_stop() == {}

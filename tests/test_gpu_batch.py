@@ -100,10 +100,14 @@ def test_batch_step_matches_single_table_adjoint(confs):
         assert np.abs(dPs[i] - dPs[i].T).max() <= 1e-12 * scale
 
 
-def test_batch_mixed_within_bar(confs):
+@pytest.mark.parametrize("fp32_classes", ["default", "all"])
+def test_batch_mixed_within_bar(confs, fp32_classes):
     ws, tabs = confs
     b = Batch(tabs, [w.points for w in ws])
-    say("batch mixed forward + field")
+    if fp32_classes == "all":
+        for L in range(7):
+            b.set_tuning(_lib.FE_BT_MIXED0 + L, 1)
+    say(f"batch mixed forward + field, FP32 classes: {fp32_classes}")
     b.forward("mixed", field=True)
     for w, t, V, G in zip(ws, tabs, b.values(), b.field()):
         if len(w.points):

@@ -91,7 +91,11 @@ def test_tuning_knobs_keep_results(ctx):
     ctx.set_tuning(3, 0)
     V4 = t.esp()
     ctx.set_tuning(3, 1)
-    for V in (V1, V2, V3, V4):
+    say("pairs in reference order")
+    ctx.set_tuning(5, 0)
+    V5 = PairTable.from_flat(tab).esp()
+    ctx.set_tuning(5, 1)
+    for V in (V1, V2, V3, V4, V5):
         assert np.abs(V - ref).max() < TOL_FP64
         assert np.abs(V - V1).max() < 1e-12
 
