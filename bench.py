@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--ppl-classes", default="",
                     help="explicit points per lane per class, e.g. 4,2,2,2,2,2,1 (implies --no-tune)")
     ap.add_argument("--wg", type=int, default=0, help="explicit target workgroups per launch")
+    ap.add_argument("--fp32-classes", default="",
+                    help="explicit 0/1 per class: FP32 contraction under mixed, e.g. 0,0,0,1,1,1,1")
     ap.add_argument("--legs", default="fwd_fp64,fwdbwd_fp64,fwdbwd_mixed",
                     help="comma list of mode_precision legs to time")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -266,6 +268,10 @@ def main() -> int:
             ctx.set_tuning(10 + L, v)
     if a.wg:
         ctx.set_tuning(1, a.wg)
+    if a.fp32_classes:
+        a.tune = False
+        for L, v in enumerate(int(x) for x in a.fp32_classes.split(",")):
+            ctx.set_tuning(20 + L, v)
     # --- table build on the GPU (reported, not part of the step) ----------
     build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)  # warm-up
     torch.cuda.synchronize()
@@ -371,6 +377,8 @@ def main() -> int:
     # --- roofline: the dominant launch of the headline leg (VALU) ------------
     ppl = info["pairs_per_L"]
     fp32_used = [0, 0, 0, 1, 1, 1, 1]  # library default under mixed precision
+    if a.fp32_classes:
+        fp32_used = [int(x) for x in a.fp32_classes.split(",")]
     if key in tuned:
         ppl_used = tuned[key]["ppl_L"]
         fp32_used = tuned[key]["fp32_classes"] or fp32_used

@@ -44,7 +44,7 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
     const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
     double F[L + 1];
     if (!__any(p * r2 < fe_tfar(L))) {
-      const double ri = rsqrt(r2);
+      const double ri = fe_rsqrt_pos(r2);
       fe_boys_far_scaled<L>(hp * (ri * ri), F);
       w *= cF * ri;
     } else {

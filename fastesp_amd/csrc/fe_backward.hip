@@ -121,8 +121,9 @@ __global__ __launch_bounds__(256) void k_dG(const double* __restrict__ rec, cons
                                             const double* __restrict__ pw, const int n_pts,
                                             const int ppc, const double* __restrict__ boys_tab,
                                             double* __restrict__ partial) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += 256) s_tab[i] = boys_tab[i];
+  constexpr int NTAB = fe_boys_rows(L) * FE_BOYS_COLS;
+  __shared__ double s_tab[NTAB];
+  for (int i = threadIdx.x; i < NTAB; i += 256) s_tab[i] = boys_tab[i];
   __syncthreads();
   const int q0 = blockIdx.y * ppc;
   fe_dg_block<L>(rec, n_pairs, blockIdx.x * 256 + threadIdx.x, pw, q0, min(n_pts, q0 + ppc),

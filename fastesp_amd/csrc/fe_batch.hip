@@ -235,9 +235,10 @@ __global__ __launch_bounds__(256) void k_dG_batch(const FeDgItem* __restrict__ i
                                                   const double* __restrict__ pw,
                                                   const double* __restrict__ boys_tab,
                                                   double* __restrict__ part) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
+  constexpr int NTAB = fe_boys_rows(L) * FE_BOYS_COLS;
+  __shared__ double s_tab[NTAB];
   __shared__ int s_it;
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += 256) s_tab[i] = boys_tab[i];
+  for (int i = threadIdx.x; i < NTAB; i += 256) s_tab[i] = boys_tab[i];
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_it = atomicAdd(counter, 1);

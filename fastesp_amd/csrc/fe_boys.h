@@ -2,14 +2,15 @@
 //
 // Scheme (standard; Helgaker, Jorgensen & Olsen, "Molecular Electronic-Structure
 // Theory", sec. 9.8.2; Gill, Johnson & Pople, IJQC 40, 745 (1991)):
-//  * T < FE_BOYS_TSW: nearest grid point T_r = r * dT (dT = 1/8), Taylor series
+//  * T < fe_tfar(M): nearest grid point T_r = r * dT (dT = 1/8), Taylor series
 //    of the top order F_M(T) = sum_{k<9} (-1)^k F_{M+k}(T_r) (T-T_r)^k / k!
 //    (|T-T_r| <= 1/16, truncation < 4e-17 relative), exp(-T) = exp(-T_r) *
 //    exp(-(T-T_r)) with a degree-9 polynomial, then the stable downward
 //    recursion F_{m-1} = (2T F_m + exp(-T)) / (2m-1).
-//  * T >= FE_BOYS_TSW = 60: asymptotic F_0 = sqrt(pi/T)/2 and the upward
-//    recursion F_m = F_{m-1} (2m-1)/(2T); the neglected incomplete-gamma tail is
-//    below 1.3e-17 relative for m <= 8 at T = 60.
+//  * T >= fe_tfar(M) (35.5 .. 59 for top order M = 0 .. 8): asymptotic
+//    F_0 = sqrt(pi/T)/2 and the upward recursion F_m = F_{m-1} (2m-1)/(2T);
+//    the neglected incomplete-gamma tail is below 2^-54 relative for m <= M.
+//    The grid covers T < FE_BOYS_TSW = 60 for every order.
 // The reference evaluator switches at T = M - 0.5 between an 80-term Taylor
 // series at T = 0 and erf + upward recursion (fastesp/native/esp_gpu.cu:36-66);
 // that series cancels just below the switch (2.4e-13 relative at M = 7), which
@@ -53,6 +54,24 @@ __host__ __device__ constexpr double fe_tfar(int M) {
        : M == 5 ? 51.5 : M == 6 ? 54.0 : M == 7 ? 56.5 : 59.0;
 }
 
+// Grid rows a kernel of top order M reads: T < fe_tfar(M) takes the table
+// (nearest row r = floor(8 T + 1/2) <= 8 fe_tfar(M)), larger T the asymptotic
+// form, so only these rows are staged in LDS.
+__host__ __device__ constexpr int fe_boys_rows(int M) {
+  return (int)(fe_tfar(M) * FE_BOYS_INVDT) + 2 < FE_BOYS_NR ? (int)(fe_tfar(M) * FE_BOYS_INVDT) + 2
+                                                             : FE_BOYS_NR;
+}
+
+// 1/sqrt(x) for finite x > 0 (callers' arguments are bounded away from zero,
+// infinity and denormals): the hardware estimate refined by the third-order
+// Newton step y + y e (1/2 + 3/8 e), e = 1 - x y^2 -- the same step as the
+// math library's rsqrt, without its selects for special operands.
+__device__ __forceinline__ double fe_rsqrt_pos(const double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, fma(0.375, e, 0.5), y);
+}
+
 template <int M>
 __device__ __forceinline__ void fe_boys_far_scaled(const double x, double* F) {
   F[0] = 1.0;
@@ -60,11 +79,12 @@ __device__ __forceinline__ void fe_boys_far_scaled(const double x, double* F) {
   for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((2 * m - 1) * x);
 }
 
-// F[0..M] at T; `tab` is the (FE_BOYS_NR x FE_BOYS_COLS) table of top order M
-// (normally staged in LDS).
+// F[0..M] at T; `tab` holds at least the first fe_boys_rows(M) rows of the
+// table of top order M (normally staged in LDS).  The asymptotic form is used
+// from fe_tfar(M) on, where it is exact to double precision.
 template <int M>
 __device__ __forceinline__ void fe_boys(const double T, double* F, const double* tab) {
-  if (T < FE_BOYS_TSW) {
+  if (T < fe_tfar(M)) {
     const int r = (int)fma(T, FE_BOYS_INVDT, 0.5);
     const double dt = fma(-FE_BOYS_DT, (double)r, T);
     const double* c = tab + r * FE_BOYS_COLS;
@@ -85,7 +105,7 @@ __device__ __forceinline__ void fe_boys(const double T, double* F, const double*
       for (int m = M; m > 0; --m) F[m - 1] = fma(t2, F[m], e) * (1.0 / (2 * m - 1));
     }
   } else {
-    const double r = rsqrt(T);
+    const double r = fe_rsqrt_pos(T);
     F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
     if constexpr (M > 0) {
       const double x = 0.5 / T;

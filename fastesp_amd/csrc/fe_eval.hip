@@ -15,8 +15,9 @@ __global__ __launch_bounds__(FE_WG) void k_eval(const double* __restrict__ rec,
                                                 double* __restrict__ partial, const int row0,
                                                 const int ld, const size_t plane,
                                                 const float tau) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
+  constexpr int NTAB = fe_boys_rows(GRAD ? L + 1 : L) * FE_BOYS_COLS;
+  __shared__ double s_tab[NTAB];
+  for (int i = threadIdx.x; i < NTAB; i += FE_WG) s_tab[i] = boys_tab[i];
   __syncthreads();
   const int chunk = blockIdx.y;
   const int p0 = chunk * ppc;
@@ -38,9 +39,10 @@ __global__ __launch_bounds__(FE_WG) void k_batch_eval(const FeBatchItem* __restr
                                                       const double* __restrict__ boys_tab,
                                                       double* __restrict__ partial,
                                                       const size_t plane, const float tau) {
-  __shared__ double s_tab[FE_BOYS_NR * FE_BOYS_COLS];
+  constexpr int NTAB = fe_boys_rows(GRAD ? L + 1 : L) * FE_BOYS_COLS;
+  __shared__ double s_tab[NTAB];
   __shared__ int s_it;
-  for (int i = threadIdx.x; i < FE_BOYS_NR * FE_BOYS_COLS; i += FE_WG) s_tab[i] = boys_tab[i];
+  for (int i = threadIdx.x; i < NTAB; i += FE_WG) s_tab[i] = boys_tab[i];
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_it = atomicAdd(counter, 1);

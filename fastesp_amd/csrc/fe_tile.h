@@ -57,7 +57,7 @@ __device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
       double w = K;
       if (far) {
         // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
-        const double ri = rsqrt(r2[q]);
+        const double ri = fe_rsqrt_pos(r2[q]);
         fe_boys_far_scaled<M>(hp * (ri * ri), F);
         w = KF * ri;
       } else {
