@@ -506,8 +506,10 @@ static int plan_fwd(fe_batch* b, bool grad, bool mixed, hipStream_t st) {
       const int64_t ppc = (nL + ch - 1) / ch;
       ch = (nL + ppc - 1) / ppc;
       const int n_pb = (n + blk - 1) / blk;
-      for (int q = 0; q < n_pb; ++q)
-        for (int64_t k = 0; k < ch; ++k) {
+      // chunk-major: workgroups that run together read the same pair records
+      // (scalar loads served by the shared scalar cache)
+      for (int64_t k = 0; k < ch; ++k)
+        for (int q = 0; q < n_pb; ++q) {
           FeBatchItem it{};
           it.out = (int64_t)plane + (int64_t)(rows + k) * ld + (int64_t)q * blk;
           it.conf = i;
@@ -632,8 +634,9 @@ static int plan_bwd(fe_batch* b) {
       sg.n_chunks = (int)ch;
       segs[L].push_back(sg);
       const int nb = (int)((nL + 255) / 256);
-      for (int pb = 0; pb < nb; ++pb)
-        for (int64_t k = 0; k < ch; ++k) {
+      // point-chunk-major: workgroups that run together read the same points
+      for (int64_t k = 0; k < ch; ++k)
+        for (int pb = 0; pb < nb; ++pb) {
           FeDgItem d{};
           d.out = (int64_t)part_total + k * NT * nL;
           d.conf = i;

@@ -23,8 +23,9 @@ __global__ __launch_bounds__(FE_WG) void k_eval(const double* __restrict__ rec,
   const int p0 = chunk * ppc;
   const int p1 = min(n_pairs, p0 + ppc);
   const int base = blockIdx.x * PPL * FE_WG;
-  fe_eval_block<L, PPL, GRAD, MIXED>(rec, g32, p0, p1, pts, base, n_pts, s_tab, tau,
-                                     partial + (size_t)(row0 + chunk) * ld + base, plane);
+  fe_eval_block<L, PPL, GRAD, MIXED>((fe_gdptr)rec, (fe_gfptr)g32, p0, p1, pts, base, n_pts,
+                                     s_tab, tau, partial + (size_t)(row0 + chunk) * ld + base,
+                                     plane);
 }
 
 // Persistent: each workgroup takes the next item of the queue until it is
@@ -50,7 +51,8 @@ __global__ __launch_bounds__(FE_WG) void k_batch_eval(const FeBatchItem* __restr
     const int it = __builtin_amdgcn_readfirstlane(s_it);
     if (it >= n_items) break;
     const FeBatchItem b = items[it];
-    fe_eval_block<L, PPL, GRAD, MIXED>(recs[b.conf], MIXED ? g32s[b.conf] : nullptr, b.p0, b.p1,
+    fe_eval_block<L, PPL, GRAD, MIXED>((fe_gdptr)recs[b.conf],
+                                       (fe_gfptr)(MIXED ? g32s[b.conf] : nullptr), b.p0, b.p1,
                                        pts, b.base, b.end, s_tab, tau, partial + b.out, plane);
   }
 }

@@ -24,8 +24,8 @@ __device__ __forceinline__ void fe_kahan_add(double& s, double& c, const double 
 }
 
 template <int L, int PPL, bool GRAD, bool MIXED>
-__device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
-                                             const float* __restrict__ g32, const int p0,
+__device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
+                                             fe_gfptr __restrict__ g32, const int p0,
                                              const int p1, const double* cx, const double* cy,
                                              const double* cz, const double* s_tab,
                                              const float tau, double* av, double* cv, double* ax,
@@ -33,7 +33,7 @@ __device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
   constexpr int S = FeL<L>::STRIDE, S32 = FE_S32(FeL<L>::NT);
   constexpr int M = GRAD ? L + 1 : L;
   for (int ip = p0; ip < p1; ++ip) {
-    const double* __restrict__ r = rec + (size_t)ip * S;
+    const fe_gdptr r = rec + (size_t)ip * S;
     const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
     double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
     int near = 0, big = 0;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
     bool fp64 = true;
     if constexpr (MIXED) fp64 = __any(big);
     const double hp = r[6], KF = K * r[7];
-    const float* __restrict__ g = g32 + (size_t)ip * S32;
+    const fe_gfptr g = g32 + (size_t)ip * S32;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       double F[M + 1];
@@ -96,8 +96,8 @@ __device__ __forceinline__ void fe_eval_tile(const double* __restrict__ rec,
 // tile, and write the partial sums (value plane; three more planes with GRAD)
 // for the points below `end`.
 template <int L, int PPL, bool GRAD, bool MIXED>
-__device__ __forceinline__ void fe_eval_block(const double* __restrict__ rec,
-                                              const float* __restrict__ g32, const int p0,
+__device__ __forceinline__ void fe_eval_block(fe_gdptr __restrict__ rec,
+                                              fe_gfptr __restrict__ g32, const int p0,
                                               const int p1, const double* __restrict__ pts,
                                               const int base, const int end,
                                               const double* s_tab, const float tau,

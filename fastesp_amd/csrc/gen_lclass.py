@@ -119,7 +119,7 @@ def gen_L(L):
     # ---------------- forward contraction, FP64
     o.append(f"template <> __device__ __forceinline__ double fe_contract<{L}>("
              "const double Rx, const double Ry, const double Rz, const double* F, "
-             "const double* __restrict__ g) {")
+             "fe_gdptr __restrict__ g) {")
     emit_monos(o, L, order, parent, axis)
     seen = set()
     for k, (ax, ay, az, m) in enumerate(keys):
@@ -140,7 +140,7 @@ def gen_L(L):
     # I = sum_k g R^a F_m ; dI/dC = sum_k g [ d(R^a) F_m - 2p R R^a F_{m+1} ]
     o.append(f"template <> __device__ __forceinline__ void fe_contract_grad<{L}>("
              "const double Rx, const double Ry, const double Rz, const double* F, "
-             "const double* __restrict__ g, const double p, double& I, double& Gx, "
+             "fe_gdptr __restrict__ g, const double p, double& I, double& Gx, "
              "double& Gy, double& Gz) {")
     emit_monos(o, L, order, parent, axis)
     midx = {m: i for i, m in enumerate(order)}
@@ -197,7 +197,7 @@ def gen_L(L):
     # ---------------- mixed: FP32 monomials and per-m polynomials, FP64 F and final sum
     o.append(f"template <> __device__ __forceinline__ double fe_contract_f32<{L}>("
              "const float Rx, const float Ry, const float Rz, const double* F, "
-             "const float* __restrict__ g) {")
+             "fe_gfptr __restrict__ g) {")
     emit_monos(o, L, order, parent, axis, T="float")
     seen = set()
     for k, (ax, ay, az, m) in enumerate(keys):
@@ -217,7 +217,7 @@ def gen_L(L):
     # ---------------- mixed gradient: FP32 monomials/polynomials, FP64 F, p and sums
     o.append(f"template <> __device__ __forceinline__ void fe_contract_grad_f32<{L}>("
              "const float Rx, const float Ry, const float Rz, const double* F, "
-             "const float* __restrict__ g, const double p, const double Rxd, const double Ryd, "
+             "fe_gfptr __restrict__ g, const double p, const double Rxd, const double Ryd, "
              "const double Rzd, double& I, double& Gx, double& Gy, double& Gz) {")
     emit_monos(o, L, order, parent, axis, T="float")
     lines = []
@@ -260,12 +260,18 @@ def main():
     out = ["// GENERATED by gen_lclass.py -- do not edit by hand.",
            f"// Per-L straight-line code for L = 0..{lmax}.", "#pragma once",
            f"#define FE_LMAX {lmax}",
+           "// Pair records are read-only while a kernel runs.  Reading them through",
+           "// constant-address-space pointers keeps their wave-uniform loads scalar",
+           "// (s_load) also when the record pointer itself is read from memory",
+           "// (batched evaluation), where the compiler cannot prove them unclobbered.",
+           "typedef const double __attribute__((address_space(4)))* fe_gdptr;",
+           "typedef const float __attribute__((address_space(4)))* fe_gfptr;",
            "template <int L> struct FeL;",
-           "template <int L> __device__ __forceinline__ double fe_contract(const double, const double, const double, const double*, const double* __restrict__);",
-           "template <int L> __device__ __forceinline__ void fe_contract_grad(const double, const double, const double, const double*, const double* __restrict__, const double, double&, double&, double&, double&);",
+           "template <int L> __device__ __forceinline__ double fe_contract(const double, const double, const double, const double*, fe_gdptr __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_contract_grad(const double, const double, const double, const double*, fe_gdptr __restrict__, const double, double&, double&, double&, double&);",
            "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, const double, double* __restrict__);",
-           "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, const float* __restrict__);",
-           "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, const float* __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
+           "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, fe_gfptr __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, fe_gfptr __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, koff, moff = [], [], [0], [0]
