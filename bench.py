@@ -53,8 +53,9 @@ def parse():
     ap.add_argument("--wg", type=int, default=0, help="explicit target workgroups per launch")
     ap.add_argument("--fp32-classes", default="",
                     help="explicit 0/1 per class: FP32 contraction under mixed, e.g. 0,0,0,1,1,1,1")
-    ap.add_argument("--legs", default="fwd_fp64,fwdbwd_fp64,fwdbwd_mixed",
-                    help="comma list of mode_precision legs to time")
+    ap.add_argument("--legs", default="fwd_fp64,fwdbwd_fp64,fwdbwd_mixed,fwdbwd_mixed32",
+                    help="comma list of mode_precision legs to time (mixed32: every class "
+                         "through the FP32 contraction)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--batch", type=int, default=256,
@@ -77,7 +78,7 @@ PHASE_NAMES.update({24 + L: f"dG_reduce_L{L}" for L in range(7)})
 
 
 def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, flops,
-                             traffic_file):
+                             traffic_file, counters_file):
     """Roofline of the launch that takes the most time in the timed region.
 
     phases: {phase id: (total ms, launches)} from fe_profile over the timed
@@ -112,17 +113,30 @@ def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, 
         peak = 1.0 / ((1 - share32) / flops.PEAK_FP64_VALU + share32 / flops.PEAK_FP32_VALU)
     else:
         peak = flops.PEAK_FP64_VALU
-    traffic = None
-    if traffic_file.exists():
-        kv = dict(line.split("=", 1) for line in traffic_file.read_text().splitlines()
-                  if "=" in line and not line.startswith(("#", "`", " ")))
-        if kv.get("kernel", "").strip() == name:
-            traffic = int(float(kv.get("traffic_bytes_per_launch", "nan")))
-    return dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
-                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic, kernel=name,
-                launches=cnt, avg_launch_ms=round(avg_ms, 4), flops_per_launch=fl,
-                flop_model=f"{kind}: N x pairs(L={L}) x {flops.__name__}."
-                           + {"fwd": "f_fwd", "fwd_grad": "f_eval_grad", "dg": "f_dg"}[kind])
+    def profile_values(path):
+        if not path.exists():
+            return {}
+        kv = dict((k.strip(), v.strip()) for k, v in
+                  (line.split("=", 1) for line in path.read_text().splitlines()
+                   if "=" in line and not line.startswith(("#", "`", " ", "|"))))
+        return kv if kv.get("kernel") == name else {}
+
+    tv = profile_values(traffic_file)
+    traffic = int(float(tv["traffic_bytes_per_launch"])) if "traffic_bytes_per_launch" in tv else None
+    out = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
+               unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic, kernel=name,
+               launches=cnt, avg_launch_ms=round(avg_ms, 4), flops_per_launch=fl,
+               flop_model=f"{kind}: N x pairs(L={L}) x {flops.__name__}."
+                          + {"fwd": "f_fwd", "fwd_grad": "f_eval_grad", "dg": "f_dg"}[kind])
+    # executed-instruction cross-check (rocprofv3 SQ_INSTS_VALU_* summary of the same launch)
+    cv = profile_values(counters_file)
+    if "executed_f64_flops_per_launch" in cv:
+        ex = float(cv["executed_f64_flops_per_launch"]) + float(cv.get("executed_f32_flops_per_launch", 0))
+        out.update(executed_flops_per_launch=ex,
+                   executed_tflops=round(ex / (avg_ms * 1e-3) / 1e12, 3),
+                   executed_frac=round(ex / (avg_ms * 1e-3) / peak, 4),
+                   valu_issue=float(cv.get("valu_issue", "nan")))
+    return out
 
 
 def cpu_baseline(w, budget_s: float) -> dict:
@@ -321,10 +335,15 @@ def main() -> int:
     tuned = {}
 
     def timed(mode, prec):
+        force32 = prec == "mixed32"
+        if force32:
+            prec = "mixed"
+            for L in range(7):
+                ctx.set_tuning(20 + L, 1)
         if a.tune:
             from fastesp_amd import autotune
-            r = autotune.tune(tab, pts, op=mode, precision=PREC[prec])
-            tuned[f"{mode}_{prec}"] = dict(ppl_L=r["best"]["ppl_L"],
+            r = autotune.tune(tab, pts, op=mode, precision=PREC[prec], fix_mixed=force32)
+            tuned[f"{mode}_{'mixed32' if force32 else prec}"] = dict(ppl_L=r["best"]["ppl_L"],
                                            target_wg=r["best"]["target_wg"],
                                            fp32_classes=r["best"]["mixed_L"],
                                            trials=len(r["all"]))
@@ -353,8 +372,13 @@ def main() -> int:
             d = V.cpu().numpy()[g["idx"]] - g["V"]
             res["max_abs_dV_au"] = float(np.abs(d).max())
             res["rms_dV_au"] = float(np.sqrt((d * d).mean()))
-        log(f"{mode} {prec}: {ms:.3f} ms/step = {res['ms_per_1e4_pts']:.3f} ms/1e4 pts, "
+        log(f"{mode} {prec}{' (FP32 in every class)' if force32 else ''}: {ms:.3f} ms/step = "
+            f"{res['ms_per_1e4_pts']:.3f} ms/1e4 pts, "
             f"max |dV| {res.get('max_abs_dV_au', float('nan')):.2e}")
+        if force32:
+            res["fp32_classes"] = [1] * 7
+            for L in range(7):  # back to the library default
+                ctx.set_tuning(20 + L, int(L >= 3))
         return res
 
     legs = {}
@@ -387,8 +411,9 @@ def main() -> int:
     else:
         ppl_used = [a.ppl or 2] * 7
     traffic_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_hbm_traffic.md"
+    counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
     roof = dominant_kernel_roofline(head["_phases"], ppl, n, a.mode, a.precision, ppl_used,
-                                    fp32_used, flops, traffic_file)
+                                    fp32_used, flops, traffic_file, counters_file)
     # whole step, for reference: all launches of one step against the blended peak
     f_fwd = flops.forward_flops(ppl, n, info["n_atoms"])
     f_bwd = flops.backward_flops(ppl, n, info["n_atoms"]) if a.mode == "fwdbwd" else 0.0

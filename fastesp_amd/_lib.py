@@ -95,6 +95,13 @@ def load():
         if not LIB_PATH.exists():
             raise HipBackendError(
                 f"{LIB_PATH} is not built; run `python -m fastesp_amd.build` first")
+        # The PyTorch ROCm wheel ships its own libamdhip64 under the same soname.
+        # Loading it first makes it the one HIP runtime of the process; the other
+        # order leaves torch bound to a runtime it does not find devices with.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(str(LIB_PATH))
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)

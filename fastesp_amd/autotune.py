@@ -77,8 +77,9 @@ def apply_cached(ctx, table, n_points: int, op: str = "fwdbwd",
 
 
 def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP64,
-         reps: int = 3, verbose: bool = False) -> dict:
-    """Time launch shapes on device-resident points (a torch float64 tensor)."""
+         reps: int = 3, verbose: bool = False, fix_mixed: bool = False) -> dict:
+    """Time launch shapes on device-resident points (a torch float64 tensor).
+    fix_mixed keeps the context's per-class FP32 choice instead of tuning it."""
     import torch
     ctx = table._ctx
     lib = ctx.lib
@@ -122,8 +123,8 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
 
     log = []
     ppl = [2] * NCLS
-    mixed = precision == _lib.FE_PREC_MIXED
-    mix = list(DEFAULT_MIXED_L)
+    mixed = precision == _lib.FE_PREC_MIXED and not fix_mixed
+    mix = [1] * NCLS if fix_mixed else list(DEFAULT_MIXED_L)
     best_wg, best_ms = None, float("inf")
     for wg in WG_CHOICES:
         ms = measure(ppl, wg, mix)
@@ -142,7 +143,8 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
                 if ms < best_ms:
                     ppl, mix, best_ms = trial, tmix, ms
     apply(ctx, ppl, best_wg, mix)
-    best = dict(ppl_L=ppl, target_wg=best_wg, ms=best_ms, mixed_L=mix if mixed else None)
+    best = dict(ppl_L=ppl, target_wg=best_wg, ms=best_ms,
+                mixed_L=mix if precision == _lib.FE_PREC_MIXED else None)
     k = shape_key(table.info()["pairs_per_L"], n, op, precision)
     d = _load()
     d[k] = dict(best, trials=len(log))

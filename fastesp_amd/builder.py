@@ -20,9 +20,18 @@ SCREEN_CPP = 1     # fastesp/native/build_pairs.cpp:318-328
 
 def build_table_hip(wfn, basis, density, points, tol: float = 1e-12, progress: bool = False,
                     rule: int = SCREEN_NUMPY, ctx: _lib.Context | None = None) -> PairTable:
-    ctx = ctx or _lib.Context.get()
     pts = np.ascontiguousarray(np.asarray(points, float).reshape(-1, 3))
-    pl = basis_payload(wfn, basis, density, pts)
+    t = build_table_payload(basis_payload(wfn, basis, density, pts), tol, rule, ctx)
+    if progress:
+        print(f"  pairs kept {t.n_pairs}")
+    return t
+
+
+def build_table_payload(pl: dict, tol: float = 1e-12, rule: int = SCREEN_NUMPY,
+                        ctx: _lib.Context | None = None) -> PairTable:
+    """Build from the basis.bin fields (``read_basis_payload`` / ``basis_payload``)."""
+    ctx = ctx or _lib.Context.get()
+    pts = np.ascontiguousarray(np.asarray(pl["pts"], float).reshape(-1, 3))
     i32 = {k: np.ascontiguousarray(pl[k], np.int32) for k in
            ("fn_atom", "fn_nprim", "fn_off", "fn_lx", "fn_ly", "fn_lz", "fn_shell")}
     f64 = {k: np.ascontiguousarray(pl[k], np.float64) for k in
@@ -38,8 +47,6 @@ def build_table_hip(wfn, basis, density, points, tol: float = 1e-12, progress: b
         C.byref(h)), "fe_table_build")
     t = PairTable(h, ctx, pts, f64["Z"], f64["xyz"])
     t.meta["sum_rule"] = t.stats()["sum_rule"]
-    if progress:
-        print(f"  pairs kept {t.n_pairs}")
     return t
 
 

@@ -1,0 +1,159 @@
+"""Command-line drop-ins for the reference's native tools and its ESP driver.
+
+The reference runs the path as two processes that exchange files
+(``fastesp/cli.py:143-163``): ``build_pairs <basis.bin> <pairs.bin> [tol]``
+(``fastesp/native/build_pairs.cpp:162-168``) and ``esp_gpu <pairs.bin>
+<out.txt> [--repeats N] [--nchunk N] [--quiet]`` (``fastesp/native/
+esp_gpu.cu:232-242``).  The same arguments, files and console protocol here:
+
+    python -m fastesp_amd build-pairs basis.bin pairs.bin [tol]
+    python -m fastesp_amd esp-gpu pairs.bin out.txt [--repeats N] [--quiet] [--precision mixed]
+    python -m fastesp_amd esp basis.bin out.txt [--tol T] [--screen numpy|cpp] [--repeats N]
+
+* build-pairs builds the table on the GPU with the C++ tool's screen
+  (``build_pairs.cpp:318-328``) and writes pairs.bin in reference order;
+  stderr carries the tool's summary lines (shells, pairs, sum rule).
+* esp-gpu evaluates a pairs.bin (from either builder), prints
+  ``pairs=.. terms=.. points=.. atoms=.. max_mono=..``, one ``run i: X ms``
+  line per repeat after a warm-up (the line ``fastesp/cli.py:166-168``
+  parses), and writes ``N`` then `` x y z V`` lines in the reference's format
+  (``esp_gpu.cu:318-323``).
+* esp does both in one process (the table never leaves the device) from a
+  basis.bin payload (``fastesp/table.py:268-316``), with the NumPy screen by
+  default as ``build_table`` uses.
+
+Errors print a message on stderr and return 1, as the native tools do.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+
+def _write_out(path, pts: np.ndarray, V: np.ndarray) -> None:
+    with Path(path).open("w") as f:
+        f.write(f"{len(pts)}\n")
+        for (x, y, z), v in zip(pts, V):
+            f.write(f" {x:20.12f} {y:20.12f} {z:20.12f} {v:22.14e}\n")
+
+
+def _timed_runs(table, pts: np.ndarray, precision: int, repeats: int, quiet: bool) -> np.ndarray:
+    """Warm-up, then `repeats` timed evaluations with the points resident on
+    the device (as esp_gpu times them: launch to synchronisation)."""
+    import torch
+    lib = table._ctx.lib
+    n = len(pts)
+    dev = torch.device("cuda", table._ctx.device)
+    d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
+    d_V = torch.empty(max(n, 1), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def run():
+        _lib.check(lib.fe_esp_forward(table._h, d_pts.data_ptr(), n, precision, _lib.FE_PART_ESP,
+                                      _lib.FE_PTR_DEVICE, stream.cuda_stream, d_V.data_ptr()),
+                   "fe_esp_forward")
+
+    run()
+    torch.cuda.synchronize(dev)
+    for r in range(repeats):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize(dev)
+        if not quiet:
+            print(f"run {r}: {(time.perf_counter() - t0) * 1e3:.3f} ms", flush=True)
+    return d_V[:n].cpu().numpy()
+
+
+def _precision(name: str) -> int:
+    return {"fp64": _lib.FE_PREC_FP64, "mixed": _lib.FE_PREC_MIXED}[name]
+
+
+def cmd_build_pairs(a) -> int:
+    from .builder import SCREEN_CPP, build_table_payload
+    from .formats import read_basis_payload
+    pl = read_basis_payload(a.basis)
+    n_shells = len(np.unique(pl["fn_shell"]))
+    print(f"shells {n_shells}  functions {pl['n_func']}", file=sys.stderr)
+    t = build_table_payload(pl, tol=a.tol, rule=SCREEN_CPP)
+    flat = t.to_flat()
+    t.write(a.pairs)
+    print(f"pairs {flat.n_pairs} terms {flat.n_terms} mono {len(flat.mo_parent)}", file=sys.stderr)
+    print(f"sum rule Tr(P S) = {t.stats()['sum_rule']:.9f}  (must equal the MOLDEN electron count)",
+          file=sys.stderr)
+    return 0
+
+
+def cmd_esp_gpu(a) -> int:
+    from .formats import read_pairs
+    from .table import PairTable
+    flat = read_pairs(a.pairs)
+    if not a.quiet:
+        print(f"pairs={flat.n_pairs} terms={flat.n_terms} points={len(flat.pts)} "
+              f"atoms={len(flat.Z)} max_mono={flat.max_mono}", flush=True)
+    t = PairTable.from_flat(flat)
+    V = _timed_runs(t, flat.pts, _precision(a.precision), a.repeats, a.quiet)
+    _write_out(a.out, flat.pts, V)
+    if not a.quiet:
+        print(f"wrote {a.out}")
+    return 0
+
+
+def cmd_esp(a) -> int:
+    from .builder import SCREEN_CPP, SCREEN_NUMPY, build_table_payload
+    from .formats import read_basis_payload
+    pl = read_basis_payload(a.basis)
+    pts = np.asarray(pl["pts"], float).reshape(-1, 3)
+    t0 = time.perf_counter()
+    t = build_table_payload(pl, tol=a.tol, rule=SCREEN_CPP if a.screen == "cpp" else SCREEN_NUMPY)
+    t_build = time.perf_counter() - t0
+    if not a.quiet:
+        print(f"  backend      : hip", flush=True)
+        print(f"  basis        : {pl['n_func']} Cartesian functions, {pl['n_prim']} primitives")
+        print(f"  pair table   : {t_build:8.4f} s  ({t.n_pairs} pairs, point-independent)")
+    V = _timed_runs(t, pts, _precision(a.precision), a.repeats, a.quiet)
+    if not a.quiet:
+        print(f"  ESP range    : {V.min() * 27.211386245988:+.4f} .. {V.max() * 27.211386245988:+.4f} V")
+    _write_out(a.out, pts, V)
+    if not a.quiet:
+        print(f"  wrote        : {a.out}")
+    return 0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m fastesp_amd", description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("build-pairs", help="basis.bin -> pairs.bin on the GPU (build_pairs)")
+    b.add_argument("basis")
+    b.add_argument("pairs")
+    b.add_argument("tol", nargs="?", type=float, default=1e-12)
+    b.set_defaults(fn=cmd_build_pairs)
+    e = sub.add_parser("esp-gpu", help="pairs.bin -> out.txt (esp_gpu)")
+    e.add_argument("pairs")
+    e.add_argument("out")
+    e.add_argument("--repeats", type=int, default=1)
+    e.add_argument("--nchunk", type=int, default=0, help="accepted for compatibility; the "
+                   "launch shape comes from the autotuned context")
+    e.add_argument("--quiet", action="store_true")
+    e.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    e.set_defaults(fn=cmd_esp_gpu)
+    s = sub.add_parser("esp", help="basis.bin -> out.txt in one process")
+    s.add_argument("basis")
+    s.add_argument("out")
+    s.add_argument("--tol", type=float, default=1e-12)
+    s.add_argument("--screen", choices=["numpy", "cpp"], default="numpy")
+    s.add_argument("--repeats", type=int, default=1)
+    s.add_argument("--quiet", action="store_true")
+    s.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    s.set_defaults(fn=cmd_esp)
+    a = ap.parse_args(argv)
+    try:
+        return a.fn(a)
+    except (OSError, ValueError, _lib.HipBackendError) as ex:
+        print(f"fastesp_amd {a.cmd}: {ex}", file=sys.stderr)
+        return 1

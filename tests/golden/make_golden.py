@@ -137,8 +137,14 @@ def make_cfg2_sub():
     fld = evaluate.field(tab, pts[:64])
     dG = evaluate.grad_G(tab, pts, gV)
     dP = evaluate.grad_P(tab, w.basis, dG)
-    np.savez_compressed(OUT / "cfg2_sub.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx, V=V, w=wts,
-                        target=target, gV=gV, field64=fld, dP=dP.astype(np.float64),
+    # a sample of dL/dP (the full 1634 x 1634 matrix would be ~20 MB): 4096
+    # random entries, the diagonal and every row sum
+    n = dP.shape[0]
+    ij = np.stack([rng.integers(0, n, 4096), rng.integers(0, n, 4096)], axis=1)
+    np.savez_compressed(OUT / "cfg2_sub.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx,
+                        V=V, w=wts, target=target, gV=gV, field64=fld, dP_ij=ij,
+                        dP_sample=dP[ij[:, 0], ij[:, 1]], dP_diag=np.diag(dP).copy(),
+                        dP_rowsum=dP.sum(axis=1), dP_absmax=np.abs(dP).max(),
                         npairs=tab.n_pairs)
 
 

@@ -112,7 +112,12 @@ def test_config2_subset_matches_oracle():
     assert np.abs(G[:64] - g["field64"]).max() < 1e-9
     say("cfg2 subset backward")
     _, dP = esp_backward(t, pts, g["gV"], want_dG=False, n_func=w.basis.n)
-    assert np.abs(dP - g["dP"]).max() < 1e-9 * max(1.0, np.abs(g["dP"]).max())
+    scale = float(g["dP_absmax"])  # relative to the largest entry
+    ij = g["dP_ij"]
+    assert np.abs(dP[ij[:, 0], ij[:, 1]] - g["dP_sample"]).max() < 1e-9 * scale
+    assert np.abs(np.diag(dP) - g["dP_diag"]).max() < 1e-9 * scale
+    assert np.abs(dP.sum(axis=1) - g["dP_rowsum"]).max() < 1e-8 * scale
+    assert abs(np.abs(dP).max() - float(g["dP_absmax"])) < 1e-9 * scale
 
 
 def test_phase_profile_counts_every_class_launch():
