@@ -155,8 +155,8 @@ def tune(table, points_dev, op: str = "fwdbwd", precision: int = _lib.FE_PREC_FP
 # ---------------------------------------------------------------------------
 # batched conformers
 # ---------------------------------------------------------------------------
-ITEMS_CHOICES = (4, 8, 16, 32, 64)
-DG_ITEMS_CHOICES = (4, 8, 16, 32, 64)
+ITEMS_CHOICES = (8, 16, 32, 64, 128)
+DG_ITEMS_CHOICES = (8, 16, 32, 64, 128)
 
 
 def tune_batch(batch, precision="mixed", field: bool = True, dldp: bool = True, reps: int = 2,
