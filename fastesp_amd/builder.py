@@ -16,6 +16,7 @@ from .table import PairTable
 
 SCREEN_NUMPY = 0   # fastesp/table.py:190-194
 SCREEN_CPP = 1     # fastesp/native/build_pairs.cpp:318-328
+SCREEN_ENGINE = 2  # fastesp/esp.py:265-270 (ESPEngine.esp_electron)
 
 
 def build_table_hip(wfn, basis, density, points, tol: float = 1e-12, progress: bool = False,

@@ -15,6 +15,7 @@
 //  kernel : k_screen, one lane per candidate, points streamed through LDS:
 //           rule 0 (NumPy, table.py:190-194) keep = !(K bound/max(dmin,0.05) < tol)
 //           rule 1 (C++, build_pairs.cpp:318-328) keep = !(K bound/max(dmin-reff,0.05) < tol)
+//           rule 2 (ESPEngine, esp.py:265-270)    keep = !(K bound/max(dmax-reff,0.05) < tol)
 //                  and G not all zero and c_i c_j != 0.
 //  compact: kept candidates are packed in candidate (= reference) order.
 #include <hip/hip_runtime.h>
@@ -215,15 +216,25 @@ __global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, 
   const int ii = min(i, max(n_cand - 1, 0));
   const double* r = rec + (size_t)ii * S;
   const double Px = r[0], Py = r[1], Pz = r[2];
-  double d2min = INFINITY;
+  // rule 2 screens with the largest distance to any point, the others with
+  // the smallest (the block-uniform rule picks one reduction)
+  const bool use_max = rule == 2;
+  double d2sel = use_max ? -INFINITY : INFINITY;
   for (int base = 0; base < n_pts; base += 1024) {
     const int m = min(1024, n_pts - base);
     __syncthreads();
     for (int q = threadIdx.x; q < 3 * m; q += 256) s_p[q] = pts[3 * (size_t)base + q];
     __syncthreads();
-    for (int q = 0; q < m; ++q) {
-      const double x = s_p[3 * q] - Px, y = s_p[3 * q + 1] - Py, z = s_p[3 * q + 2] - Pz;
-      d2min = fmin(d2min, x * x + y * y + z * z);
+    if (use_max) {
+      for (int q = 0; q < m; ++q) {
+        const double x = s_p[3 * q] - Px, y = s_p[3 * q + 1] - Py, z = s_p[3 * q + 2] - Pz;
+        d2sel = fmax(d2sel, x * x + y * y + z * z);
+      }
+    } else {
+      for (int q = 0; q < m; ++q) {
+        const double x = s_p[3 * q] - Px, y = s_p[3 * q + 1] - Py, z = s_p[3 * q + 2] - Pz;
+        d2sel = fmin(d2sel, x * x + y * y + z * z);
+      }
     }
   }
   if (i >= n_cand) return;
@@ -231,15 +242,19 @@ __global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, 
   int k = flags & 1;
   if (k) {
     const double kb = r[5];
-    double dd = sqrt(d2min);
+    double dd = sqrt(fmax(d2sel, 0.0));
     if (rule == 1) {
       dd -= r[6];
       k = (flags & 2) && (flags & 4);
+    } else if (rule == 2) {
+      dd -= r[6];
     }
     if (dd < 0.05) dd = 0.05;
     if (kb / dd < tol) k = 0;
   }
-  keep[i] = k;
+  // 1 kept, 0 a candidate the screen dropped, -1 no candidate (K < 1e-30 or an
+  // empty density block: not in the reference's primitive-pair list)
+  keep[i] = (flags & 1) ? k : -1;
 }
 
 __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, const int S,
@@ -293,7 +308,7 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   if (!fn_atom || !fn_nprim || !fn_off || !fn_lx || !fn_ly || !fn_lz || !fn_shell || !fc ||
       !pexp || !pcoef || !dens || (n_points && !pts) || (n_atoms && (!Z || !xyz)))
     return fe_set_error(FE_ERR_ARG, "fe_table_build: null array");
-  if (rule != 0 && rule != 1) return fe_set_error(FE_ERR_ARG, "fe_table_build: rule must be 0 or 1");
+  if (rule < 0 || rule > 2) return fe_set_error(FE_ERR_ARG, "fe_table_build: rule must be 0, 1 or 2");
   if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
     return fe_set_error(FE_ERR_ARG, "fe_table_build: bad ptr_kind");
   if (!(tol >= 0.0)) return fe_set_error(FE_ERR_ARG, "fe_table_build: tol must be >= 0");
@@ -509,8 +524,10 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (!ncand[L]) continue;
     std::vector<int> idx, s2k(ncand[L], -1);
-    for (int64_t q = 0; q < ncand[L]; ++q)
-      if (hk[L][q]) idx.push_back((int)q);
+    for (int64_t q = 0; q < ncand[L]; ++q) {
+      if (hk[L][q] > 0) idx.push_back((int)q);
+      if (hk[L][q] >= 0) ++t->n_candidates;
+    }
     // original (reference-order) position of every kept record, for export:
     // candidate slot -> global candidate index via its shell pair (slots of
     // one class are contiguous per shell pair in shell-pair order)

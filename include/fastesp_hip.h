@@ -96,7 +96,9 @@ int fe_table_from_pairs(fe_ctx* ctx, int n_pairs, int n_terms,
  *         (fastesp/table.py:190-194);
  * rule 1: the C++ screen, keep = !(K bound / max(dmin - reff, 0.05) < tol) and
  *         G not all zero and c_i c_j != 0 (fastesp/native/build_pairs.cpp:
- *         243,309,318-328).
+ *         243,309,318-328);
+ * rule 2: the ESPEngine screen, keep = !(K bound / max(dmax - reff, 0.05) < tol)
+ *         with dmax the largest distance to any point (fastesp/esp.py:265-270).
  * `ptr_kind` applies to dens (n_func x n_func, row-major) and pts (n_points x 3);
  * the basis arrays are host arrays.  Shells up to f (L = l_A + l_B <= 6). */
 int fe_table_build(fe_ctx* ctx, int n_func, int n_prim, const int32_t* fn_atom,
@@ -131,7 +133,9 @@ int fe_table_free(fe_table* t);
 
 /* Table statistics: info[0] n_pairs, info[1] n_terms, info[2] n_atoms,
  * info[3..3+FE_LMAX] pairs per class L, info[10] device bytes,
- * info[11] candidate pairs before the point screen (0 if unknown). */
+ * info[11] candidate pairs before the point screen: primitive pairs with
+ * K >= 1e-30 in shell pairs whose density block reaches 1e-15, the
+ * reference's primitive-pair list (0 for tables uploaded from pairs.bin). */
 int fe_table_info(const fe_table* t, int64_t* info, int n_info);
 
 /* ESP at n points: V = [nuclear] - [electron] as selected by `parts`.

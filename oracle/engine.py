@@ -189,14 +189,38 @@ def min_distance(P: np.ndarray, pts: np.ndarray, chunk: int = 4096) -> np.ndarra
     return out
 
 
+def max_distance(P: np.ndarray, pts: np.ndarray, chunk: int = 4096) -> np.ndarray:
+    """dmax[i] = max_k |pts_k - P_i| (the ``d.max()`` of ``fastesp/esp.py:266-267``)."""
+    out = np.empty(len(P))
+    pts = np.asarray(pts, float)
+    for s in range(0, len(P), chunk):
+        d2 = ((P[s:s + chunk, None, :] - pts[None, :, :]) ** 2).sum(-1)
+        out[s:s + chunk] = np.sqrt(d2.max(axis=1))
+    return out
+
+
 def build_table(wfn, basis, density, points, tol: float = 1e-12,
-                prims: PrimPairs | None = None) -> OracleTable:
-    """``fastesp/table.py:176-259``: screen, monomial schedule and flatten."""
+                prims: PrimPairs | None = None, rule: str = "numpy") -> OracleTable:
+    """``fastesp/table.py:176-259``: screen, monomial schedule and flatten.
+
+    ``rule="engine"`` applies instead the per-pair skip of
+    ``ESPEngine.esp_electron`` (``fastesp/esp.py:265-270``):
+    skip if K bound / max(dmax - reff, 0.05) < tol, with dmax the largest
+    distance from the pair centre to any point.  The evaluation of the kept
+    pairs is the same sum, so ``evaluate.electron`` of that table is
+    ``esp_electron``.
+    """
     if prims is None:
         prims = build_prim_pairs(basis, density)
     pts = np.asarray(points, float).reshape(-1, 3)
-    dmin = min_distance(prims.P, pts)
-    val = prims.K * prims.bound / np.maximum(dmin, 0.05)
+    if rule == "numpy":
+        dmin = min_distance(prims.P, pts)
+        val = prims.K * prims.bound / np.maximum(dmin, 0.05)
+    elif rule == "engine":
+        dmax = max_distance(prims.P, pts)
+        val = prims.K * prims.bound / np.maximum(dmax - prims.reff, 0.05)
+    else:
+        raise ValueError(f"rule {rule!r}")
     keep = ~(val < tol)
     idx = np.nonzero(keep)[0]
     L = prims.L[idx]

@@ -85,6 +85,20 @@ def esp(table, points=None) -> np.ndarray:
     return nuclear(table, points) - electron(table, points)
 
 
+# charges of ``ESPEngine.esp`` (``fastesp/esp.py:293``): only these six
+# elements; any other symbol raises KeyError there
+ENGINE_Z = {"H": 1.0, "C": 6.0, "N": 7.0, "O": 8.0, "S": 16.0, "P": 15.0}
+
+
+def engine_nuclear(symbols, coords, points) -> np.ndarray:
+    """sum_A Z_A/|C-R_A| with the charges of ``fastesp/esp.py:291-296``."""
+    pts = np.asarray(points, float).reshape(-1, 3)
+    nuc = np.zeros(len(pts))
+    for sym, xyz in zip(symbols, np.asarray(coords, float)):
+        nuc += ENGINE_Z[sym] / np.sqrt(((pts - xyz) ** 2).sum(axis=1))
+    return nuc
+
+
 # ---------------------------------------------------------------------------
 def _dmono(R, keys, axis):
     """d R^a / dR_axis for every key alpha: a_axis R^(a - e_axis)."""

@@ -1,0 +1,86 @@
+"""GPU parity of ``ESPEngine`` (``fastesp/esp.py:150-296``) against the oracle.
+
+The engine evaluates the same contracted pairs as the table path under its own
+skip rule, K bound / max(dmax - reff, 0.05) < tol (dmax: the largest distance
+from the pair centre to any point), and its own nuclear charges.
+"""
+import numpy as np
+import pytest
+
+from fastesp_amd import synth
+from fastesp_amd.esp import ESPEngine
+from oracle import engine, evaluate
+from tests.helpers import cf_toy, make_mol
+
+pytestmark = pytest.mark.gpu
+TOL_FP64 = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    w = synth.config0()
+    return w, engine.build_prim_pairs(w.basis, w.density)
+
+
+@pytest.mark.parametrize("tol", [1e-15, 1e-12, 1e-7])
+def test_engine_electron_matches_oracle(cfg0, tol):
+    w, prims = cfg0
+    say(f"ESPEngine cfg0 tol={tol}")
+    eng = ESPEngine(w.basis, w.density)
+    got = eng.esp_electron(w.points, tol=tol)
+    tab = engine.build_table(w.wfn, w.basis, w.density, w.points, tol, prims=prims, rule="engine")
+    assert np.abs(got - evaluate.electron(tab)).max() < TOL_FP64
+    t = eng._table(w.points, tol)[1]
+    info = t.info()
+    assert info["n_pairs"] == tab.n_pairs
+    assert info["n_candidates"] == len(prims.p)
+
+
+def test_engine_esp_and_report(cfg0, capsys):
+    w, prims = cfg0
+    say("ESPEngine.esp cfg0 verbose")
+    eng = ESPEngine(w.basis, w.density)
+    V = eng.esp(w.points, verbose=True)
+    tab = engine.build_table(w.wfn, w.basis, w.density, w.points, 1e-15, prims=prims,
+                             rule="engine")
+    want = evaluate.engine_nuclear(w.wfn.symbols, w.wfn.coords, w.points) - evaluate.electron(tab)
+    assert np.abs(V - want).max() < TOL_FP64
+    out = capsys.readouterr().out
+    assert f"    primitive pairs: used {tab.n_pairs}, skipped {len(prims.p) - tab.n_pairs}" in out
+
+
+def test_engine_prims_match_oracle():
+    """``ESPEngine.prims``: every candidate in the order of ``_build``."""
+    wfn, basis, P, _ = cf_toy()
+    say("ESPEngine.prims cf_toy")
+    prims = ESPEngine(basis, P).prims
+    ref = engine.build_prim_pairs(basis, P)
+    assert len(prims) == len(ref.p)
+    for i, pp in enumerate(prims):
+        assert pp.p == pytest.approx(ref.p[i], rel=1e-15)
+        np.testing.assert_allclose(pp.center, ref.P[i], rtol=1e-14, atol=1e-14)
+        assert pp.K == pytest.approx(ref.K[i], rel=1e-13)
+        assert pp.bound == pytest.approx(ref.bound[i], rel=1e-12)
+        assert pp.reff == pytest.approx(ref.reff[i], rel=1e-14)
+        keys = engine.pair_type(int(ref.L[i]))[0]
+        assert (pp.alpha == keys[:, :3]).all() and (pp.m == keys[:, 3]).all()
+        assert (pp.npow == pp.m).all()
+        scale = np.abs(ref.G[i]).max()
+        assert np.abs(pp.coef - ref.G[i]).max() <= 1e-13 * max(scale, 1e-300)
+
+
+def test_engine_charges_outside_its_table_raise():
+    """``fastesp/esp.py:293`` knows H, C, N, O, S, P only."""
+    wfn, basis, P, _ = make_mol([("F", (0.0, 0.0, 0.0)), ("H", (1.7, 0.0, 0.0))],
+                                [[(0, [4.0, 0.8], [0.5, 0.6]), (1, [1.0], [1.0])],
+                                 [(0, [1.0], [1.0])]], nelec=10)
+    say("ESPEngine F-H KeyError")
+    eng = ESPEngine(basis, P)
+    pts = np.array([[3.0, 1.0, 0.0]])
+    assert np.isfinite(eng.esp_electron(pts)).all()
+    with pytest.raises(KeyError):
+        eng.esp(pts)

@@ -131,6 +131,50 @@ def test_screening_does_not_change_the_answer():
         assert np.abs(got - full).max() < 1e-9
 
 
+def test_engine_screen_rule(cfg0):
+    """The skip of ``ESPEngine.esp_electron`` (``fastesp/esp.py:265-270``):
+    K bound / max(dmax - reff, 0.05) < tol, dmax the LARGEST distance to a point.
+
+    * tol = 0 keeps every candidate (the whole ``ESPEngine.prims`` list);
+    * adding a far point can only drop pairs under this rule (dmax grows) and
+      can only keep more under the table rule (dmin shrinks) -- a min/max
+      mix-up fails one of the two;
+    * at the engine's default tol (1e-15) the electron sum moves by < 1e-12,
+      at 1e-12 by < 1e-9 (the screening invariance of
+      ``tests/test_pipeline.py:167-178``; the max-distance rule screens more
+      loosely than the table's, so larger tolerances move it more).
+    """
+    w = cfg0
+    pts = w.points[:200]
+    prims = engine.build_prim_pairs(w.basis, w.density)
+    full = engine.build_table(w.wfn, w.basis, w.density, pts, 0.0, prims=prims, rule="engine")
+    assert full.n_pairs == len(prims.p)
+    far = np.vstack([pts, [[80.0, -60.0, 45.0]]])
+    for tol in (1e-9, 1e-7):
+        a = engine.build_table(w.wfn, w.basis, w.density, pts, tol, prims=prims, rule="engine")
+        b = engine.build_table(w.wfn, w.basis, w.density, far, tol, prims=prims, rule="engine")
+        c = engine.build_table(w.wfn, w.basis, w.density, pts, tol, prims=prims)
+        d = engine.build_table(w.wfn, w.basis, w.density, far, tol, prims=prims)
+        assert b.n_pairs < a.n_pairs and d.n_pairs >= c.n_pairs
+    ref = evaluate.electron(full)
+    for tol, bound in ((1e-15, 1e-12), (1e-12, 1e-9)):
+        t = engine.build_table(w.wfn, w.basis, w.density, pts, tol, prims=prims, rule="engine")
+        assert t.n_pairs < full.n_pairs
+        assert np.abs(evaluate.electron(t) - ref).max() < bound
+
+
+def test_engine_nuclear_charges():
+    """``fastesp/esp.py:291-296``: six elements only; others raise KeyError."""
+    pts = np.array([[1.0, 2.0, 3.0], [-2.0, 0.5, 0.0]])
+    xyz = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0]])
+    got = evaluate.engine_nuclear(["O", "H"], xyz, pts)
+    want = 8.0 / np.linalg.norm(pts - xyz[0], axis=1) + 1.0 / np.linalg.norm(pts - xyz[1], axis=1)
+    np.testing.assert_allclose(got, want, rtol=1e-15)
+    assert evaluate.ENGINE_Z["S"] == 16.0 and evaluate.ENGINE_Z["P"] == 15.0
+    with pytest.raises(KeyError):
+        evaluate.engine_nuclear(["F"], xyz[:1], pts)
+
+
 # ---------------------------------------------------------------------------
 # adjoint
 # ---------------------------------------------------------------------------
