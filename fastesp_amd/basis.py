@@ -10,8 +10,12 @@ fastesp ``Wavefunction``/``Basis`` pair can hand it over unchanged:
   sharing the shell's contraction coefficients (which already carry the radial
   prefactor ``N_s(alpha, l)``, ``fastesp/basis.py:53-66``).
 
-The spherical -> Cartesian transform of MOLDEN input is not needed here: the
-workloads are generated directly in the Cartesian basis.
+* ``pure_to_cartesian(l)`` / ``Basis.block_transform()`` /
+  ``Basis.density_matrix_cartesian()`` re-express a MOLDEN wavefunction stored
+  in pure (spherical) functions in the Cartesian span of each shell
+  (``fastesp/basis.py:1-13,105-180,240-270``): the density is
+  P_cart = T P_pure T^T with T built shell by shell from S-orthonormal real
+  solid harmonics.
 """
 from __future__ import annotations
 
@@ -89,11 +93,113 @@ class Wavefunction:
         return np.array([Z_OF[s] for s in self.symbols], np.float64)
 
 
+# ---------------------------------------------------------------------------
+# Pure (spherical) -> Cartesian
+# ---------------------------------------------------------------------------
+def _odd_dfact(n: int) -> int:
+    """n!! for odd n, (-1)!! = 1."""
+    r = 1
+    while n > 1:
+        r *= n
+        n -= 2
+    return r
+
+
+@lru_cache(maxsize=None)
+def cartesian_overlap_block(l: int) -> np.ndarray:
+    """Overlap of the Cartesian components of one l-shell with itself, one
+    primitive of exponent 1 and coefficient N_s(1, l) (``fastesp/basis.py:
+    94-113``).  It does not depend on the exponent: N_s^2 carries (2 alpha)^{l+3/2}.
+
+    1-D moments: int x^n exp(-2 x^2) dx = (n-1)!! / 4^{n/2} sqrt(pi/2), n even.
+    """
+    comps = cart_order(l)
+    pref = float(primitive_norm(1.0, l)) ** 2 * (np.pi / 2.0) ** 1.5
+
+    def mom(n: int) -> float:
+        return 0.0 if n % 2 else _odd_dfact(n - 1) / 4.0 ** (n // 2)
+
+    return np.array([[pref * mom(a[0] + b[0]) * mom(a[1] + b[1]) * mom(a[2] + b[2])
+                      for b in comps] for a in comps])
+
+
+def _poly_mul(p: dict, q: dict) -> dict:
+    out: dict = {}
+    for (a, b, c), u in p.items():
+        for (d, e, f), v in q.items():
+            k = (a + d, b + e, c + f)
+            out[k] = out.get(k, 0) + u * v
+    return {k: v for k, v in out.items() if v != 0}
+
+
+@lru_cache(maxsize=None)
+def solid_harmonics(l: int) -> tuple:
+    """Real regular solid harmonics of degree l as integer polynomials
+    {(ax, ay, az): coefficient}, in MOLDEN order m = 0, +1, -1, +2, -2, ...
+
+    S_l^{+-m} = Pi_l^m(z, r^2) * {Re, Im} (x + i y)^m with
+    Pi_l^m = sum_t (-1)^t C(l,t) C(2l-2t,l) (l-2t)!/(l-2t-m)! r^{2t} z^{l-2t-m}
+    (Helgaker, Jorgensen & Olsen, Molecular Electronic-Structure Theory, sec.
+    6.4.2; no Condon-Shortley phase; the common factor of each row is left out).
+
+    Phase convention: the rows m = +3 and m = -3 of l = 3 are negated.  The
+    reference's table (``fastesp/basis.py:119-148``) carries these two signs,
+    which it read back from an ORCA MOLDEN file; a wavefunction's density
+    depends on the phases of the functions its coefficients refer to.
+    """
+    from math import comb, factorial
+
+    rows = []
+    for m in [0] + [s * k for k in range(1, l + 1) for s in (1, -1)]:
+        a = abs(m)
+        pi: dict = {}
+        for t in range((l - a) // 2 + 1):
+            c = (-1) ** t * comb(l, t) * comb(2 * l - 2 * t, l) * factorial(l - 2 * t) \
+                // factorial(l - 2 * t - a)
+            r2t = {(0, 0, 0): 1}
+            for _ in range(t):
+                r2t = _poly_mul(r2t, {(2, 0, 0): 1, (0, 2, 0): 1, (0, 0, 2): 1})
+            term = _poly_mul(r2t, {(0, 0, l - 2 * t - a): c})
+            for k, v in term.items():
+                pi[k] = pi.get(k, 0) + v
+        if m >= 0:   # Re (x + i y)^a
+            ang = {(a - 2 * k, 2 * k, 0): (-1) ** k * comb(a, 2 * k) for k in range(a // 2 + 1)}
+        else:        # Im (x + i y)^a
+            ang = {(a - 2 * k - 1, 2 * k + 1, 0): (-1) ** k * comb(a, 2 * k + 1)
+                   for k in range((a - 1) // 2 + 1)}
+        poly = _poly_mul({k: v for k, v in pi.items() if v != 0}, ang)
+        if l == 3 and a == 3:
+            poly = {k: -v for k, v in poly.items()}
+        rows.append(poly)
+    return tuple(rows)
+
+
+@lru_cache(maxsize=None)
+def pure_to_cartesian(l: int) -> np.ndarray:
+    """T (2l+1, ncart): rows are the solid harmonics of ``solid_harmonics(l)`` in
+    the N_s-normalised Cartesian components, scaled so that T S T^T = 1 with
+    S = ``cartesian_overlap_block(l)`` (``fastesp/basis.py:151-169``).  The
+    identity for l <= 1."""
+    comps = cart_order(l)
+    if l <= 1:
+        return np.eye(len(comps))
+    col = {c: i for i, c in enumerate(comps)}
+    v = np.zeros((2 * l + 1, len(comps)))
+    for r, poly in enumerate(solid_harmonics(l)):
+        for k, c in poly.items():
+            v[r, col[k]] = c
+    S = cartesian_overlap_block(l)
+    return v / np.sqrt(np.einsum("ri,ij,rj->r", v, S, v))[:, None]
+
+
 class Basis:
     """Cartesian basis, one entry per component (``fastesp/basis.py:207-238``)."""
 
-    def __init__(self, wfn: Wavefunction):
+    def __init__(self, wfn: Wavefunction, use_primitive_norm: bool = True):
         self.wfn = wfn
+        # the MOLDEN coefficients already carry N_s(alpha, l); the flag exists
+        # for call compatibility (the reference stores it and does not use it)
+        self.use_primitive_norm = use_primitive_norm
         centers, atoms, ang, exps, coefs, shell_of = [], [], [], [], [], []
         self.shells = []
         for sh in wfn.shells:
@@ -128,6 +234,34 @@ class Basis:
         for i, sh in enumerate(self.shells):
             out[i + 1] = out[i] + len(cart_order(sh.l))
         return out
+
+    def block_transform(self) -> np.ndarray:
+        """(n_cart, n_pure) map from stored pure coefficients to Cartesian
+        components, block-diagonal over the shells (``fastesp/basis.py:240-262``).
+        Every stored shell is taken as 2l+1 pure functions, as the reference does."""
+        npure = [2 * sh.l + 1 for sh in self.wfn.shells]
+        T = np.zeros((self.n, int(sum(npure))))
+        oc = op = 0
+        for sh, n_p in zip(self.wfn.shells, npure):
+            nc = len(cart_order(sh.l))
+            T[oc:oc + nc, op:op + n_p] = pure_to_cartesian(sh.l).T
+            oc += nc
+            op += n_p
+        return T
+
+    def density_matrix_cartesian(self, min_occ: float = 1e-10) -> np.ndarray:
+        """P = T (sum_i occ_i c_i c_i^T) T^T over the MOs with occ > min_occ
+        (``fastesp/basis.py:264-270``)."""
+        w = self.wfn
+        T = self.block_transform()
+        if w.mo_coeff.shape[0] != T.shape[1]:
+            raise ValueError(
+                f"MO coefficients have {w.mo_coeff.shape[0]} rows, the shells {T.shape[1]} pure "
+                "functions: Cartesian d/f input (no [5D]/[7F]) is not supported")
+        occ = np.asarray(w.mo_occ) > min_occ
+        c = w.mo_coeff[:, occ] * np.sqrt(np.asarray(w.mo_occ)[occ])[None, :]
+        Tc = T @ c
+        return Tc @ Tc.T
 
 
 # ---------------------------------------------------------------------------
@@ -215,3 +349,10 @@ def overlap_matrix(basis: Basis) -> np.ndarray:
             cols = off[b_of][:, None, None] + np.arange(len(cb_))[None, None, :]
             S[rows, cols] = blocks
     return S
+
+
+def build_basis(path, use_primitive_norm: bool = True):
+    """read_molden + Basis (``fastesp/basis.py:300-303``)."""
+    from .molden import read_molden
+    wfn = read_molden(path)
+    return wfn, Basis(wfn, use_primitive_norm=use_primitive_norm)

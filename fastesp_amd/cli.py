@@ -9,6 +9,7 @@ esp_gpu.cu:232-242``).  The same arguments, files and console protocol here:
     python -m fastesp_amd build-pairs basis.bin pairs.bin [tol]
     python -m fastesp_amd esp-gpu pairs.bin out.txt [--repeats N] [--quiet] [--precision mixed]
     python -m fastesp_amd esp basis.bin out.txt [--tol T] [--screen numpy|cpp] [--repeats N]
+    python -m fastesp_amd esp WF.molden --points pts.txt [-o esp.txt] [--tol T] [--angstrom] [-q]
 
 * build-pairs builds the table on the GPU with the C++ tool's screen
   (``build_pairs.cpp:318-328``) and writes pairs.bin in reference order;
@@ -21,6 +22,11 @@ esp_gpu.cu:232-242``).  The same arguments, files and console protocol here:
 * esp does both in one process (the table never leaves the device) from a
   basis.bin payload (``fastesp/table.py:268-316``), with the NumPy screen by
   default as ``build_table`` uses.
+* esp with ``--points`` is the reference's own driver, ``fastesp esp
+  WF.molden --points pts.txt -o esp.txt`` (``run_esp``, ``fastesp/cli.py:
+  118-200,286-322``): MOLDEN reading, the pure -> Cartesian density, the
+  point file (Bohr, or Angstrom with ``--angstrom``), the table build and the
+  evaluation, the same report lines and the same output file format.
 
 Errors print a message on stderr and return 1, as the native tools do.
 """
@@ -125,6 +131,78 @@ def cmd_esp(a) -> int:
     return 0
 
 
+def read_points(path, geom: str = "bohr") -> np.ndarray:
+    """Rows of the first three numbers of every parsable line, Angstrom
+    converted to Bohr (``fastesp/cli.py:211-227``)."""
+    rows = []
+    for line in Path(path).read_text().splitlines():
+        f = line.split()
+        if len(f) >= 3:
+            try:
+                rows.append([float(x) for x in f[:3]])
+            except ValueError:
+                continue
+    if not rows:
+        raise ValueError(f"cannot parse points from {path}")
+    pts = np.asarray(rows, float)
+    if geom == "angstrom":
+        from .basis import BOHR
+        pts = pts / BOHR
+    return pts
+
+
+def run_esp(molden, points_file, out_file=None, backend: str = "auto", tol: float = 1e-12,
+            geom: str = "bohr", verbose: bool = True, precision: str = "fp64") -> np.ndarray:
+    """``fastesp.cli.run_esp`` (``fastesp/cli.py:118-200``) on the HIP backend."""
+    if backend not in ("auto", "hip"):
+        raise ValueError(f"backend {backend!r}: this package provides the 'hip' backend only")
+    from .basis import Basis
+    from .builder import SCREEN_NUMPY, build_table_payload
+    from .formats import basis_payload
+    from .molden import read_molden
+    t0 = time.perf_counter()
+    wfn = read_molden(molden)
+    basis = Basis(wfn)
+    density = basis.density_matrix_cartesian()
+    t_load = time.perf_counter() - t0
+    pts = read_points(points_file, geom)
+    t0 = time.perf_counter()
+    table = build_table_payload(basis_payload(wfn, basis, density, pts), tol, SCREEN_NUMPY)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    V = table.esp(pts, precision=_precision(precision))
+    t_solve = time.perf_counter() - t0
+    if verbose:
+        print("  backend      : hip")
+        print(f"  basis        : {basis.n} Cartesian functions, {basis.n_prim} primitives")
+        print(f"  load         : {t_load:8.3f} s")
+        print(f"  pair table   : {t_build:8.4f} s  (point-independent)")
+        print(f"  ESP solve    : {t_solve:8.4f} s")
+        tot = t_build + t_solve
+        print(f"  total (excl. load): {tot:.4f} s for {len(pts)} points "
+              f"({1e3 * tot / len(pts):.4f} ms/point)")
+        print(f"  ESP range    : {V.min() * 27.211386245988:+.4f} .. "
+              f"{V.max() * 27.211386245988:+.4f} V")
+    if out_file:
+        _write_out(out_file, pts, V)
+        if verbose:
+            print(f"  wrote        : {out_file}")
+    return V
+
+
+def cmd_esp_any(a) -> int:
+    if a.points is not None:
+        out = a.out_opt or a.out
+        run_esp(a.input, a.points, out, backend=a.backend, tol=a.tol,
+                geom="angstrom" if a.angstrom else "bohr", verbose=not a.quiet,
+                precision=a.precision)
+        return 0
+    if a.out is None:
+        raise ValueError("esp: basis.bin input needs an output file (or give --points for MOLDEN)")
+    a.basis = a.input
+    return cmd_esp(a)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m fastesp_amd", description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -142,18 +220,24 @@ def main(argv=None) -> int:
     e.add_argument("--quiet", action="store_true")
     e.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     e.set_defaults(fn=cmd_esp_gpu)
-    s = sub.add_parser("esp", help="basis.bin -> out.txt in one process")
-    s.add_argument("basis")
-    s.add_argument("out")
-    s.add_argument("--tol", type=float, default=1e-12)
+    s = sub.add_parser("esp", help="basis.bin -> out.txt, or WF.molden --points pts.txt")
+    s.add_argument("input", help="basis.bin payload, or a MOLDEN file when --points is given")
+    s.add_argument("out", nargs="?", default=None)
+    s.add_argument("--points", default=None,
+                   help="text file, one 'x y z' per line (Bohr unless --angstrom): MOLDEN mode")
+    s.add_argument("-o", "--out", dest="out_opt", default=None)
+    s.add_argument("--backend", choices=["auto", "hip"], default="auto")
+    s.add_argument("--angstrom", action="store_true", help="input points in Angstrom")
+    s.add_argument("--tol", type=float, default=1e-12,
+                   help="primitive pair screening threshold in Hartree/e")
     s.add_argument("--screen", choices=["numpy", "cpp"], default="numpy")
     s.add_argument("--repeats", type=int, default=1)
-    s.add_argument("--quiet", action="store_true")
+    s.add_argument("-q", "--quiet", action="store_true")
     s.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    s.set_defaults(fn=cmd_esp)
+    s.set_defaults(fn=cmd_esp_any)
     a = ap.parse_args(argv)
     try:
         return a.fn(a)
-    except (OSError, ValueError, _lib.HipBackendError) as ex:
+    except (OSError, ValueError, IndexError, KeyError, _lib.HipBackendError) as ex:
         print(f"fastesp_amd {a.cmd}: {ex}", file=sys.stderr)
         return 1

@@ -157,6 +157,29 @@ def random_density(basis: Basis, nelec: int, rng, S: np.ndarray | None = None):
     return density_from_orbitals(C), S
 
 
+def pure_orbitals(wfn: Wavefunction, nelec: int, rng, S: np.ndarray | None = None):
+    """MO data of a closed-shell wavefunction stored in pure functions, as a
+    MOLDEN file holds it: C_pure ((sum 2l+1) x n_occ) Gaussian random with
+    C^T (T^T S T) C = 1, occupations 2.  Sets ``wfn.mo_coeff``, ``mo_occ`` and
+    ``mo_energy`` and returns the Cartesian density T C C^T T^T * 2 (the
+    construction of ``tests/test_pipeline.py:57-78``)."""
+    if nelec % 2:
+        raise ValueError("odd electron count")
+    basis = Basis(wfn)
+    if S is None:
+        S = overlap_matrix(basis)
+    T = basis.block_transform()
+    Sp = T.T @ S @ T
+    nocc = nelec // 2
+    C = rng.standard_normal((T.shape[1], nocc))
+    w, v = np.linalg.eigh(C.T @ Sp @ C)
+    C = C @ (v @ np.diag(w ** -0.5) @ v.T)
+    wfn.mo_coeff = C
+    wfn.mo_occ = np.full(nocc, 2.0)
+    wfn.mo_energy = -np.linspace(1.0, 0.1, nocc)
+    return basis.density_matrix_cartesian()
+
+
 # ---------------------------------------------------------------------------
 def _fibonacci_sphere(n: int) -> np.ndarray:
     i = np.arange(n) + 0.5

@@ -92,3 +92,33 @@ def test_esp_gpu_reads_reference_built_pairs(cfg0_basis, tmp_path):
     assert r.stdout == ""
     assert np.abs(read_out(out)[:, 3] - g["V_tol0"]).max() < 1e-10
 
+
+
+def test_molden_driver_matches_oracle(tmp_path):
+    """``fastesp esp WF.molden --points pts.txt -o esp.txt`` (``fastesp/cli.py:
+    118-200``): the reference's Cartesian density of the MOLDEN fixture and
+    the oracle's FP64 evaluation, points in Bohr and in Angstrom."""
+    from fastesp_amd.basis import BOHR
+    from fastesp_amd.molden import read_molden
+    from oracle import engine, evaluate
+    ref = np.load(GOLD / "molden_ref.npz")
+    w = read_molden(GOLD / "toy.molden")
+    from fastesp_amd.basis import Basis
+    b = Basis(w)
+    pts = np.random.default_rng(11).normal(0.0, 3.0, (200, 3)) + [0.4, 0.6, 0.0]
+    want = evaluate.esp(engine.build_table(w, b, ref["P"], pts, tol=1e-12))
+    for geom, scale in (("bohr", 1.0), ("angstrom", BOHR)):
+        pf = tmp_path / f"pts_{geom}.txt"
+        np.savetxt(pf, pts * scale, fmt="%.17g")
+        out = tmp_path / f"esp_{geom}.txt"
+        args = ["esp", GOLD / "toy.molden", "--points", pf, "-o", out]
+        if geom == "angstrom":
+            args.append("--angstrom")
+        r = run(*args)
+        assert r.returncode == 0, r.stderr
+        for key in ("backend      : hip", "basis        : 44 Cartesian functions",
+                    "pair table   :", "ESP solve    :", "ESP range    :", "wrote        :"):
+            assert key in r.stdout
+        rows = read_out(out)
+        np.testing.assert_allclose(rows[:, :3], pts, rtol=0, atol=1e-11)
+        assert np.abs(rows[:, 3] - want).max() < 1e-10
