@@ -28,15 +28,31 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+STAMP = LIB.with_name(LIB.name + ".stamp")
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-munsafe-fp-atomics", "-Wno-unused-command-line-argument"]
+
+
+def source_hash(arch: str = "gfx950", lmax: int = 6) -> str:
+    """Key of a build: every source and header under csrc/, the flags, the
+    target and LMAX (the cache key of ``fastesp/cli.py:51-67``: sources + arch)."""
+    import hashlib
+    h = hashlib.sha256()
+    for p in sorted(CSRC.iterdir()):
+        if p.suffix in (".hip", ".h", ".py") and p.is_file():
+            h.update(p.name.encode())
+            h.update(p.read_bytes())
+    h.update(" ".join(FLAGS + [arch, str(lmax)]).encode())
+    return h.hexdigest()[:16]
+
+
 def build(arch: str = "gfx950", lmax: int = 6, verbose: bool = True) -> Path:
     subprocess.run([sys.executable, str(CSRC / "gen_lclass.py"), str(lmax)], check=True,
                    capture_output=not verbose)
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={arch}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-munsafe-fp-atomics", "-Wno-unused-command-line-argument",
-           "-o", str(tmp), *srcs]
+    cmd = [hipcc(), f"--offload-arch={arch}", *FLAGS, "-o", str(tmp), *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -46,7 +62,20 @@ def build(arch: str = "gfx950", lmax: int = 6, verbose: bool = True) -> Path:
     if verbose and err.strip():
         print(err[-3000:])
     os.replace(tmp, LIB)
+    STAMP.write_text(source_hash(arch, lmax) + "\n")
     return LIB
+
+
+def build_native(force: bool = False, arch: str | None = None, quiet: bool = False) -> Path:
+    """``fastesp build-native`` (``fastesp/cli.py:81-115``): compile only when the
+    library is missing or was built from other sources, flags or target."""
+    arch = arch or os.environ.get("FASTESP_HIP_ARCH", "gfx950")
+    key = source_hash(arch)
+    if not force and LIB.exists() and STAMP.exists() and STAMP.read_text().strip() == key:
+        if not quiet:
+            print(f"  up to date ({key}, {arch})")
+        return LIB
+    return build(arch, verbose=not quiet)
 
 
 def main() -> int:

@@ -10,6 +10,8 @@ esp_gpu.cu:232-242``).  The same arguments, files and console protocol here:
     python -m fastesp_amd esp-gpu pairs.bin out.txt [--repeats N] [--quiet] [--precision mixed]
     python -m fastesp_amd esp basis.bin out.txt [--tol T] [--screen numpy|cpp] [--repeats N]
     python -m fastesp_amd esp WF.molden --points pts.txt [-o esp.txt] [--tol T] [--angstrom] [-q]
+    python -m fastesp_amd build-native [--arch gfx950] [--force]
+    python -m fastesp_amd selftest [-q]
 
 * build-pairs builds the table on the GPU with the C++ tool's screen
   (``build_pairs.cpp:318-328``) and writes pairs.bin in reference order;
@@ -203,6 +205,31 @@ def cmd_esp_any(a) -> int:
     return cmd_esp(a)
 
 
+def has_gpu() -> bool:
+    """A HIP device this library can open (the reference asks ``nvidia-smi -L``,
+    ``fastesp/cli.py:203-208``): ``/dev/kfd`` present and ``fe_init`` succeeds."""
+    if not Path("/dev/kfd").exists():
+        return False
+    try:
+        _lib.Context.get()
+        return True
+    except _lib.HipBackendError:
+        return False
+
+
+def cmd_build_native(a) -> int:
+    from .build import build_native
+    path = build_native(force=a.force, arch=a.arch)
+    print(f"  libfastesp_hip: {path}")
+    print(f"  HIP device     : {'yes' if has_gpu() else 'no'}")
+    return 0
+
+
+def cmd_selftest(a) -> int:
+    from .selftest import run
+    return run(verbose=not a.quiet)
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m fastesp_amd", description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -235,6 +262,14 @@ def main(argv=None) -> int:
     s.add_argument("-q", "--quiet", action="store_true")
     s.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     s.set_defaults(fn=cmd_esp_any)
+    n = sub.add_parser("build-native", help="compile libfastesp_hip.so if its sources changed")
+    n.add_argument("--arch", default=None, help="offload target, default gfx950")
+    n.add_argument("--force", action="store_true")
+    n.set_defaults(fn=cmd_build_native)
+    t = sub.add_parser("selftest", help="check the device integrals against closed forms "
+                                        "and quadrature")
+    t.add_argument("-q", "--quiet", action="store_true")
+    t.set_defaults(fn=cmd_selftest)
     a = ap.parse_args(argv)
     try:
         return a.fn(a)

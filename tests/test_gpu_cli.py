@@ -122,3 +122,16 @@ def test_molden_driver_matches_oracle(tmp_path):
         rows = read_out(out)
         np.testing.assert_allclose(rows[:, :3], pts, rtol=0, atol=1e-11)
         assert np.abs(rows[:, 3] - want).max() < 1e-10
+
+
+def test_selftest_passes():
+    """``fastesp selftest`` (``fastesp/cli.py:230-282``) with the device integrals."""
+    r = run("selftest")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "SELFTEST PASS" in r.stdout
+
+
+def test_build_native_reports_device():
+    r = run("build-native")
+    assert r.returncode == 0, r.stderr
+    assert "HIP device     : yes" in r.stdout
