@@ -17,7 +17,10 @@ Printed JSON (one line): the driver's contract fields plus
   cpu_baseline -- the C++/OpenMP port of the reference path (table build +
                   evaluation) timed on this host's cores on a bounded point
                   sample, scaled to ms per 1e4 points;
-  accuracy     -- max |dV| against the FP64 oracle at 256 sampled points.
+  accuracy     -- max |dV| against the FP64 oracle at 256 sampled points;
+  batched      -- the 256-conformer graph-captured step;
+  density      -- the electron density at the same points on the FP64 matrix
+                  cores, its roofline and its NumPy baseline.
 """
 from __future__ import annotations
 
@@ -63,6 +66,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-precisions", default="mixed,fp64")
+    ap.add_argument("--density-steps", type=int, default=10,
+                    help="timed evaluations of the electron-density leg (0 = skip it)")
     return ap.parse_args()
 
 
@@ -71,7 +76,8 @@ def log(msg):
 
 
 # phase ids of fe_profile
-PHASE_NAMES = {0: "sort", 1: "reduce", 2: "pack_gV", 3: "dG_to_dP"}
+PHASE_NAMES = {0: "sort", 1: "reduce", 2: "pack_gV", 3: "dG_to_dP", 4: "density_phi",
+               5: "density_mm", 6: "density_reduce"}
 PHASE_NAMES.update({8 + L: f"eval_L{L}" for L in range(7)})
 PHASE_NAMES.update({16 + L: f"dG_L{L}" for L in range(7)})
 PHASE_NAMES.update({24 + L: f"dG_reduce_L{L}" for L in range(7)})
@@ -255,6 +261,59 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
     return out
 
 
+def density_leg(a, w, torch, ctx, stream, with_cpu: bool) -> dict:
+    """Electron density rho(C) at the config-1 points (fe_density_*): basis
+    values, then the quadratic form on the FP64 matrix cores.  The roofline of
+    the matrix kernel is the FP64 matrix peak; the CPU baseline is the NumPy
+    restatement of the reference's density (one process) on a point sample,
+    which also checks the device values at those points."""
+    from fastesp_amd.density import ElectronDensity
+    n = len(w.points)
+    ed = ElectronDensity(w.wfn, w.basis, w.density, ctx)
+    pts = torch.from_numpy(np.ascontiguousarray(w.points)).to("cuda:0")
+    rho = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    sh = stream.cuda_stream
+    for _ in range(2):
+        ed.rho_device(pts.data_ptr(), n, rho.data_ptr(), sh)
+    torch.cuda.synchronize()
+    ctx.profile_start()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.density_steps):
+        ed.rho_device(pts.data_ptr(), n, rho.data_ptr(), sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ph = ctx.profile_stop()
+    ms = e0.elapsed_time(e1) / a.density_steps
+    mm_ms = ph[5][0] / ph[5][1]
+    info = ed.info()
+    achieved = info["mm_flops"] / (mm_ms * 1e-3)
+    out = dict(ms_per_eval=round(ms, 4), ms_per_1e4_pts=round(ms / (n / 1e4), 4),
+               n_points=n, n_functions=int(w.basis.n),
+               phases_ms={PHASE_NAMES[k]: round(v[0] / v[1], 4) for k, v in sorted(ph.items())},
+               roofline=dict(kernel="k_density_mm", bound="mfma", unit="TFLOP/s",
+                             achieved=round(achieved / 1e12, 3), peak=78.6,
+                             frac=round(achieved / 78.6e12, 4),
+                             flops_per_launch=int(info["mm_flops"]), avg_launch_ms=round(mm_ms, 4),
+                             flop_model="N_pad x 2 x 64^2 x nJ (nJ + 1) / 2 (block-upper P)"))
+    if with_cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import density as odens
+        sel = np.random.default_rng(77).choice(n, 1000, replace=False)
+        t0 = time.perf_counter()
+        want = odens.rho(w.basis, w.density, w.points[sel])
+        dt = time.perf_counter() - t0
+        got = rho.cpu().numpy()[sel]
+        out["cpu_baseline"] = dict(
+            value=round(dt * 1e3 / len(sel) * 1e4, 2), unit="ms/1e4 pts", cores=1, kind="port",
+            sample=f"NumPy restatement of our_density (benchmarks/check_density_mwfn.py:60-75) "
+                   f"on {len(sel)} sampled points, scaled linearly to 1e4")
+        out["max_rel_err_vs_oracle"] = float(np.abs(got - want).max() / np.abs(want).max())
+    ed.close()
+    return out
+
+
 def main() -> int:
     a = parse()
     if a.gpus != 1:
@@ -430,6 +489,10 @@ def main() -> int:
         leg.pop("_phases", None)
 
     batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None
+    dens = None
+    if a.density_steps:
+        log("electron density leg")
+        dens = density_leg(a, w, torch, ctx, stream, with_cpu=not a.no_cpu)
 
     cpu = None
     if not a.no_cpu:
@@ -462,6 +525,7 @@ def main() -> int:
         "roofline": roof,
         "roofline_step": roof_step,
         "batched": batched,
+        "density": dens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -74,6 +74,12 @@ SIGNATURES = {
     "fe_batch_read": [_vp, C.c_int, C.c_int, _vp, _vp],
     "fe_batch_info": [_vp, C.POINTER(C.c_int64), C.c_int],
     "fe_profile": [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.c_int],
+    "fe_density_create": [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _vp,
+                          C.c_int, _vp, C.POINTER(_vp)],
+    "fe_density_eval": [_vp, _vp, C.c_int64, C.c_int, _vp, _vp],
+    "fe_density_info": [_vp, C.POINTER(C.c_int64), C.c_int],
+    "fe_density_free": [_vp],
+    "fe_vdw_candidates": [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _dp, _ip],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 

@@ -59,6 +59,9 @@ enum {
   FE_PH_REDUCE = 1,
   FE_PH_PACK = 2,
   FE_PH_DP = 3,
+  FE_PH_DENS_PHI = 4,  // basis values at the points (density)
+  FE_PH_DENS_MM = 5,   // density quadratic form on the matrix cores
+  FE_PH_DENS_RED = 6,  // density block reduction
   FE_PH_EVAL0 = 8,    // + L: evaluation launch of class L
   FE_PH_DG0 = 16,     // + L: dL/dG launch of class L
   FE_PH_DGRED0 = 24,  // + L: dL/dG chunk reduction of class L

@@ -53,6 +53,7 @@ extern "C" {
 
 typedef struct fe_ctx fe_ctx;
 typedef struct fe_table fe_table;
+typedef struct fe_density fe_density;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int fe_version(void);
@@ -238,6 +239,32 @@ int fe_batch_read(fe_batch* b, int what, int ptr_kind, void* stream, double* dst
 /* [n_conf, n_points, n_pairs, forward items, dG items, partial bytes,
  *  dL/dP doubles, shell pairs, captured, items/WG, dG items/WG, class mask] */
 int fe_batch_info(const fe_batch* b, int64_t* out, int n_out);
+
+/* ---- electron density and surface points ------------------------------
+ * rho(C) = sum_{mu nu} phi_mu(C) P_{mu nu} phi_nu(C): replaces `our_density`
+ * of benchmarks/check_density_mwfn.py:60-75 (the per-point density that shares
+ * the ESP's basis).  fe_density_create takes the basis.bin fields of
+ * fastesp/table.py:268-293 (host arrays) and the density (n_func x n_func,
+ * row-major; host or device per ptr_kind; only its symmetric part enters);
+ * fe_density_eval writes rho at n_points points (host or device buffers).
+ * fe_density_info: [n_func, padded n_func, matrix-core FLOPs of the last
+ * evaluation, chunk capacity]. */
+int fe_density_create(fe_ctx* ctx, int n_func, int n_prim, const int32_t* fn_nprim,
+                      const int32_t* fn_off, const int32_t* fn_lx, const int32_t* fn_ly,
+                      const int32_t* fn_lz, const double* fc, const double* pexp,
+                      const double* pcoef, const double* dens, int ptr_kind, void* stream,
+                      fe_density** out);
+int fe_density_eval(fe_density* d, const double* pts, int64_t n_points, int ptr_kind,
+                    void* stream, double* rho);
+int fe_density_info(const fe_density* d, int64_t* info, int n_info);
+int fe_density_free(fe_density* d);
+
+/* Candidate points of the vdW surface: counts[a] Fibonacci-sphere points of
+ * radius rad[a] about atom a (Bohr), atom-major, and ok[k] = 1 when point k
+ * lies outside every sphere (|p - R_b| >= rad_b - 1e-9).  The device part of
+ * examples/vdw_surface_esp.py:38-52 (vdw_surface_points); host arrays. */
+int fe_vdw_candidates(fe_ctx* ctx, int n_atoms, const double* xyz, const double* rad,
+                      const int64_t* counts, double* pts, int32_t* ok);
 
 #ifdef __cplusplus
 }
