@@ -78,6 +78,7 @@ SIGNATURES = {
                           C.c_int, _vp, C.POINTER(_vp)],
     "fe_density_eval": [_vp, _vp, C.c_int64, C.c_int, _vp, _vp],
     "fe_density_info": [_vp, C.POINTER(C.c_int64), C.c_int],
+    "fe_density_set_tuning": [_vp, C.c_int, C.c_int64],
     "fe_density_free": [_vp],
     "fe_vdw_candidates": [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _dp, _ip],
 }

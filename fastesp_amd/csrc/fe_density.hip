@@ -17,14 +17,14 @@
 //  * k_density_reduce: rho[i] = sum_J partial[J][i] in block order
 //    (bitwise reproducible).
 // The roofline is the FP64 matrix peak: a 64 x 64 output tile reads
-// 2 x 64 x 16 doubles per 2 x 64 x 64 x 16 FLOPs.
+// 2 x 64 x 32 doubles (mostly L2 hits) per 2 x 64 x 64 x 32 FLOPs.
 #include <algorithm>
 #include <cmath>
 
 #include "fe_internal.h"
 
 #define FE_DENS_TILE 64
-#define FE_DENS_BK 16
+#define FE_DENS_BK 32
 #define FE_DENS_CHUNK 32768
 
 typedef double fe_v4d __attribute__((ext_vector_type(4)));
@@ -43,6 +43,7 @@ struct fe_density {
   double* d_rho = nullptr;     // staged host output
   int64_t io_cap = 0;
   int64_t last_flops = 0;      // FLOPs of the matrix kernel in the last evaluation
+  bool tile128 = false;        // 128 x 64 tiles (k_density_mm128) instead of 64 x 64
 };
 
 __global__ __launch_bounds__(256) void k_density_pack(const double* __restrict__ P, const int n,
@@ -92,9 +93,12 @@ __global__ __launch_bounds__(256) void k_basis_values(const double* __restrict__
 // and columns 32 (w % 2) .. of the 64 x 64 tile, as 2 x 2 MFMA 16 x 16 blocks.
 // MFMA 16x16x4 f64 lane layout: A[i][k] and B[k][j] with i, j = lane % 16,
 // k = lane / 16; D[i][j] in register i / 4 of lane j + 16 (i % 4).
+// The K loop is software-pipelined: the next K slab is loaded into registers
+// while the matrix cores work on the slab in LDS.
 __global__ __launch_bounds__(256) void k_density_mm(const double* __restrict__ phiT, const int ld,
                                                     const double* __restrict__ Pu, const int npad,
                                                     double* __restrict__ partial) {
+  constexpr int NQ = FE_DENS_BK * FE_DENS_TILE / 256;  // doubles per thread per slab
   __shared__ double As[FE_DENS_BK][FE_DENS_TILE];
   __shared__ double Bs[FE_DENS_BK][FE_DENS_TILE];
   __shared__ double red[2][FE_DENS_TILE];
@@ -105,15 +109,30 @@ __global__ __launch_bounds__(256) void k_density_mm(const double* __restrict__ p
   const size_t col0 = (size_t)it * FE_DENS_TILE;
   fe_v4d acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
   const int kend = (J + 1) * FE_DENS_TILE;
+  // slab element q of this thread: row k = (tid >> 6) + 4 q, column c = lane
+  const double* pa = phiT + (size_t)(tid >> 6) * ld + col0 + lane;
+  const double* pb = Pu + (size_t)(tid >> 6) * npad + (size_t)J * FE_DENS_TILE + lane;
+  double ra[NQ], rb[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    ra[q] = pa[(size_t)4 * q * ld];
+    rb[q] = pb[(size_t)4 * q * npad];
+  }
   for (int k0 = 0; k0 < kend; k0 += FE_DENS_BK) {
 #pragma unroll
-    for (int q = 0; q < FE_DENS_BK * FE_DENS_TILE / 256; ++q) {
-      const int e = tid + 256 * q;
-      const int k = e >> 6, c = e & 63;
-      As[k][c] = phiT[(size_t)(k0 + k) * ld + col0 + c];
-      Bs[k][c] = Pu[(size_t)(k0 + k) * npad + (size_t)J * FE_DENS_TILE + c];
+    for (int q = 0; q < NQ; ++q) {
+      As[(tid >> 6) + 4 * q][lane] = ra[q];
+      Bs[(tid >> 6) + 4 * q][lane] = rb[q];
     }
     __syncthreads();
+    if (k0 + FE_DENS_BK < kend) {
+      const size_t ka = (size_t)(k0 + FE_DENS_BK);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        ra[q] = pa[(ka + 4 * q) * ld];
+        rb[q] = pb[(ka + 4 * q) * npad];
+      }
+    }
 #pragma unroll
     for (int kk = 0; kk < FE_DENS_BK; kk += 4) {
       const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
@@ -154,6 +173,92 @@ __global__ __launch_bounds__(256) void k_density_mm(const double* __restrict__ p
   }
   __syncthreads();
   if (tid < FE_DENS_TILE) partial[(size_t)J * ld + col0 + tid] = red[0][tid] + red[1][tid];
+}
+
+// 128 points x 64 functions per workgroup: wave w owns rows 32 w .. 32 w + 31
+// and all 64 columns (2 x 4 MFMA blocks), so each LDS fragment feeds more
+// matrix instructions.  Same slab pipeline and epilogue as k_density_mm.
+__global__ __launch_bounds__(256) void k_density_mm128(const double* __restrict__ phiT,
+                                                       const int ld,
+                                                       const double* __restrict__ Pu,
+                                                       const int npad,
+                                                       double* __restrict__ partial) {
+  constexpr int BM = 2 * FE_DENS_TILE, BK = 16;  // BK 16 keeps the slab registers small
+  constexpr int NQA = BK * BM / 256, NQB = BK * FE_DENS_TILE / 256;
+  __shared__ double As[BK][BM];
+  __shared__ double Bs[BK][FE_DENS_TILE];
+  const int it = blockIdx.x, J = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = 32 * w;
+  const int li = lane & 15, lk = lane >> 4;
+  const size_t col0 = (size_t)it * BM;
+  fe_v4d acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = fe_v4d{0, 0, 0, 0};
+  const int kend = (J + 1) * FE_DENS_TILE;
+  // A slab element q of this thread: row (tid >> 7) + 2 q, column tid & 127;
+  // B slab element q: row (tid >> 6) + 4 q, column lane
+  const double* pa = phiT + (size_t)(tid >> 7) * ld + col0 + (tid & 127);
+  const double* pb = Pu + (size_t)(tid >> 6) * npad + (size_t)J * FE_DENS_TILE + lane;
+  double ra[NQA], rb[NQB];
+#pragma unroll
+  for (int q = 0; q < NQA; ++q) ra[q] = pa[(size_t)2 * q * ld];
+#pragma unroll
+  for (int q = 0; q < NQB; ++q) rb[q] = pb[(size_t)4 * q * npad];
+  for (int k0 = 0; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < NQA; ++q) As[(tid >> 7) + 2 * q][tid & 127] = ra[q];
+#pragma unroll
+    for (int q = 0; q < NQB; ++q) Bs[(tid >> 6) + 4 * q][lane] = rb[q];
+    __syncthreads();
+    if (k0 + BK < kend) {
+      const size_t ka = (size_t)(k0 + BK);
+#pragma unroll
+      for (int q = 0; q < NQA; ++q) ra[q] = pa[(ka + 2 * q) * ld];
+#pragma unroll
+      for (int q = 0; q < NQB; ++q) rb[q] = pb[(ka + 4 * q) * npad];
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[2], bv[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[kk + lk][wr + 16 * a + li];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk + lk][16 * b + li];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // row wr + 16 a + 4 r + lk, columns 16 b + li of block J
+  double s[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const size_t row = col0 + wr + 16 * a + 4 * r + lk;
+      double v = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        v += acc[a][b][r] * phiT[((size_t)J * FE_DENS_TILE + 16 * b + li) * ld + row];
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      v += __shfl_xor(v, 4);
+      v += __shfl_xor(v, 8);
+      s[a][r] = v;
+    }
+  if (li == 0) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        partial[(size_t)J * ld + col0 + wr + 16 * a + 4 * r + lk] = s[a][r];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_density_reduce(const double* __restrict__ partial,
@@ -317,8 +422,8 @@ extern "C" int fe_density_eval(fe_density* d, const double* pts, int64_t n_point
   fe_ctx* c = d->ctx;
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
-  const int64_t chunk = std::min<int64_t>(
-      FE_DENS_CHUNK, (n_points + FE_DENS_TILE - 1) / FE_DENS_TILE * FE_DENS_TILE);
+  // points padded to 128 so both tile widths divide every chunk
+  const int64_t chunk = std::min<int64_t>(FE_DENS_CHUNK, (n_points + 127) / 128 * 128);
   const int nJ = d->npad / FE_DENS_TILE;
   if (chunk > d->chunk_cap) {
     dens_free_ws(d);
@@ -353,15 +458,19 @@ extern "C" int fe_density_eval(fe_density* d, const double* pts, int64_t n_point
   d->last_flops = 0;
   for (int64_t base = 0; base < n_points; base += chunk) {
     const int m = (int)std::min<int64_t>(chunk, n_points - base);
-    const int ld = (int)((m + FE_DENS_TILE - 1) / FE_DENS_TILE * FE_DENS_TILE);
+    const int ld = (m + 127) / 128 * 128;
     int ps = fe_prof_begin(c, FE_PH_DENS_PHI, st);
     hipLaunchKernelGGL(k_basis_values, dim3((unsigned)((ld + 255) / 256), (unsigned)d->npad),
                        dim3(256), 0, st, d_pts, base, n_points, ld, d->d_fn, d->d_fi, d->d_prim,
                        d->n, d->d_phiT);
     fe_prof_end(c, ps, st);
     ps = fe_prof_begin(c, FE_PH_DENS_MM, st);
-    hipLaunchKernelGGL(k_density_mm, dim3((unsigned)(ld / FE_DENS_TILE), (unsigned)nJ),
-                       dim3(256), 0, st, d->d_phiT, ld, d->d_Pu, d->npad, d->d_part);
+    if (d->tile128)
+      hipLaunchKernelGGL(k_density_mm128, dim3((unsigned)(ld / (2 * FE_DENS_TILE)), (unsigned)nJ),
+                         dim3(256), 0, st, d->d_phiT, ld, d->d_Pu, d->npad, d->d_part);
+    else
+      hipLaunchKernelGGL(k_density_mm, dim3((unsigned)(ld / FE_DENS_TILE), (unsigned)nJ),
+                         dim3(256), 0, st, d->d_phiT, ld, d->d_Pu, d->npad, d->d_part);
     fe_prof_end(c, ps, st);
     ps = fe_prof_begin(c, FE_PH_DENS_RED, st);
     hipLaunchKernelGGL(k_density_reduce, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
@@ -375,6 +484,13 @@ extern "C" int fe_density_eval(fe_density* d, const double* pts, int64_t n_point
                              hipMemcpyDeviceToHost, st));
     FE_HIPCHK(hipStreamSynchronize(st));
   }
+  return FE_OK;
+}
+
+extern "C" int fe_density_set_tuning(fe_density* d, int key, int64_t value) {
+  if (!d) return fe_set_error(FE_ERR_ARG, "fe_density_set_tuning: null argument");
+  if (key != 0) return fe_set_error(FE_ERR_ARG, "fe_density_set_tuning: unknown key %d", key);
+  d->tile128 = value != 0;
   return FE_OK;
 }
 

@@ -59,6 +59,11 @@ class ElectronDensity:
         _lib.check(self._ctx.lib.fe_density_eval(self._h, d_pts, n, _lib.FE_PTR_DEVICE, stream,
                                                  d_rho), "fe_density_eval")
 
+    def set_tile128(self, on: bool) -> None:
+        """128 x 64 or 64 x 64 (default) point x function tiles of the matrix kernel."""
+        _lib.check(self._ctx.lib.fe_density_set_tuning(self._h, 0, int(bool(on))),
+                   "fe_density_set_tuning")
+
     def info(self) -> dict:
         v = (C.c_int64 * 4)()
         _lib.check(self._ctx.lib.fe_density_info(self._h, v, 4), "fe_density_info")

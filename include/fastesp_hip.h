@@ -257,6 +257,8 @@ int fe_density_create(fe_ctx* ctx, int n_func, int n_prim, const int32_t* fn_npr
 int fe_density_eval(fe_density* d, const double* pts, int64_t n_points, int ptr_kind,
                     void* stream, double* rho);
 int fe_density_info(const fe_density* d, int64_t* info, int n_info);
+/* key 0: 1 = 128 x 64 point x function tiles, 0 = 64 x 64 (default) */
+int fe_density_set_tuning(fe_density* d, int key, int64_t value);
 int fe_density_free(fe_density* d);
 
 /* Candidate points of the vdW surface: counts[a] Fibonacci-sphere points of

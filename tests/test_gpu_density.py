@@ -42,8 +42,11 @@ def test_density_config0_blocks_and_chunks():
     say("density config0 40000 points")
     ed = ElectronDensity(wl.wfn, wl.basis, wl.density)
     got = ed.rho(pts)
-    _check(got, density.rho(wl.basis, wl.density, pts))
+    want = density.rho(wl.basis, wl.density, pts)
+    _check(got, want)
     assert ed.info()["n_pad"] % 64 == 0
+    ed.set_tile128(False)   # the 64 x 64 tile kernel
+    _check(ed.rho(pts), want)
     # only the symmetric part of P enters
     A = np.random.default_rng(6).normal(size=wl.density.shape) * 1e-3
     asym = ElectronDensity(wl.wfn, wl.basis, wl.density + A - A.T).rho(pts[:2000])
