@@ -488,6 +488,13 @@ def main() -> int:
     for leg in legs.values():
         leg.pop("_phases", None)
 
+    # sustained FMA rates of this device (independent chains): the FP32 the
+    # mixed contraction can use is non-packed, so its per-instruction rate is
+    # what bounds the mixed-precision gain
+    valu_rate = {"fp64_tflops": round(ctx.valu_rate(_lib.FE_PREC_FP64), 2),
+                 "fp32_nonpacked_tflops": round(ctx.valu_rate(_lib.FE_PREC_MIXED), 2)}
+    log(f"measured VALU FMA rate: {valu_rate}")
+
     batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None
     dens = None
     if a.density_steps:
@@ -524,6 +531,7 @@ def main() -> int:
         "table_build_ms": round(t_build * 1e3, 2),
         "roofline": roof,
         "roofline_step": roof_step,
+        "valu_rate_measured": valu_rate,
         "batched": batched,
         "density": dens,
         "cpu_baseline": cpu,

@@ -13,7 +13,9 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libfastesp_hip.so"
+# the in-tree build; FASTESP_HIP_LIB names another build of the same ABI
+LIB_PATH = Path(os.environ.get("FASTESP_HIP_LIB") or
+                Path(__file__).resolve().parent / "libfastesp_hip.so")
 
 FE_OK = 0
 FE_PTR_HOST = 0
@@ -81,6 +83,7 @@ SIGNATURES = {
     "fe_density_set_tuning": [_vp, C.c_int, C.c_int64],
     "fe_density_free": [_vp],
     "fe_vdw_candidates": [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _dp, _ip],
+    "fe_valu_rate": [_vp, C.c_int, _dp],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 
@@ -165,6 +168,13 @@ class Context:
         cnt = (C.c_int64 * 32)()
         check(self.lib.fe_profile(self.handle, 0, ms, cnt, 32), "fe_profile")
         return {i: (ms[i], int(cnt[i])) for i in range(32) if cnt[i]}
+
+    def valu_rate(self, precision: int) -> float:
+        """Measured FMA rate (TFLOP/s) of FP64 (FE_PREC_FP64) or non-packed FP32
+        (FE_PREC_MIXED) vector code on this device."""
+        v = C.c_double()
+        check(self.lib.fe_valu_rate(self.handle, precision, C.byref(v)), "fe_valu_rate")
+        return v.value
 
     def boys(self, T, M: int) -> np.ndarray:
         T = np.ascontiguousarray(T, np.float64).ravel()

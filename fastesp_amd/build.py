@@ -17,7 +17,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 SOURCES = ["fe_core.hip", "fe_eval.hip", "fe_mixed.hip", "fe_backward.hip", "fe_build.hip",
-           "fe_sort.hip", "fe_batch.hip", "fe_density.hip"]
+           "fe_sort.hip", "fe_batch.hip", "fe_density.hip", "fe_probe.hip"]
 LIB = PKG / "libfastesp_hip.so"
 
 
@@ -47,12 +47,17 @@ def source_hash(arch: str = "gfx950", lmax: int = 6) -> str:
     return h.hexdigest()[:16]
 
 
-def build(arch: str = "gfx950", lmax: int = 6, verbose: bool = True) -> Path:
+def build(arch: str = "gfx950", lmax: int = 6, verbose: bool = True, out: Path | None = None,
+          defines: tuple = ()) -> Path:
+    """Compile the library (to `out`, default the in-tree LIB; `defines` are
+    extra -D options, e.g. register budgets for a variant build)."""
     subprocess.run([sys.executable, str(CSRC / "gen_lclass.py"), str(lmax)], check=True,
                    capture_output=not verbose)
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={arch}", *FLAGS, "-o", str(tmp), *srcs]
+    dst = Path(out) if out is not None else LIB
+    tmp = dst.with_suffix(".so.tmp")
+    cmd = [hipcc(), f"--offload-arch={arch}", *FLAGS, *(f"-D{d}" for d in defines),
+           "-o", str(tmp), *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -61,9 +66,10 @@ def build(arch: str = "gfx950", lmax: int = 6, verbose: bool = True) -> Path:
         raise RuntimeError(f"hipcc failed:\n{err[-6000:]}")
     if verbose and err.strip():
         print(err[-3000:])
-    os.replace(tmp, LIB)
-    STAMP.write_text(source_hash(arch, lmax) + "\n")
-    return LIB
+    os.replace(tmp, dst)
+    if out is None and not defines:
+        STAMP.write_text(source_hash(arch, lmax) + "\n")
+    return dst
 
 
 def build_native(force: bool = False, arch: str | None = None, quiet: bool = False) -> Path:

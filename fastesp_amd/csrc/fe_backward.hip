@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_pack_pw(const double* __restrict__ pts,
 // one lane per pair of class L; points (x, y, z, gV) of chunk blockIdx.y are
 // wave-uniform; partial[(chunk * NT + k) * n_pairs + pair]
 template <int L>
-__global__ __launch_bounds__(256) void k_dG(const double* __restrict__ rec, const int n_pairs,
+__global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG(const double* __restrict__ rec, const int n_pairs,
                                             const double* __restrict__ pw, const int n_pts,
                                             const int ppc, const double* __restrict__ boys_tab,
                                             double* __restrict__ partial) {

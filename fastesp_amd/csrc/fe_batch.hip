@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_pack_pw_perm(const double* __restrict__
 
 // persistent dL/dG: items from a queue, one lane per pair
 template <int L>
-__global__ __launch_bounds__(256) void k_dG_batch(const FeDgItem* __restrict__ items,
+__global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG_batch(const FeDgItem* __restrict__ items,
                                                   const int n_items, int* __restrict__ counter,
                                                   const double* const* __restrict__ recs,
                                                   const double* __restrict__ pw,

@@ -7,7 +7,7 @@
 #include "fe_tile.h"
 
 template <int L, int PPL, bool GRAD, bool MIXED>
-__global__ __launch_bounds__(FE_WG) void k_eval(const double* __restrict__ rec,
+__global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_eval(const double* __restrict__ rec,
                                                 const float* __restrict__ g32, const int n_pairs,
                                                 const int ppc, const double* __restrict__ pts,
                                                 const int n_pts,
@@ -32,7 +32,7 @@ __global__ __launch_bounds__(FE_WG) void k_eval(const double* __restrict__ rec,
 // drained (every workgroup reaches the exit test after at most n_items + 1
 // increments of the counter).
 template <int L, int PPL, bool GRAD, bool MIXED>
-__global__ __launch_bounds__(FE_WG) void k_batch_eval(const FeBatchItem* __restrict__ items,
+__global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_batch_eval(const FeBatchItem* __restrict__ items,
                                                       const int n_items, int* __restrict__ counter,
                                                       const double* const* __restrict__ recs,
                                                       const float* const* __restrict__ g32s,

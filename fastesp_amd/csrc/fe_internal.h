@@ -13,6 +13,26 @@
 
 #define FE_WG 256  // threads per workgroup of the evaluation kernels (4 waves)
 
+// Minimum resident waves per SIMD requested from the compiler for the
+// evaluation (k_eval, k_batch_eval; 2 points per lane) and dL/dG (k_dG,
+// k_dG_batch) kernels of class L: the register budget becomes 512 / waves
+// VGPRs.  1 leaves the allocation to the compiler.  Overridable at build time (-DFE_EVAL_MINW_L=...,
+// -DFE_DG_MINW_L=..., comma lists of 7 values).
+#ifndef FE_EVAL_MINW_L
+#define FE_EVAL_MINW_L 1, 1, 1, 1, 1, 1, 1
+#endif
+#ifndef FE_DG_MINW_L
+#define FE_DG_MINW_L 1, 1, 1, 1, 1, 2, 1
+#endif
+__host__ __device__ constexpr int fe_eval_minw(int L, int ppl) {
+  constexpr int t[7] = {FE_EVAL_MINW_L};
+  return ppl == 2 ? t[L] : 1;  // the table is for 2 points per lane
+}
+__host__ __device__ constexpr int fe_dg_minw(int L) {
+  constexpr int t[7] = {FE_DG_MINW_L};
+  return t[L];
+}
+
 int fe_set_error(int code, const char* fmt, ...);
 
 #define FE_HIPCHK(x)                                                                         \

@@ -190,6 +190,12 @@ int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
  * recorded into a hipGraph are not timed. */
 int fe_profile(fe_ctx* ctx, int op, double* ms, int64_t* count, int n_phase);
 
+/* Sustained VALU FMA rate of the device in TFLOP/s, measured with independent
+ * FMA chains: FE_PREC_FP64 for FP64, FE_PREC_MIXED for FP32 as this library is
+ * compiled (non-packed).  No counterpart in the reference: it calibrates the
+ * roofline peaks of the benchmark. */
+int fe_valu_rate(fe_ctx* ctx, int precision, double* tflops);
+
 /* ---- batched conformers ------------------------------------------------
  * No counterpart in the reference (it evaluates one table per esp_gpu
  * process, fastesp/cli.py:143-163).  A batch references n_conf tables of one
