@@ -19,6 +19,8 @@ Printed JSON (one line): the driver's contract fields plus
                   sample, scaled to ms per 1e4 points;
   accuracy     -- max |dV| against the FP64 oracle at 256 sampled points;
   batched      -- the 256-conformer graph-captured step;
+  config3      -- 4538 functions, 200000 points: FP64 and mixed fwd+bwd with
+                  their errors against the FP64 oracle;
   density      -- the electron density at the same points on the FP64 matrix
                   cores, its roofline and its NumPy baseline.
 """
@@ -68,6 +70,8 @@ def parse():
     ap.add_argument("--batch-precisions", default="mixed,fp64")
     ap.add_argument("--density-steps", type=int, default=10,
                     help="timed evaluations of the electron-density leg (0 = skip it)")
+    ap.add_argument("--config3-steps", type=int, default=3,
+                    help="timed fwd+bwd steps of the config-3 leg, FP64 and mixed (0 = skip it)")
     return ap.parse_args()
 
 
@@ -314,6 +318,77 @@ def density_leg(a, w, torch, ctx, stream, with_cpu: bool) -> dict:
     return out
 
 
+def config3_leg(a, torch, ctx, stream) -> dict:
+    """Config 3: 150 atoms, 4538 functions up to f, 200000 points on four vdW
+    shells.  Forward + field + backward (dL/dP) per step in FP64 and in mixed
+    precision with the launch shape tuned on config 1; errors of V and of the
+    field against the FP64 oracle at the 256 points of tests/golden/cfg3.npz,
+    and of mixed against FP64 at every point."""
+    from fastesp_amd import _lib, synth
+    from fastesp_amd.table import build_table
+    g = np.load(ROOT / "tests" / "golden" / "cfg3.npz")
+    w = synth.config3()
+    n = len(w.points)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    t_build = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    sh = stream.cuda_stream
+    lib = ctx.lib
+    pts = torch.from_numpy(np.ascontiguousarray(w.points)).to(dev)
+    V = torch.empty(n, dtype=torch.float64, device=dev)
+    dVdC = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    dLdP = torch.empty((w.basis.n, w.basis.n), dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(3003)
+    wts = torch.from_numpy(rng.uniform(0.5, 1.0, n)).to(dev)
+    _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), n, _lib.FE_PREC_FP64, _lib.FE_PART_ESP,
+                                  _lib.FE_PTR_DEVICE, sh, V.data_ptr()), "fe_esp_forward")
+    target = V.clone() + torch.from_numpy(rng.normal(0, 1e-3, n)).to(dev)
+    idx = torch.from_numpy(g["idx"]).to(dev)
+    out = dict(molecule="config3 150 atoms", n_functions=int(w.basis.n), n_points=n,
+               n_pairs=int(tab.n_pairs), table_build_ms=round(t_build * 1e3, 1), legs={})
+    res = {}
+    for prec, p in (("fp64", _lib.FE_PREC_FP64), ("mixed", _lib.FE_PREC_MIXED)):
+        def step():
+            _lib.check(lib.fe_esp_forward_grad(tab._h, pts.data_ptr(), n, p, _lib.FE_PTR_DEVICE,
+                                               sh, V.data_ptr(), dVdC.data_ptr()),
+                       "fe_esp_forward_grad")
+            gV = 2.0 * wts * (V - target)
+            _lib.check(lib.fe_esp_backward(tab._h, pts.data_ptr(), n, gV.data_ptr(), p,
+                                           _lib.FE_PTR_DEVICE, sh, None, dLdP.data_ptr()),
+                       "fe_esp_backward")
+        step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.config3_steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.config3_steps
+        Vs, Fs = V[idx].cpu().numpy(), dVdC[idx].cpu().numpy()
+        res[prec] = (V.cpu().numpy(), dVdC.cpu().numpy(), dLdP.cpu().numpy())
+        dv = Vs - g["V"]
+        leg = dict(ms_per_step=round(ms, 2), ms_per_1e4_pts=round(ms / (n / 1e4), 4),
+                   max_abs_dV_vs_oracle_au=float(np.abs(dv).max()),
+                   rms_dV_vs_oracle_au=float(np.sqrt((dv ** 2).mean())))
+        if "field" in g.files:
+            leg["max_abs_dfield_vs_oracle_au"] = float(np.abs(Fs - g["field"]).max())
+        out["legs"][prec] = leg
+        log(f"config3 {prec}: {ms:.1f} ms/step, max |dV| vs oracle {leg['max_abs_dV_vs_oracle_au']:.2e}")
+    Vd, Fd, Pd = (res["mixed"][k] - res["fp64"][k] for k in range(3))
+    out["mixed_vs_fp64_all_points"] = dict(
+        max_abs_dV_au=float(np.abs(Vd).max()), rms_dV_au=float(np.sqrt((Vd ** 2).mean())),
+        max_abs_dfield_au=float(np.abs(Fd).max()),
+        max_rel_ddLdP=float(np.abs(Pd).max() / np.abs(res["fp64"][2]).max()))
+    out["speedup_mixed_vs_fp64_fwdbwd"] = round(
+        out["legs"]["fp64"]["ms_per_step"] / out["legs"]["mixed"]["ms_per_step"], 3)
+    tab.close()
+    return out
+
+
 def main() -> int:
     a = parse()
     if a.gpus != 1:
@@ -496,6 +571,10 @@ def main() -> int:
     log(f"measured VALU FMA rate: {valu_rate}")
 
     batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None
+    cfg3 = None
+    if a.config3_steps:
+        log("config 3 leg (200000 points)")
+        cfg3 = config3_leg(a, torch, ctx, stream)
     dens = None
     if a.density_steps:
         log("electron density leg")
@@ -533,6 +612,7 @@ def main() -> int:
         "roofline_step": roof_step,
         "valu_rate_measured": valu_rate,
         "batched": batched,
+        "config3": cfg3,
         "density": dens,
         "cpu_baseline": cpu,
     }

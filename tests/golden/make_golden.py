@@ -148,20 +148,49 @@ def make_cfg2_sub():
                         npairs=tab.n_pairs)
 
 
+def _port_table(ct, w):
+    """The C++ port's screened table as an oracle table (pair fields, terms in
+    the class key order) for ``evaluate.field`` / ``evaluate.esp``."""
+    from types import SimpleNamespace
+    from oracle.ints import pair_type
+    p6, g = ct.export()
+    L = p6[:, 5].astype(np.int32)
+    nt = np.array([pair_type(int(l))[0].shape[0] for l in range(7)])[L]
+    off = np.concatenate([[0], np.cumsum(nt)[:-1]]).astype(np.int64)
+    return SimpleNamespace(pp_exp=p6[:, 0], pp_cx=p6[:, 1], pp_cy=p6[:, 2], pp_cz=p6[:, 3],
+                           pp_K=p6[:, 4], pp_L=L, pp_off=off, tm_g=g, pts=w.points,
+                           xyz=np.asarray(w.wfn.coords, float), Z=w.wfn.Z)
+
+
+def _field_chunk(args):
+    tab, pts = args
+    return evaluate.field(tab, pts), evaluate.esp(tab, pts)
+
+
 def make_cfg3():
     """Config 3 (150 atoms, ~4500 functions, 200000 points): the NumPy oracle is
-    too slow to screen ~1e6 pairs against 2e5 points, so this fixture uses the
+    too slow to screen ~1e6 pairs against 2e5 points, so the screen and V use the
     C++/OpenMP restatement (oracle/cpu_port.cpp, NumPy screen rule), which
-    matches the NumPy oracle to 5e-14 a.u. on config 0 (test_cpu_port.py)."""
+    matches the NumPy oracle to 5e-14 a.u. on config 0 (test_cpu_port.py).  The
+    field at the same 256 points is the NumPy oracle's analytic field over the
+    port's table, whose NumPy V must agree with the port's V."""
     from oracle.cpu_port import CpuTable
     w = synth.config3()
     ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, rule=0)
     rng = np.random.default_rng(303)
     idx = np.sort(rng.choice(len(w.points), 256, replace=False))
     V = ct.esp(w.points[idx])
+    tab = _port_table(ct, w)
+    chunks = np.array_split(w.points[idx], 8)
+    with ProcessPoolExecutor(8) as ex:
+        res = list(ex.map(_field_chunk, [(tab, c) for c in chunks]))
+    field = np.concatenate([r[0] for r in res])
+    Vnp = np.concatenate([r[1] for r in res])
+    print(f"cfg3: NumPy V over the port's table vs the port: {np.abs(Vnp - V).max():.2e}")
+    assert np.abs(Vnp - V).max() < 1e-11
     np.savez_compressed(OUT / "cfg3.npz", hash=w.input_hash(), fp=w.fingerprint(), idx=idx, V=V,
-                        npairs=ct.n_pairs, n_candidates=ct.n_candidates, sum_rule=ct.sum_rule,
-                        n_func=w.basis.n)
+                        field=field, npairs=ct.n_pairs, n_candidates=ct.n_candidates,
+                        sum_rule=ct.sum_rule, n_func=w.basis.n)
 
 
 JOBS = dict(boys=make_boys, cfg0=make_cfg0, cfg1_300=make_cfg1_300, cfg1_30k=make_cfg1_30k,

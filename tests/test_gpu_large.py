@@ -56,3 +56,25 @@ def test_cfg3_fp64_and_mixed_full_size(cfg3):
     assert errm < 1e-8
     assert np.abs(Vm - V).max() < 1e-8
     np.testing.assert_allclose(t.nuclear() - t.electron(), V, rtol=0, atol=1e-10)
+
+
+def test_cfg3_field_fp64_and_mixed(cfg3):
+    """dV/dC at 200000 points against the oracle's analytic field at the 256
+    sampled points (FP64 <= 1e-9 a.u., mixed <= 1e-7 a.u.)."""
+    from fastesp_amd.autograd import esp_with_grad
+    w, g, t = cfg3
+    if "field" not in g.files:
+        pytest.skip("fixture without the oracle field (regenerate tests/golden/cfg3.npz)")
+    idx = g["idx"]
+    say("cfg3 fp64 field 200000")
+    V, F = esp_with_grad(t, w.points)
+    assert np.abs(V[idx] - g["V"]).max() < 1e-10
+    err = np.abs(F[idx] - g["field"]).max()
+    say(f"cfg3 fp64 field max err {err:.3e}")
+    assert err < 1e-9
+    say("cfg3 mixed field 200000")
+    Vm, Fm = esp_with_grad(t, w.points, precision=_lib.FE_PREC_MIXED)
+    errm = np.abs(Fm[idx] - g["field"]).max()
+    say(f"cfg3 mixed field max err {errm:.3e}, max |mixed - fp64| {np.abs(Fm - F).max():.3e}")
+    assert errm < 1e-7
+    assert np.abs(Vm - V).max() < 1e-8
