@@ -548,7 +548,8 @@ def main() -> int:
     counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
     roof = dominant_kernel_roofline(head["_phases"], ppl, n, a.mode, a.precision, ppl_used,
                                     fp32_used, flops, traffic_file, counters_file)
-    # whole step, for reference: all launches of one step against the blended peak
+    # whole step, for reference: all launches of one step against the vector peak
+    # (FP32 here is non-packed: flops.PEAK_FP32_VALU equals the FP64 peak)
     f_fwd = flops.forward_flops(ppl, n, info["n_atoms"])
     f_bwd = flops.backward_flops(ppl, n, info["n_atoms"]) if a.mode == "fwdbwd" else 0.0
     fl = f_fwd + f_bwd

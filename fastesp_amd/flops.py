@@ -19,7 +19,10 @@ NMONO = [1, 4, 10, 20, 35, 56, 84]
 STRIDE = [16, 16, 24, 32, 56, 88, 144]
 
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
-PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
+PEAK_FP32_PACKED = 157.3e12  # MI355X FP32 vector with packed (2-wide) FMAs, FLOP/s
+# FP32 as this library is compiled (packed FP32 disabled): one FMA per lane per
+# issue, the FP64 rate (fe_valu_rate measures both on the device)
+PEAK_FP32_VALU = PEAK_FP64_VALU
 PEAK_HBM = 8.0e12          # B/s
 
 
