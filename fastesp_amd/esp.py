@@ -28,9 +28,6 @@ from .formats import basis_payload
 # fastesp/esp.py:293
 ENGINE_Z = {"H": 1.0, "C": 6.0, "N": 7.0, "O": 8.0, "S": 16.0, "P": 15.0}
 
-_NMONO = (1, 4, 10, 20, 35, 56, 84)
-
-
 class PrimPair:
     """One contracted primitive pair (``fastesp/esp.py:150-156``)."""
     __slots__ = ("p", "center", "K", "alpha", "m", "coef", "bound", "npow", "reff")
@@ -86,8 +83,16 @@ class ESPEngine:
     def _payload(self, points) -> dict:
         return basis_payload(self.basis.wfn, self.basis, self.density, points)
 
-    def _table(self, points, tol: float, Z=None):
+    @staticmethod
+    def _points(points) -> np.ndarray:
         pts = np.ascontiguousarray(np.asarray(points, float).reshape(-1, 3))
+        if len(pts) == 0:
+            # the reference's d.max() over an empty point set raises
+            raise ValueError("zero-size array to reduction operation maximum which has no identity")
+        return pts
+
+    def _table(self, points, tol: float, Z=None):
+        pts = self._points(points)
         pl = self._payload(pts)
         if Z is not None:
             pl["Z"] = Z
@@ -108,6 +113,7 @@ class ESPEngine:
 
     def esp(self, points, tol: float = 1e-15, verbose: bool = False) -> np.ndarray:
         """nuclear - electron in Hartree/e (``fastesp/esp.py:285-296``)."""
+        self._points(points)   # the electron part fails first, as in the reference
         w = self.basis.wfn
         Z = np.array([ENGINE_Z[s] for s in w.symbols], np.float64)
         pts, t = self._table(points, tol, Z)

@@ -138,3 +138,13 @@ def test_phase_profile_counts_every_class_launch():
     assert ph[1][1] == 1 and ph[2][1] == 1 and ph[3][1] == 1
     ctx.profile_start()
     assert ctx.profile_stop() == {}
+
+
+def test_backward_without_points_is_zero():
+    """No points: dL/dG = 0 and dL/dP = 0 (every output written)."""
+    wfn, basis, P, _ = cf_toy()
+    t = build_table_hip(wfn, basis, P, np.zeros((4, 3)) + 3.0, tol=0.0)
+    dG, dP = esp_backward(t, np.zeros((0, 3)), np.zeros(0), n_func=basis.n)
+    assert not dG.any() and not dP.any()
+    V, F = esp_with_grad(t, np.zeros((0, 3)))
+    assert V.shape == (0,) and F.shape == (0, 3)

@@ -68,3 +68,17 @@ def test_surface_points_match_oracle():
         want = density.vdw_surface_points(wl.wfn.symbols, wl.wfn.coords, n, seed=seed)
         assert got.shape == want.shape
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_density_edge_cases():
+    """No points, one point, points on the nuclei (r = 0: every s function at
+    its maximum, every other component zero) and a ragged count (not a
+    multiple of the 64- or 128-point tiles)."""
+    w = read_molden(GOLD / "toy.molden")
+    b = Basis(w)
+    P = b.density_matrix_cartesian()
+    ed = ElectronDensity(w, b, P)
+    say("density edge cases")
+    assert ed.rho(np.zeros((0, 3))).shape == (0,)
+    for pts in (w.coords[:1], w.coords, np.random.default_rng(9).normal(0, 2, (131, 3))):
+        _check(ed.rho(pts), density.rho(b, P, pts))

@@ -84,3 +84,11 @@ def test_engine_charges_outside_its_table_raise():
     assert np.isfinite(eng.esp_electron(pts)).all()
     with pytest.raises(KeyError):
         eng.esp(pts)
+
+
+def test_engine_without_points_raises_like_the_reference():
+    """``d.max()`` over no points raises in ``fastesp/esp.py:266``."""
+    wfn, basis, P, _ = cf_toy()
+    eng = ESPEngine(basis, P)
+    with pytest.raises(ValueError):
+        eng.esp_electron(np.zeros((0, 3)))
