@@ -26,26 +26,28 @@
 #include "fe_adjoint.h"
 #include "fe_reduce.h"
 
-__global__ __launch_bounds__(64) void k_esp_reduce_grad(const double* __restrict__ partial,
-                                                         const int n_rows, const int ld,
-                                                         const size_t plane,
-                                                         const double* __restrict__ pts,
-                                                         const int n_pts,
-                                                         const double* __restrict__ atoms,
-                                                         const int n_atoms,
-                                                         const int* __restrict__ perm,
-                                                         double* __restrict__ V,
-                                                         double* __restrict__ dVdC) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= n_pts) return;
-  double e, gx, gy, gz, nuc, nx, ny, nz;
-  fe_sum_rows4(partial + i, n_rows, ld, plane, e, gx, gy, gz);
+// 64 points per 256-thread block, each point's rows split over the 4 waves
+__global__ __launch_bounds__(256) void k_esp_reduce_grad(const double* __restrict__ partial,
+                                                          const int n_rows, const int ld,
+                                                          const size_t plane,
+                                                          const double* __restrict__ pts,
+                                                          const int n_pts,
+                                                          const double* __restrict__ atoms,
+                                                          const int n_atoms,
+                                                          const int* __restrict__ perm,
+                                                          double* __restrict__ V,
+                                                          double* __restrict__ dVdC) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  double sum[4];
+  fe_sum_rows_split<4>(partial + i, i < n_pts, n_rows, ld, plane, sum);
+  if (threadIdx.x >= 64 || i >= n_pts) return;
+  double nuc, nx, ny, nz;
   fe_nuclear_grad(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], atoms, n_atoms, nuc, nx, ny, nz);
   const int o = perm ? perm[i] : i;
-  V[o] = nuc - e;
-  dVdC[3 * o] = nx - gx;
-  dVdC[3 * o + 1] = ny - gy;
-  dVdC[3 * o + 2] = nz - gz;
+  V[o] = nuc - sum[0];
+  dVdC[3 * o] = nx - sum[1];
+  dVdC[3 * o + 1] = ny - sum[2];
+  dVdC[3 * o + 2] = nz - sum[3];
 }
 
 extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, int precision,
@@ -87,7 +89,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, plane, true, precision == FE_PREC_MIXED, st);
   if (rc) return rc;
   const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
-  hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st,
+  hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st,
                      partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);
   fe_prof_end(c, pr, st);
@@ -136,7 +138,10 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, const int6
   if (i < n) x[i] = v;
 }
 
-// dG[pair * NT + k] = sum over chunks (fixed order); optional flat scatter
+// dG[pair * NT + k] = sum over chunks; optional flat scatter.  64 (k, pair)
+// elements per 256-thread block: wave s adds the chunks c = s (mod 4) with two
+// chains, and the four slices are combined in a fixed order in LDS (bitwise
+// reproducible).
 template <int L>
 __global__ __launch_bounds__(256) void k_dG_reduce(const double* __restrict__ partial,
                                                    const int n_pairs, const int n_chunks,
@@ -144,11 +149,27 @@ __global__ __launch_bounds__(256) void k_dG_reduce(const double* __restrict__ pa
                                                    const int* __restrict__ flatoff,
                                                    double* __restrict__ flat) {
   constexpr int NT = FeL<L>::NT;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)n_pairs * NT) return;
-  const int k = (int)(e / n_pairs), pair = (int)(e - (int64_t)k * n_pairs);
-  double s = 0.0;
-  for (int c = 0; c < n_chunks; ++c) s += partial[((size_t)c * NT + k) * n_pairs + pair];
+  __shared__ double s_red[4][64];
+  const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = e < (int64_t)n_pairs * NT;
+  const int k = valid ? (int)(e / n_pairs) : 0;
+  const int pair = valid ? (int)(e - (int64_t)k * n_pairs) : 0;
+  double a = 0.0, b = 0.0;
+  if (valid) {
+    const size_t stride = (size_t)NT * n_pairs;
+    const double* p = partial + (size_t)k * n_pairs + pair;
+    int c = sl;
+    for (; c + 4 < n_chunks; c += 8) {
+      a += p[(size_t)c * stride];
+      b += p[(size_t)(c + 4) * stride];
+    }
+    if (c < n_chunks) a += p[(size_t)c * stride];
+  }
+  s_red[sl][lane] = a + b;
+  __syncthreads();
+  if (sl != 0 || !valid) return;
+  const double s = (s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]);
   dG[(size_t)pair * NT + k] = s;
   if (flat) flat[flatoff[pair] + k] = s;
 }
@@ -198,7 +219,7 @@ static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, in
   fe_prof_end(t->ctx, pg, st);
   const int64_t ne = nL * FeL<L>::NT;
   const int pr = fe_prof_begin(t->ctx, FE_PH_DGRED0 + L, st);
-  hipLaunchKernelGGL(k_dG_reduce<L>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_dG_reduce<L>, dim3((unsigned)((ne + 63) / 64)), dim3(256), 0, st,
                      part + d.part_off[L], (int)nL, d.chunks[L], dG + d.dg_off[L],
                      t->d_flatoff[L], flat);
   fe_prof_end(t->ctx, pr, st);

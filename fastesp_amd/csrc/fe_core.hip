@@ -589,17 +589,21 @@ extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx,
 // kernels
 // ---------------------------------------------------------------------------
 // V = [sum_A Z_A/|C-R_A|] - [sum of partial rows], rows added in a fixed order.
-__global__ __launch_bounds__(64) void k_esp_reduce(const double* __restrict__ partial,
-                                                    const int n_rows, const int ld,
-                                                    const double* __restrict__ pts, const int n_pts,
-                                                    const double* __restrict__ atoms,
-                                                    const int n_atoms, const int parts,
-                                                    const int* __restrict__ perm,
-                                                    double* __restrict__ V) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= n_pts) return;
+// 64 points per 256-thread block, each point's rows split over the 4 waves
+__global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ partial,
+                                                     const int n_rows, const int ld,
+                                                     const double* __restrict__ pts,
+                                                     const int n_pts,
+                                                     const double* __restrict__ atoms,
+                                                     const int n_atoms, const int parts,
+                                                     const int* __restrict__ perm,
+                                                     double* __restrict__ V) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  double sum[1];
+  fe_sum_rows_split<1>(partial + i, i < n_pts, n_rows, ld, 0, sum);
+  if (threadIdx.x >= 64 || i >= n_pts) return;
   const int o = perm ? perm[i] : i;
-  const double e = fe_sum_rows(partial + i, n_rows, ld);
+  const double e = sum[0];
   const double nuc = (parts & FE_PART_NUCLEAR)
                          ? fe_nuclear(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], atoms, n_atoms)
                          : 0.0;
@@ -704,7 +708,7 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
   }
   const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
   const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
-  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, partial,
+  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, partial,
                      nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, perm, d_V);
   fe_prof_end(c, pr, st);
   FE_HIPCHK(hipGetLastError());

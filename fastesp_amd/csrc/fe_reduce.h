@@ -52,6 +52,46 @@ __device__ __forceinline__ void fe_sum_rows4(const double* __restrict__ p, const
   gz += gz1;
 }
 
+// Split reduction for 256-thread blocks of 64 points: wave s of the block adds
+// the rows r = s (mod 4) of its 64 points (coalesced rows), the four slices
+// meet in LDS and are added in the fixed order (s0 + s1) + (s2 + s3), so the
+// result stays bitwise reproducible.  NP planes `plane` apart.  Every thread of
+// the block must call it (it synchronises); returns the sums in the threads of
+// wave 0.
+template <int NP>
+__device__ __forceinline__ void fe_sum_rows_split(const double* __restrict__ p, const bool valid,
+                                                  const int n_rows, const size_t ld,
+                                                  const size_t plane, double* out) {
+  __shared__ double s_red[NP][4][64];
+  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+  double a[NP], b[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) a[q] = b[q] = 0.0;
+  if (valid) {
+    int r = s;
+    for (; r + 4 < n_rows; r += 8) {
+      const size_t o = (size_t)r * ld, o1 = o + 4 * ld;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        a[q] += p[q * plane + o];
+        b[q] += p[q * plane + o1];
+      }
+    }
+    if (r < n_rows) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q) a[q] += p[q * plane + (size_t)r * ld];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) s_red[q][s][lane] = a[q] + b[q];
+  __syncthreads();
+  if (s == 0) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      out[q] = (s_red[q][0][lane] + s_red[q][1][lane]) + (s_red[q][2][lane] + s_red[q][3][lane]);
+  }
+}
+
 // sum_A Z_A / |C - R_A|; atoms as (x, y, z, Z)
 __device__ __forceinline__ double fe_nuclear(const double x, const double y, const double z,
                                              const double* __restrict__ atoms, const int n_atoms) {
