@@ -2,7 +2,7 @@
 //
 // fe_dg_block<L>: one lane per pair of class L; the points (x, y, z, gV) of
 //   [q0, q1) are wave-uniform scalar loads, so there is no cross-lane
-//   reduction.  Writes out[k * stride + pair] = -K sum_i gV_i R_i^a F_m(T_i).
+//   reduction.  Writes out[k * stride + pair] = -sum_i gV_i K R_i^a F_m(T_i).
 // fe_dg_to_dp_sp: dL/dG -> dL/dP for one shell pair (64 lanes):
 //   d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t], d blk_{mu nu} = c_i c_j
 //   sum_t d etab_t Ex Ey Ez; both triangles of P receive d blk because the
@@ -35,10 +35,11 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   const int ip = min(pair, n_pairs - 1);
   const double* r = rec + (size_t)ip * S;
   const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
-  const double hp = r[6], cF = r[7];
+  const double hp = r[6], KF = r[7];
   double acc[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) acc[k] = 0.0;
+  // the weight carries K (and F_0 in the far field), so acc sums K R^a F_m gV
   auto point = [&](const double x, const double y, const double z, double w) {
     const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
     const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
@@ -46,9 +47,10 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
     if (!__any(p * r2 < fe_tfar(L))) {
       const double ri = fe_rsqrt_pos(r2);
       fe_boys_far_scaled<L>(hp * (ri * ri), F);
-      w *= cF * ri;
+      w *= KF * ri;
     } else {
       fe_boys<L>(p * r2, F, s_tab);
+      w *= K;
     }
     fe_dg_accum<L>(Rx, Ry, Rz, F, w, acc);
   };
@@ -65,7 +67,7 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   }
   if (pair < n_pairs) {
 #pragma unroll
-    for (int k = 0; k < NT; ++k) out[(size_t)k * stride + pair] = -K * acc[k];
+    for (int k = 0; k < NT; ++k) out[(size_t)k * stride + pair] = -acc[k];
   }
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ double fe_exp_small(const double x) {
 // exactly to double precision.  A pair of class L then contributes
 //   K F_0 sum_m F'_m a_m(R),  F'_m = prod_{j<=m} (2j-1) x,  x = 1/(2 p R^2),
 // with K F_0 = K sqrt(pi/p)/2 * |R|^-1: no table, no exp, no division per point
-// (1/(2p) and sqrt(pi/p)/2 are stored per pair).
+// (1/(2p) and K sqrt(pi/p)/2 are stored per pair).
 __host__ __device__ constexpr double fe_tfar(int M) {
   return M <= 0 ? 35.5 : M == 1 ? 39.5 : M == 2 ? 43.0 : M == 3 ? 46.0 : M == 4 ? 49.0
        : M == 5 ? 51.5 : M == 6 ? 54.0 : M == 7 ? 56.5 : 59.0;

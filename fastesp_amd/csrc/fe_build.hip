@@ -265,8 +265,9 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, 
   const int r = (int)(e / S), c = (int)(e - (int64_t)r * S);
   const double* s = src + (size_t)idx[r] * S;
   // slots 6/7 of a candidate hold reff and the screen flags; the compacted
-  // record keeps the per-pair far-field constants there instead
-  dst[e] = c == 6 ? 0.5 / s[3] : c == 7 ? 0.88622692545275801365 / sqrt(s[3]) : s[c];
+  // record keeps the per-pair far-field constants there instead: 1/(2p) and
+  // K sqrt(pi/p)/2
+  dst[e] = c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
 }
 
 // (Px, Py, Pz, p) of the records idx[0..n) of a candidate buffer, packed

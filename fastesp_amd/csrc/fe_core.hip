@@ -4,7 +4,7 @@
 // L = l_A + l_B (0..FE_LMAX) are stored contiguously, one 64-byte aligned
 // record of FeL<L>::STRIDE doubles per primitive pair:
 //
-//     [0] Px [1] Py [2] Pz [3] p [4] K [5] K*bound [6] 1/(2p) [7] sqrt(pi/p)/2
+//     [0] Px [1] Py [2] Pz [3] p [4] K [5] K*bound [6] 1/(2p) [7] K sqrt(pi/p)/2
 //     [8 .. 8+NT)  g_k in canonical key order, zero padded to a multiple of 8
 //
 // A wavefront evaluates 64 points (x points-per-lane) against the same pair,
@@ -462,7 +462,7 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
     r[4] = pp_K[i];
     r[5] = INFINITY;  // K*bound unknown for uploaded tables
     r[6] = 0.5 / pp_exp[i];                        // 1/(2p)
-    r[7] = 0.88622692545275801365 / sqrt(pp_exp[i]);  // sqrt(pi/p)/2
+    r[7] = pp_K[i] * (0.88622692545275801365 / sqrt(pp_exp[i]));  // K sqrt(pi/p)/2
     for (int j = 0; j < nt; ++j) r[8 + j] = tm_g[off + j];
     orig[L].push_back(i);
   }

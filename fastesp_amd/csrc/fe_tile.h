@@ -49,7 +49,7 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
     const bool far = !__any(near);
     bool fp64 = true;
     if constexpr (MIXED) fp64 = __any(big);
-    const double hp = r[6], KF = K * r[7];
+    const double hp = r[6], KF = r[7];
     const fe_gfptr g = g32 + (size_t)ip * S32;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {

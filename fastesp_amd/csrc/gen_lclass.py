@@ -165,10 +165,11 @@ def gen_L(L):
                 add(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
     o.extend(lines)
     ms = sorted({k[3] for k in keys})
-    # I = sum_m a_m F_m ; S1 = sum_m a_m F_{m+1}
-    i_expr = "0.0"
-    s_expr = "0.0"
-    for m in ms:
+    # I = sum_m a_m F_m ; S1 = sum_m a_m F_{m+1}; each chain starts with a
+    # product (not fma(., ., 0.0)) so a caller's F[0] = 1 folds away
+    i_expr = f"a{ms[0]} * F[{ms[0]}]"
+    s_expr = f"a{ms[0]} * F[{ms[0] + 1}]"
+    for m in ms[1:]:
         i_expr = f"fma(a{m}, F[{m}], {i_expr})"
         s_expr = f"fma(a{m}, F[{m + 1}], {s_expr})"
     o.append(f"  I = {i_expr};")
@@ -207,8 +208,9 @@ def gen_L(L):
             o.append(f"  float a{m} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
         else:
             o.append(f"  a{m} = " + (f"a{m} + g[{k}];" if mj == 0 else f"fmaf(g[{k}], m{mj}, a{m});"))
-    expr = "0.0"
-    for m in range(L, -1, -1):
+    top = max(seen)
+    expr = f"(double)a{top} * F[{top}]"
+    for m in range(top - 1, -1, -1):
         if m in seen:
             expr = f"fma((double)a{m}, F[{m}], {expr})"
     o.append(f"  return {expr};")
@@ -239,8 +241,9 @@ def gen_L(L):
                 lo[d] -= 1
                 add32(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
     o.extend(lines)
-    i_expr, s_expr = "0.0", "0.0"
-    for m in ms:
+    i_expr = f"(double)a{ms[0]} * F[{ms[0]}]"
+    s_expr = f"(double)a{ms[0]} * F[{ms[0] + 1}]"
+    for m in ms[1:]:
         i_expr = f"fma((double)a{m}, F[{m}], {i_expr})"
         s_expr = f"fma((double)a{m}, F[{m + 1}], {s_expr})"
     o.append(f"  I = {i_expr};")
