@@ -139,9 +139,10 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, const int6
 }
 
 // dG[pair * NT + k] = sum over chunks; optional flat scatter.  64 (k, pair)
-// elements per 256-thread block: wave s adds the chunks c = s (mod 4) with two
-// chains, and the four slices are combined in a fixed order in LDS (bitwise
-// reproducible).
+// elements per 256-thread block: wave s adds the chunks c = s (mod 4) with four
+// chains (four loads in flight per lane: the high classes have few elements
+// and hundreds of chunks, so the sum is latency-bound), and the four slices are
+// combined in a fixed order in LDS (bitwise reproducible).
 template <int L>
 __global__ __launch_bounds__(256) void k_dG_reduce(const double* __restrict__ partial,
                                                    const int n_pairs, const int n_chunks,
@@ -155,18 +156,22 @@ __global__ __launch_bounds__(256) void k_dG_reduce(const double* __restrict__ pa
   const bool valid = e < (int64_t)n_pairs * NT;
   const int k = valid ? (int)(e / n_pairs) : 0;
   const int pair = valid ? (int)(e - (int64_t)k * n_pairs) : 0;
-  double a = 0.0, b = 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (valid) {
     const size_t stride = (size_t)NT * n_pairs;
     const double* p = partial + (size_t)k * n_pairs + pair;
     int c = sl;
-    for (; c + 4 < n_chunks; c += 8) {
-      a += p[(size_t)c * stride];
-      b += p[(size_t)(c + 4) * stride];
+    for (; c + 12 < n_chunks; c += 16) {
+      const double x0 = p[(size_t)c * stride], x1 = p[(size_t)(c + 4) * stride];
+      const double x2 = p[(size_t)(c + 8) * stride], x3 = p[(size_t)(c + 12) * stride];
+      a0 += x0;
+      a1 += x1;
+      a2 += x2;
+      a3 += x3;
     }
-    if (c < n_chunks) a += p[(size_t)c * stride];
+    for (; c < n_chunks; c += 4) a0 += p[(size_t)c * stride];
   }
-  s_red[sl][lane] = a + b;
+  s_red[sl][lane] = (a0 + a1) + (a2 + a3);
   __syncthreads();
   if (sl != 0 || !valid) return;
   const double s = (s_red[0][lane] + s_red[1][lane]) + (s_red[2][lane] + s_red[3][lane]);
