@@ -318,13 +318,25 @@ def density_leg(a, w, torch, ctx, stream, with_cpu: bool) -> dict:
     return out
 
 
-def config3_leg(a, torch, ctx, stream) -> dict:
+C3_FP32_CLASSES = [0, 0, 0, 1, 1, 1, 1]  # the library default under mixed precision
+
+
+def config3_leg(a, torch, ctx, stream, shape=None) -> dict:
     """Config 3: 150 atoms, 4538 functions up to f, 200000 points on four vdW
     shells.  Forward + field + backward (dL/dP) per step in FP64 and in mixed
-    precision with the launch shape tuned on config 1; errors of V and of the
-    field against the FP64 oracle at the 256 points of tests/golden/cfg3.npz,
-    and of mixed against FP64 at every point."""
+    precision, at the points-per-lane and workgroup target tuned on config 1
+    (`shape`) and, for mixed, the FP32 contraction in classes 3..6 (the library
+    default, so the leg measures the FP32 error at this size whatever the
+    config-1 tuner kept); errors of V and of the field against the FP64 oracle
+    at the 256 points of tests/golden/cfg3.npz, and of mixed against FP64 at
+    every point."""
     from fastesp_amd import _lib, synth
+    if shape:
+        for L, v in enumerate(shape["ppl_L"]):
+            ctx.set_tuning(10 + L, int(v))
+        ctx.set_tuning(1, int(shape["target_wg"]))
+    for L, v in enumerate(C3_FP32_CLASSES):
+        ctx.set_tuning(20 + L, v)
     from fastesp_amd.table import build_table
     g = np.load(ROOT / "tests" / "golden" / "cfg3.npz")
     w = synth.config3()
@@ -347,7 +359,10 @@ def config3_leg(a, torch, ctx, stream) -> dict:
     target = V.clone() + torch.from_numpy(rng.normal(0, 1e-3, n)).to(dev)
     idx = torch.from_numpy(g["idx"]).to(dev)
     out = dict(molecule="config3 150 atoms", n_functions=int(w.basis.n), n_points=n,
-               n_pairs=int(tab.n_pairs), table_build_ms=round(t_build * 1e3, 1), legs={})
+               n_pairs=int(tab.n_pairs), table_build_ms=round(t_build * 1e3, 1),
+               ppl_L=list(shape["ppl_L"]) if shape else None,
+               target_wg=shape["target_wg"] if shape else None,
+               mixed_fp32_classes=C3_FP32_CLASSES, legs={})
     res = {}
     for prec, p in (("fp64", _lib.FE_PREC_FP64), ("mixed", _lib.FE_PREC_MIXED)):
         def step():
@@ -575,7 +590,7 @@ def main() -> int:
     cfg3 = None
     if a.config3_steps:
         log("config 3 leg (200000 points)")
-        cfg3 = config3_leg(a, torch, ctx, stream)
+        cfg3 = config3_leg(a, torch, ctx, stream, tuned.get(key))
     dens = None
     if a.density_steps:
         log("electron density leg")
