@@ -108,7 +108,9 @@ __device__ __forceinline__ void fe_boys(const double T, double* F, const double*
     const double r = fe_rsqrt_pos(T);
     F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2
     if constexpr (M > 0) {
-      const double x = 0.5 / T;
+      // 1/(2T) from the reciprocal square root already at hand (a few ulp;
+      // a division would cost a dozen instructions on every lane of the branch)
+      const double x = 0.5 * (r * r);
 #pragma unroll
       for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((2 * m - 1) * x);
     }
