@@ -1,0 +1,69 @@
+"""The bench.py contract: one JSON line with the metric, the roofline of the
+dominant launch and the CPU baseline; one GPU only.
+
+CPU: the multi-GPU refusal and the CPU-baseline leg on a small molecule.
+GPU: a short run of the headline legs, checked key by key.
+"""
+import json
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+REQUIRED = {"metric": str, "value": float, "unit": str, "n_gpus": int, "steps": int,
+            "warmup": int, "ms_per_step": float, "higher_is_better": bool, "scaling": str,
+            "dtype": str, "data": str, "config": dict}
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
+
+
+def test_cpu_baseline_leg_on_config0():
+    sys.path.insert(0, str(ROOT))
+    import bench
+    from fastesp_amd import synth
+    w = synth.config0()
+    out = bench.cpu_baseline(w, 0.2)
+    assert out["kind"] == "port" and out["unit"] == "ms/1e4 pts"
+    assert out["cores"] >= 1 and out["value"] > 0
+    assert "scaled linearly to 1e4" in out["sample"]
+    if (ROOT / "oracle" / "_ref" / "build_pairs").exists():
+        assert out["reference_build_pairs_rc"] == 0
+
+
+@pytest.mark.gpu
+def test_bench_json_line():
+    print("[gpu-case] bench.py short headline run", flush=True)
+    cmd = [sys.executable, "-u", str(ROOT / "bench.py"), "--steps", "2", "--warmup", "1",
+           "--no-tune", "--no-cpu", "--batch", "0", "--config3-steps", "0",
+           "--density-steps", "0", "--legs", "fwd_fp64,fwdbwd_fp64,fwdbwd_mixed"]
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ))
+    assert r.returncode == 0, r.stderr[-3000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    for k, t in REQUIRED.items():
+        assert isinstance(d[k], t), k
+    assert d["n_gpus"] == 1 and d["steps"] == 2 and d["warmup"] == 1
+    assert d["higher_is_better"] is False and d["vs_baseline"] is None
+    assert d["config"]["workload"] == "esp_fwdbwd_mixed"
+    assert d["value"] > 0 and np.isclose(d["value"], d["ms_per_step"] / 3.0, rtol=1e-3)
+    roof = d["roofline"]
+    for k in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
+        assert k in roof, k
+    assert roof["peak"] == 78.6 and roof["unit"] == "TFLOP/s"
+    assert np.isclose(roof["frac"], roof["achieved"] / roof["peak"], rtol=1e-2)
+    assert roof["kernel"].startswith("k_eval<")
+    assert d["cpu_baseline"] is None  # --no-cpu
+    for leg in ("fwd_fp64", "fwdbwd_fp64", "fwdbwd_mixed"):
+        assert d["legs"][leg]["max_abs_dV_au"] < 1e-10, leg
+    assert d["reference_published_build_plus_fwd"]["reference"] == 819.0
