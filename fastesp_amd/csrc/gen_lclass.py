@@ -104,6 +104,53 @@ def emit_monos(out, L, order, parent, axis, T="double", pre="m", R=("Rx", "Ry", 
         out.append(f"  const {T} {pre}{j} = {src + ' * ' + r if src else r};")
 
 
+def parent_chain(alpha):
+    """alpha and its ancestors down to 0, each step lowering the first nonzero
+    axis (the parent rule of schedule_of)."""
+    out = [tuple(alpha)]
+    a = list(alpha)
+    while sum(a) > 0:
+        for ax in range(3):
+            if a[ax] > 0:
+                a[ax] -= 1
+                break
+        out.append(tuple(a))
+    return out
+
+
+def dg_split(L, svec):
+    """Split every key (a, m) of class L into gamma (ancestor of a at degree
+    |a| - min(|a|, s_m)) and beta = a - gamma; return the product count, the
+    splits, the monomials needed (closed under parents) and the weighted
+    prefixes per order."""
+    keys, _ = keys_of(L)
+    ms = sorted({k[3] for k in keys})
+    monos, weighted, terms = set(), {m: set() for m in ms}, []
+    for k, (ax, ay, az, m) in enumerate(keys):
+        a = (ax, ay, az)
+        gamma = parent_chain(a)[min(sum(a), svec[ms.index(m)])]
+        beta = tuple(a[i] - gamma[i] for i in range(3))
+        terms.append((k, m, gamma, beta))
+        weighted[m].update(parent_chain(gamma))
+        monos.update(parent_chain(beta))
+    cost = sum(1 for b in monos if sum(b) >= 2) + sum(len(v) for v in weighted.values())
+    return cost, terms, monos, weighted
+
+
+def dg_plan(L):
+    """The cheapest split among s_m = min(m, c), c = 0..max degree (this family
+    holds the optimum over all s vectors for L <= 6, checked by enumeration)."""
+    keys, _ = keys_of(L)
+    ms = sorted({k[3] for k in keys})
+    maxd = max(sum(k[:3]) for k in keys)
+    best = None
+    for c in range(maxd + 1):
+        r = dg_split(L, [min(m, c) for m in ms])
+        if best is None or r[0] < best[0]:
+            best = r
+    return best
+
+
 def gen_L(L):
     keys, acc = keys_of(L)
     order, parent, axis, aidx = schedule_of(L)
@@ -183,16 +230,42 @@ def gen_L(L):
     o.append("}")
 
     # ---------------- dL/dG accumulation: acc[k] += w * R^a_k * F_mk
+    # Each term is one fma(X, Y, acc) with X Y = w_m R^a, X = w_m R^gamma a
+    # weighted prefix (gamma an ancestor of a on the parent chain) and Y = R^beta,
+    # beta = a - gamma, a monomial of degree min(|a|, s_m).  s_m is chosen per
+    # Boys order to minimise the products that build the X's and Y's (w_m R^a
+    # for every term of a high order would rebuild long monomials once per
+    # order; shared monomials alone would need every monomial of the class).
     o.append(f"template <> __device__ __forceinline__ void fe_dg_accum<{L}>("
              "const double Rx, const double Ry, const double Rz, const double* F, "
              "const double w, double* __restrict__ acc) {")
-    emit_monos(o, L, order, parent, axis)
-    wf = {}
+    midx = {mm: i for i, mm in enumerate(order)}
+    _, terms, mclos, weighted = dg_plan(L)
+    def rname(j):  # a monomial by its schedule index: R_axis at degree 1
+        return ("Rx", "Ry", "Rz")[axis[j]] if sum(order[j]) == 1 else f"m{j}"
+
+    for j, mm in enumerate(order):  # monomials of degree >= 2 that a Y needs
+        if sum(mm) >= 2 and mm in mclos:
+            o.append(f"  const double m{j} = {rname(parent[j])} * {('Rx', 'Ry', 'Rz')[axis[j]]};")
     for m in ms:
         o.append(f"  const double w{m} = w * F[{m}];")
-    for k, (ax, ay, az, m) in enumerate(keys):
-        mj = aidx[k]
-        o.append(f"  acc[{k}] = " + (f"acc[{k}] + w{m};" if mj == 0 else f"fma(w{m}, m{mj}, acc[{k}]);"))
+        for j, mm in enumerate(order):  # weighted prefixes in parent-before-child order
+            if sum(mm) >= 1 and mm in weighted[m]:
+                src = f"w{m}" if parent[j] == 0 else f"v{m}_{parent[j]}"
+                o.append(f"  const double v{m}_{j} = {src} * {('Rx', 'Ry', 'Rz')[axis[j]]};")
+
+    def wname(m, gamma):
+        return f"w{m}" if sum(gamma) == 0 else f"v{m}_{midx[gamma]}"
+
+    def yname(beta):
+        if sum(beta) == 1:
+            return ("Rx", "Ry", "Rz")[beta.index(1)]
+        return f"m{midx[beta]}"
+
+    for k, m, gamma, beta in terms:
+        x = wname(m, gamma)
+        o.append(f"  acc[{k}] = " + (f"acc[{k}] + {x};" if sum(beta) == 0
+                                     else f"fma({x}, {yname(beta)}, acc[{k}]);"))
     o.append("}")
 
     # ---------------- mixed: FP32 monomials and per-m polynomials, FP64 F and final sum

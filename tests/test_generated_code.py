@@ -1,0 +1,83 @@
+"""The generated per-class device code (fastesp_amd/csrc/fe_lclass.inc), run on
+the CPU: each FP64 function body is a list of straight-line assignments, which
+is evaluated here statement by statement and compared with the sums it must
+form over the class's keys (a, m):
+
+* fe_contract<L>:       sum_k g_k R^a F_m
+* fe_contract_grad<L>:  the same, and sum_k g_k [d(R^a)/dR F_m - 2p R R^a F_{m+1}]
+* fe_dg_accum<L>:       acc_k += w R^a F_m (the product plan of the generator)
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+INC = Path(__file__).resolve().parents[1] / "fastesp_amd" / "csrc" / "fe_lclass.inc"
+
+
+def body(src: str, fn: str, L: int) -> list:
+    start = src.index(f"{fn}<{L}>(")
+    start = src.index("{", start) + 1
+    end = src.index("\n}\n", start)
+    out = []
+    for line in src[start:end].strip().splitlines():
+        s = line.strip().rstrip(";")
+        s = re.sub(r"^(const )?double ", "", s)
+        out.append(s)
+    return out
+
+
+def run(lines, env):
+    env = dict(env, fma=lambda a, b, c: a * b + c)
+    for s in lines:
+        if s.startswith("return "):
+            return eval(s[7:], {}, env)
+        exec(s, {}, env)
+    return env
+
+
+def keys_of(src: str, L: int):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen", INC.with_name("gen_lclass.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    return gen.keys_of(L)[0]
+
+
+@pytest.fixture(scope="module")
+def src():
+    return INC.read_text()
+
+
+@pytest.mark.parametrize("L", range(7))
+def test_generated_contractions(src, L):
+    keys = keys_of(src, L)
+    rng = np.random.default_rng(40 + L)
+    R = rng.uniform(-1.5, 1.5, 3)
+    F = list(rng.uniform(0.1, 1.0, L + 2))
+    g = list(rng.normal(size=len(keys)))
+    p, w = 0.7, 1.3
+    env = dict(Rx=R[0], Ry=R[1], Rz=R[2], F=F, g=g, p=p)
+    mono = [np.prod(R ** np.array(k[:3])) for k in keys]
+    want = sum(g[i] * mono[i] * F[k[3]] for i, k in enumerate(keys))
+    got = run(body(src, "fe_contract", L), env)
+    assert got == pytest.approx(want, rel=1e-13, abs=1e-13)
+
+    e = run(body(src, "fe_contract_grad", L), env)
+    assert e["I"] == pytest.approx(want, rel=1e-13, abs=1e-13)
+    for d, name in enumerate(("Gx", "Gy", "Gz")):
+        gd = 0.0
+        for i, k in enumerate(keys):
+            a = np.array(k[:3])
+            if a[d] > 0:
+                lo = a.copy()
+                lo[d] -= 1
+                gd += g[i] * a[d] * np.prod(R ** lo) * F[k[3]]
+            gd -= g[i] * 2 * p * R[d] * mono[i] * F[k[3] + 1]
+        assert e[name] == pytest.approx(gd, rel=1e-12, abs=1e-12), name
+
+    acc = [0.25] * len(keys)
+    run(body(src, "fe_dg_accum", L), dict(env, w=w, acc=acc))
+    want_acc = [0.25 + w * mono[i] * F[k[3]] for i, k in enumerate(keys)]
+    np.testing.assert_allclose(acc, want_acc, rtol=1e-13, atol=1e-14)
