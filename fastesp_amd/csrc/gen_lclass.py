@@ -203,13 +203,21 @@ def gen_L(L):
         else:
             lines.append(f"  {var} = " + (f"{var} + {g};" if mono_j == 0 else f"fma({g}, m{mono_j}, {var});"))
 
+    # The top Boys order L holds exactly the keys of degree L (t = L, j = 0), so
+    # a_L is homogeneous and Euler's identity gives it from its gradient, which
+    # the field needs anyway: a_L = (R . grad a_L) / L.  Its degree-L
+    # monomials are then never formed.
+    euler = L >= 2
     for k, (ax, ay, az, m) in enumerate(keys):
-        add(f"a{m}", f"g[{k}]", aidx[k])
+        if not (euler and m == L):
+            add(f"a{m}", f"g[{k}]", aidx[k])
         for d, al in enumerate((ax, ay, az)):
             if al > 0:
                 lo = [ax, ay, az]
                 lo[d] -= 1
                 add(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+    if euler:
+        lines.append(f"  const double a{L} = fma(Rz, dz{L}, fma(Ry, dy{L}, Rx * dx{L})) * {1.0 / L!r};")
     o.extend(lines)
     ms = sorted({k[3] for k in keys})
     # I = sum_m a_m F_m ; S1 = sum_m a_m F_{m+1}; each chain starts with a
@@ -306,13 +314,16 @@ def gen_L(L):
         else:
             lines.append(f"  {var} = " + (f"{var} + {g};" if mono_j == 0 else f"fmaf({g}, m{mono_j}, {var});"))
 
-    for k, (ax, ay, az, m) in enumerate(keys):
-        add32(f"a{m}", f"g[{k}]", aidx[k])
+    for k, (ax, ay, az, m) in enumerate(keys):  # a_L from Euler's identity, as above
+        if not (euler and m == L):
+            add32(f"a{m}", f"g[{k}]", aidx[k])
         for d, al in enumerate((ax, ay, az)):
             if al > 0:
                 lo = [ax, ay, az]
                 lo[d] -= 1
                 add32(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+    if euler:
+        lines.append(f"  const float a{L} = fmaf(Rz, dz{L}, fmaf(Ry, dy{L}, Rx * dx{L})) * {1.0 / L!r}f;")
     o.extend(lines)
     i_expr = f"(double)a{ms[0]} * F[{ms[0]}]"
     s_expr = f"(double)a{ms[0]} * F[{ms[0] + 1}]"
