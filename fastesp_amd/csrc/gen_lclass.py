@@ -151,6 +151,37 @@ def dg_plan(L):
     return best
 
 
+def value_polys(L, keys, parent, axis, aidx, T, fma):
+    """Statements forming a_m = sum over the keys of order m of g_k R^a_k.
+
+    The top order L is homogeneous of degree L and its monomials serve nothing
+    else here, so a_L = Rx Px + Ry Py + Rz Pz with each key's term in the P of
+    its parent axis, g_k R^(parent); the degree-L monomials are never formed."""
+    out, init = [], set()
+
+    def acc(var, k, mj):
+        if var not in init:
+            init.add(var)
+            out.append(f"  {T} {var} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
+        else:
+            out.append(f"  {var} = " + (f"{var} + g[{k}];" if mj == 0 else f"{fma}(g[{k}], m{mj}, {var});"))
+
+    split = L >= 2
+    for k, (ax, ay, az, m) in enumerate(keys):
+        mj = aidx[k]
+        if split and m == L:
+            acc(f"p{'xyz'[axis[mj]]}", k, parent[mj])
+        else:
+            acc(f"a{m}", k, mj)
+    if split:
+        e = None
+        for d in "xyz":
+            if f"p{d}" in init:
+                e = f"R{d} * p{d}" if e is None else f"{fma}(R{d}, p{d}, {e})"
+        out.append(f"  const {T} a{L} = {e};")
+    return out
+
+
 def gen_L(L):
     keys, acc = keys_of(L)
     order, parent, axis, aidx = schedule_of(L)
@@ -168,14 +199,8 @@ def gen_L(L):
              "const double Rx, const double Ry, const double Rz, const double* F, "
              "fe_gdptr __restrict__ g) {")
     emit_monos(o, L, order, parent, axis)
-    seen = set()
-    for k, (ax, ay, az, m) in enumerate(keys):
-        mj = aidx[k]
-        if m not in seen:
-            seen.add(m)
-            o.append(f"  double a{m} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
-        else:
-            o.append(f"  a{m} = " + (f"a{m} + g[{k}];" if mj == 0 else f"fma(g[{k}], m{mj}, a{m});"))
+    o.extend(value_polys(L, keys, parent, axis, aidx, "double", "fma"))
+    seen = {k[3] for k in keys}
     expr = f"a{L} * F[{L}]" if L in seen else "0.0"
     for m in range(L - 1, -1, -1):
         if m in seen:
@@ -281,14 +306,8 @@ def gen_L(L):
              "const float Rx, const float Ry, const float Rz, const double* F, "
              "fe_gfptr __restrict__ g) {")
     emit_monos(o, L, order, parent, axis, T="float")
-    seen = set()
-    for k, (ax, ay, az, m) in enumerate(keys):
-        mj = aidx[k]
-        if m not in seen:
-            seen.add(m)
-            o.append(f"  float a{m} = " + (f"g[{k}];" if mj == 0 else f"g[{k}] * m{mj};"))
-        else:
-            o.append(f"  a{m} = " + (f"a{m} + g[{k}];" if mj == 0 else f"fmaf(g[{k}], m{mj}, a{m});"))
+    o.extend(value_polys(L, keys, parent, axis, aidx, "float", "fmaf"))
+    seen = {k[3] for k in keys}
     top = max(seen)
     expr = f"(double)a{top} * F[{top}]"
     for m in range(top - 1, -1, -1):

@@ -6,6 +6,7 @@ form over the class's keys (a, m):
 * fe_contract<L>:       sum_k g_k R^a F_m
 * fe_contract_grad<L>:  the same, and sum_k g_k [d(R^a)/dR F_m - 2p R R^a F_{m+1}]
 * fe_dg_accum<L>:       acc_k += w R^a F_m (the product plan of the generator)
+* the FP32 forms fe_contract_f32 / fe_contract_grad_f32, evaluated in FP64
 """
 import re
 from pathlib import Path
@@ -23,7 +24,9 @@ def body(src: str, fn: str, L: int) -> list:
     out = []
     for line in src[start:end].strip().splitlines():
         s = line.strip().rstrip(";")
-        s = re.sub(r"^(const )?double ", "", s)
+        s = re.sub(r"^(const )?(double|float) ", "", s)
+        s = s.replace("fmaf(", "fma(").replace("(double)", "")
+        s = re.sub(r"(\d\.\d*)f\b", r"\1", s)  # float literals
         out.append(s)
     return out
 
@@ -76,6 +79,14 @@ def test_generated_contractions(src, L):
                 gd += g[i] * a[d] * np.prod(R ** lo) * F[k[3]]
             gd -= g[i] * 2 * p * R[d] * mono[i] * F[k[3] + 1]
         assert e[name] == pytest.approx(gd, rel=1e-12, abs=1e-12), name
+
+    # the mixed-precision forms (FP32 in the device code; the algebra is the same)
+    got32 = run(body(src, "fe_contract_f32", L), env)
+    assert got32 == pytest.approx(want, rel=1e-12, abs=1e-12)
+    e32 = run(body(src, "fe_contract_grad_f32", L), dict(env, Rxd=R[0], Ryd=R[1], Rzd=R[2]))
+    assert e32["I"] == pytest.approx(want, rel=1e-12, abs=1e-12)
+    for name in ("Gx", "Gy", "Gz"):
+        assert e32[name] == pytest.approx(e[name], rel=1e-12, abs=1e-12), name
 
     acc = [0.25] * len(keys)
     run(body(src, "fe_dg_accum", L), dict(env, w=w, acc=acc))
