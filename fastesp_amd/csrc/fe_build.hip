@@ -270,6 +270,14 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, 
   dst[e] = c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
 }
 
+// the scaled coefficients c g_k of each compacted record of class L (one lane
+// per record)
+__global__ __launch_bounds__(256) void k_fill_dcoef(double* __restrict__ rec, const int S,
+                                                    const int n, const int L) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) fe_fill_dcoef_any(L, rec + (size_t)i * S + 8);
+}
+
 // (Px, Py, Pz, p) of the records idx[0..n) of a candidate buffer, packed
 __global__ __launch_bounds__(256) void k_head(const double* __restrict__ src, const int S,
                                               const int* __restrict__ idx, const int n,
@@ -605,6 +613,9 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     const int64_t ne = (int64_t)idx.size() * S;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, cand[L], S,
                        d_idx, (int)idx.size(), t->d_rec[L]);
+    if (fe_host::NDCOEF[L])
+      hipLaunchKernelGGL(k_fill_dcoef, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0,
+                         st, t->d_rec[L], S, (int)idx.size(), L);
     (void)hipStreamSynchronize(st);
     (void)hipFree(d_idx);
     t->orig[L] = std::move(orig);

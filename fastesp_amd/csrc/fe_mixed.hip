@@ -26,11 +26,12 @@
 template <int L>
 __global__ __launch_bounds__(256) void k_make_g32(const double* __restrict__ rec, const int n,
                                                   float* __restrict__ g32) {
-  constexpr int S = FeL<L>::STRIDE, NT = FeL<L>::NT, S32 = FE_S32(NT);
+  // the g_k and the scaled coefficients that follow them in the record
+  constexpr int S = FeL<L>::STRIDE, NG = FeL<L>::NT + FeL<L>::ND, S32 = FE_S32(NG);
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)n * S32) return;
   const int pr = (int)(e / S32), k = (int)(e - (int64_t)pr * S32);
-  g32[e] = k < NT ? (float)rec[(size_t)pr * S + 8 + k] : 0.0f;
+  g32[e] = k < NG ? (float)rec[(size_t)pr * S + 8 + k] : 0.0f;
 }
 
 
@@ -38,7 +39,7 @@ __global__ __launch_bounds__(256) void k_make_g32(const double* __restrict__ rec
 int fe_ensure_g32(fe_table* t, hipStream_t st) {
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (!t->n_L[L] || t->d_rec32[L]) continue;
-    const int S32 = FE_S32(fe_host::NTERM[L]);
+    const int S32 = FE_S32(fe_host::NTERM[L] + fe_host::NDCOEF[L]);
     const size_t b = (size_t)t->n_L[L] * S32 * sizeof(float);
     if (hipMalloc(&t->d_rec32[L], b) != hipSuccess) {
       (void)hipGetLastError();

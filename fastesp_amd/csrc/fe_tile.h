@@ -30,7 +30,7 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
                                              const double* cz, const double* s_tab,
                                              const float tau, double* av, double* cv, double* ax,
                                              double* ay, double* az) {
-  constexpr int S = FeL<L>::STRIDE, S32 = FE_S32(FeL<L>::NT);
+  constexpr int S = FeL<L>::STRIDE, S32 = FE_S32(FeL<L>::NT + FeL<L>::ND);
   constexpr int M = GRAD ? L + 1 : L;
   for (int ip = p0; ip < p1; ++ip) {
     const fe_gdptr r = rec + (size_t)ip * S;

@@ -94,6 +94,27 @@ def stride_of(nt):
     return 8 + ((nt + 7) // 8) * 8
 
 
+DCOEF_LMAX = 3
+
+
+def dcoef_of(L):
+    """(key, multiplier) of every integer-scaled coefficient c g_k the field
+    contraction uses (c = a_axis >= 2 of the key's monomial).  For L <= 3 they
+    are stored in the record after the g_k, so the kernels do not rebuild them
+    per pair; the longer records of L >= 4 would cost more in scalar loads and
+    registers than the multiplies (measured: class 5 +52%, class 6 +110%), so
+    those classes keep forming c g_k in the kernel."""
+    if L > DCOEF_LMAX:
+        return []
+    keys, _ = keys_of(L)
+    out = []
+    for k, (ax, ay, az, m) in enumerate(keys):
+        for al in (ax, ay, az):
+            if al >= 2 and (k, al) not in out:
+                out.append((k, al))
+    return out
+
+
 def emit_monos(out, L, order, parent, axis, T="double", pre="m", R=("Rx", "Ry", "Rz")):
     for j, m in enumerate(order):
         if j == 0:
@@ -186,10 +207,16 @@ def gen_L(L):
     keys, acc = keys_of(L)
     order, parent, axis, aidx = schedule_of(L)
     nt, nm = len(keys), len(order)
+    dc = dcoef_of(L)
+    dci = {kc: nt + j for j, kc in enumerate(dc)}  # (key, multiplier) -> slot after the g_k
     o = []
-    o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}")
+    o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}, scaled coefficients = {len(dc)}")
     o.append(f"template <> struct FeL<{L}> {{ static constexpr int NT = {nt}, NM = {nm}, "
-             f"STRIDE = {stride_of(nt)}; }};")
+             f"ND = {len(dc)}, STRIDE = {stride_of(nt + len(dc))}; }};")
+    o.append(f"template <> __host__ __device__ inline void fe_fill_dcoef<{L}>(double* g) {{")
+    for j, (k, c) in enumerate(dc):
+        o.append(f"  g[{nt + j}] = {c}.0 * g[{k}];")
+    o.append("}")
 
     def mono(j, pre="m", one="1.0"):
         return one if j == 0 else f"{pre}{j}"
@@ -220,8 +247,13 @@ def gen_L(L):
     lines = []
     init = set()
 
-    def add(var, coef_expr, mono_j, scale=1):
-        g = coef_expr if scale == 1 else f"({scale}.0 * {coef_expr})"
+    def coef(k, scale, lit=""):
+        if scale == 1:
+            return f"g[{k}]"
+        return f"g[{dci[(k, scale)]}]" if (k, scale) in dci else f"({scale}.0{lit} * g[{k}])"
+
+    def add(var, k, mono_j, scale=1):
+        g = coef(k, scale)
         if var not in init:
             init.add(var)
             lines.append(f"  double {var} = " + (f"{g};" if mono_j == 0 else f"{g} * m{mono_j};"))
@@ -235,12 +267,12 @@ def gen_L(L):
     euler = L >= 2
     for k, (ax, ay, az, m) in enumerate(keys):
         if not (euler and m == L):
-            add(f"a{m}", f"g[{k}]", aidx[k])
+            add(f"a{m}", k, aidx[k])
         for d, al in enumerate((ax, ay, az)):
             if al > 0:
                 lo = [ax, ay, az]
                 lo[d] -= 1
-                add(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+                add(f"d{'xyz'[d]}{m}", k, midx[tuple(lo)], al)
     if euler:
         lines.append(f"  const double a{L} = fma(Rz, dz{L}, fma(Ry, dy{L}, Rx * dx{L})) * {1.0 / L!r};")
     o.extend(lines)
@@ -325,8 +357,8 @@ def gen_L(L):
     lines = []
     init = set()
 
-    def add32(var, coef_expr, mono_j, scale=1):
-        g = coef_expr if scale == 1 else f"({scale}.0f * {coef_expr})"
+    def add32(var, k, mono_j, scale=1):
+        g = coef(k, scale, "f")
         if var not in init:
             init.add(var)
             lines.append(f"  float {var} = " + (f"{g};" if mono_j == 0 else f"{g} * m{mono_j};"))
@@ -335,12 +367,12 @@ def gen_L(L):
 
     for k, (ax, ay, az, m) in enumerate(keys):  # a_L from Euler's identity, as above
         if not (euler and m == L):
-            add32(f"a{m}", f"g[{k}]", aidx[k])
+            add32(f"a{m}", k, aidx[k])
         for d, al in enumerate((ax, ay, az)):
             if al > 0:
                 lo = [ax, ay, az]
                 lo[d] -= 1
-                add32(f"d{'xyz'[d]}{m}", f"g[{k}]", midx[tuple(lo)], al)
+                add32(f"d{'xyz'[d]}{m}", k, midx[tuple(lo)], al)
     if euler:
         lines.append(f"  const float a{L} = fmaf(Rz, dz{L}, fmaf(Ry, dy{L}, Rx * dx{L})) * {1.0 / L!r}f;")
     o.extend(lines)
@@ -378,9 +410,10 @@ def main():
            "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, const double, double* __restrict__);",
            "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, fe_gfptr __restrict__);",
            "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, fe_gfptr __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
+           "template <int L> __host__ __device__ inline void fe_fill_dcoef(double*);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
-    nts, nms, koff, moff = [], [], [0], [0]
+    nts, nms, nds, koff, moff = [], [], [], [0], [0]
     allkeys, allmono = [], []
     for L in range(lmax + 1):
         o, keys, order, parent, axis, aidx = gen_L(L)
@@ -388,15 +421,27 @@ def main():
         out.append("")
         nts.append(len(keys))
         nms.append(len(order))
+        nds.append(len(dcoef_of(L)))
         for k, key in enumerate(keys):
             allkeys.append((*key, aidx[k]))
         for j in range(len(order)):
             allmono.append((*order[j], parent[j], axis[j]))
         koff.append(koff[-1] + len(keys))
         moff.append(moff[-1] + len(order))
+    out.append("// fe_fill_dcoef<L> for a class known at run time")
+    out.append("__host__ __device__ inline void fe_fill_dcoef_any(const int L, double* g) {")
+    out.append("  switch (L) {")
+    for L in range(lmax + 1):
+        out.append(f"    case {L}: fe_fill_dcoef<{L}>(g); break;")
+    out.append("    default: break;")
+    out.append("  }")
+    out.append("}")
+    out.append("")
     host.append(f"static const int NTERM[{lmax + 1}] = {{{', '.join(map(str, nts))}}};")
     host.append(f"static const int NMONO[{lmax + 1}] = {{{', '.join(map(str, nms))}}};")
-    host.append(f"static const int STRIDE[{lmax + 1}] = {{{', '.join(str(stride_of(n)) for n in nts)}}};")
+    host.append(f"static const int NDCOEF[{lmax + 1}] = {{{', '.join(map(str, nds))}}};")
+    host.append(f"static const int STRIDE[{lmax + 1}] = "
+                f"{{{', '.join(str(stride_of(n + d)) for n, d in zip(nts, nds))}}};")
     host.append(f"static const int KEY_OFF[{lmax + 2}] = {{{', '.join(map(str, koff))}}};")
     host.append(f"static const int MONO_OFF[{lmax + 2}] = {{{', '.join(map(str, moff))}}};")
     host.append("// ax, ay, az, m, aidx")

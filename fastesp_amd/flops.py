@@ -10,13 +10,14 @@ Backward (field + dL/dG), per (point, pair):
 Mixed precision: (nmono - 1) + 3 nterm of the forward (and of the backward
 contraction) are FP32, the rest FP64.
 
-Bytes per pair record of the device table: 8 * STRIDE(L) (header + g).
+Bytes per pair record of the device table: 8 * STRIDE(L) (header, g and the
+integer multiples c g the field contraction reads).
 """
 from __future__ import annotations
 
 NTERM = [1, 4, 11, 24, 46, 80, 130]
 NMONO = [1, 4, 10, 20, 35, 56, 84]
-STRIDE = [16, 16, 24, 32, 56, 88, 144]
+STRIDE = [16, 16, 24, 48, 56, 88, 144]
 
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_PACKED = 157.3e12  # MI355X FP32 vector with packed (2-wide) FMAs, FLOP/s

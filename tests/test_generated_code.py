@@ -59,7 +59,9 @@ def test_generated_contractions(src, L):
     rng = np.random.default_rng(40 + L)
     R = rng.uniform(-1.5, 1.5, 3)
     F = list(rng.uniform(0.1, 1.0, L + 2))
-    g = list(rng.normal(size=len(keys)))
+    nd = int(re.search(rf"struct FeL<{L}> {{[^}}]*ND = (\d+)", src).group(1))
+    g = list(rng.normal(size=len(keys))) + [0.0] * nd
+    run(body(src, "fe_fill_dcoef", L), dict(g=g))  # the record's scaled coefficients
     p, w = 0.7, 1.3
     env = dict(Rx=R[0], Ry=R[1], Rz=R[2], F=F, g=g, p=p)
     mono = [np.prod(R ** np.array(k[:3])) for k in keys]
@@ -92,3 +94,9 @@ def test_generated_contractions(src, L):
     run(body(src, "fe_dg_accum", L), dict(env, w=w, acc=acc))
     want_acc = [0.25 + w * mono[i] * F[k[3]] for i, k in enumerate(keys)]
     np.testing.assert_allclose(acc, want_acc, rtol=1e-13, atol=1e-14)
+
+
+def test_record_strides_match_the_flop_model(src):
+    from fastesp_amd import flops
+    m = re.search(r"static const int STRIDE\[\d+\] = \{([^}]*)\}", src)
+    assert [int(x) for x in m.group(1).split(",")] == flops.STRIDE
