@@ -108,10 +108,18 @@ __device__ __forceinline__ void fe_dg_to_dp_sp(const DPArgs& a, const int sp) {
   const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
   const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
   double acc0 = 0.0, acc1 = 0.0;  // entries e = tid and tid + 64 of the ni x nj block
-  for (int ii = 0; ii < npi; ++ii)
-    for (int jj = 0; jj < npj; ++jj) {
-      const int kept = s2k[slot0 + ii * npj + jj];
-      if (kept < 0) continue;
+  // candidate slots are read 64 at a time, one lane each, and only the kept
+  // ones are visited, in slot order (a slot-by-slot loop would wait for every
+  // slot's load in turn, and contracted shell pairs have dozens of slots)
+  const int nslot = npi * npj;
+  for (int c0 = 0; c0 < nslot; c0 += 64) {
+    const int kl = c0 + tid < nslot ? s2k[slot0 + c0 + tid] : -1;
+    uint64_t live = __ballot(kl >= 0);
+    while (live) {
+      const int lane = __builtin_ctzll(live);
+      live &= live - 1;
+      const int kept = __builtin_amdgcn_readlane(kl, lane);
+      const int ii = (c0 + lane) / npj, jj = c0 + lane - ii * npj;
       const double al = a.pexp[pi0 + ii], be = a.pexp[pj0 + jj];
       const double p = al + be;
       const double cc = a.pcoef[pi0 + ii] * a.pcoef[pj0 + jj];
@@ -151,6 +159,7 @@ __device__ __forceinline__ void fe_dg_to_dp_sp(const DPArgs& a, const int sp) {
       }
       __syncthreads();
     }
+  }
   for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
     const int mi = e / nj, mj = e - mi * nj;
     const double v = slot == 0 ? acc0 : acc1;
