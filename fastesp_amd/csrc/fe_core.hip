@@ -648,8 +648,8 @@ FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
     c = (nL + ppc - 1) / ppc;
     pl.ppc[L] = (int)ppc;
     pl.chunks[L] = (int)c;
-    pl.row0[L] = rows;
-    rows += (int)c;
+    pl.nadd[L] = rows;
+    rows = std::max(rows, (int)c);
   }
   pl.rows = rows;
   return pl;

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_eval(const doub
                                                 const int ppc, const double* __restrict__ pts,
                                                 const int n_pts,
                                                 const double* __restrict__ boys_tab,
-                                                double* __restrict__ partial, const int row0,
+                                                double* __restrict__ partial, const int nadd,
                                                 const int ld, const size_t plane,
                                                 const float tau) {
   constexpr int NTAB = fe_boys_rows(GRAD ? L + 1 : L) * FE_BOYS_COLS;
@@ -24,8 +24,8 @@ __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_eval(const doub
   const int p1 = min(n_pairs, p0 + ppc);
   const int base = blockIdx.x * PPL * FE_WG;
   fe_eval_block<L, PPL, GRAD, MIXED>((fe_gdptr)rec, (fe_gfptr)g32, p0, p1, pts, base, n_pts,
-                                     s_tab, tau, partial + (size_t)(row0 + chunk) * ld + base,
-                                     plane);
+                                     s_tab, tau, partial + (size_t)chunk * ld + base, plane,
+                                     chunk < nadd);
 }
 
 // Persistent: each workgroup takes the next item of the queue until it is
@@ -53,7 +53,8 @@ __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_batch_eval(cons
     const FeBatchItem b = items[it];
     fe_eval_block<L, PPL, GRAD, MIXED>((fe_gdptr)recs[b.conf],
                                        (fe_gfptr)(MIXED ? g32s[b.conf] : nullptr), b.p0, b.p1,
-                                       pts, b.base, b.end, s_tab, tau, partial + b.out, plane);
+                                       pts, b.base, b.end, s_tab, tau, partial + b.out, plane,
+                                       false);
   }
 }
 
@@ -70,7 +71,7 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
   const int ph = fe_prof_begin(t->ctx, FE_PH_EVAL0 + L, st);
 #define FE_K(G, X)                                                                            \
   hipLaunchKernelGGL((k_eval<L, PPL, G, X>), grid, dim3(FE_WG), 0, st, t->d_rec[L], g32,      \
-                     (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.row0[L], pl.ld,      \
+                     (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.nadd[L], pl.ld,      \
                      plane, tau)
   if (grad) {
     if (mixed) FE_K(true, true); else FE_K(true, false);

@@ -152,14 +152,19 @@ struct fe_table {
   std::vector<int64_t> h_sp_slot;
 };
 
+// Partial sums: pair chunk c of every class adds into row c.  The classes run
+// in the order 0..FE_LMAX on one stream; class L writes rows [nadd[L],
+// chunks[L]) and adds into rows [0, nadd[L]), which earlier classes wrote, so
+// each row holds its classes' sums in a fixed order and the reduction reads
+// max(chunks) rows rather than one set per class.
 struct FwdPlan {
   int ld = 0;     // leading dimension of a partial row (>= n, multiple of 1024)
-  int rows = 0;   // total partial rows
+  int rows = 0;   // partial rows (the largest chunk count of a class)
   int n_pb[FE_LMAX + 1] = {};   // point blocks of the class launch
   int ppl[FE_LMAX + 1] = {};    // points per lane of the class launch (1, 2, 4)
   int chunks[FE_LMAX + 1] = {};
   int ppc[FE_LMAX + 1] = {};
-  int row0[FE_LMAX + 1] = {};
+  int nadd[FE_LMAX + 1] = {};   // rows already holding sums when class L runs
 };
 
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);

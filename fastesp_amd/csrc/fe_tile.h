@@ -94,14 +94,15 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
 
 // Load PPL points per lane of the block starting at `base` (clamped), run the
 // tile, and write the partial sums (value plane; three more planes with GRAD)
-// for the points below `end`.
+// for the points below `end`, or add them to the sums already there (`add`).
 template <int L, int PPL, bool GRAD, bool MIXED>
 __device__ __forceinline__ void fe_eval_block(fe_gdptr __restrict__ rec,
                                               fe_gfptr __restrict__ g32, const int p0,
                                               const int p1, const double* __restrict__ pts,
                                               const int base, const int end,
                                               const double* s_tab, const float tau,
-                                              double* __restrict__ out, const size_t plane) {
+                                              double* __restrict__ out, const size_t plane,
+                                              const bool add) {
   double cx[PPL], cy[PPL], cz[PPL], av[PPL], cv[PPL], ax[PPL], ay[PPL], az[PPL];
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
@@ -116,11 +117,12 @@ __device__ __forceinline__ void fe_eval_block(fe_gdptr __restrict__ rec,
   for (int q = 0; q < PPL; ++q) {
     const int o = q * FE_WG + (int)threadIdx.x;
     if (base + o < end) {
-      out[o] = av[q] + cv[q];
+      const double v = av[q] + cv[q];
+      out[o] = add ? out[o] + v : v;
       if constexpr (GRAD) {
-        out[plane + o] = ax[q];
-        out[2 * plane + o] = ay[q];
-        out[3 * plane + o] = az[q];
+        out[plane + o] = add ? out[plane + o] + ax[q] : ax[q];
+        out[2 * plane + o] = add ? out[2 * plane + o] + ay[q] : ay[q];
+        out[3 * plane + o] = add ? out[3 * plane + o] + az[q] : az[q];
       }
     }
   }
