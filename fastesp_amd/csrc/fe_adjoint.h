@@ -36,23 +36,51 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   const double* r = rec + (size_t)ip * S;
   const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
   const double hp = r[6], KF = r[7];
+  // Per-order weights without the pair's constants.  In the far field
+  //   K F_m = C_m |R|^-(2m+1),  C_m = K sqrt(pi/p)/2 (2m-1)!! (1/(2p))^m,
+  // so the far weights are w |R|^-(2m+1) (one product per order) and the
+  // tabulated ones w K F_m / C_m; the sums are scaled by C_m once at the end.
+  // The per-lane factors 1/C_m cost L + 1 live registers, so the class whose
+  // kernel runs under a register cap (two waves per SIMD) keeps the weights
+  // w K F_m with K F_0 folded in (no final scale).
+  constexpr bool SPLIT = fe_dg_minw(L) == 1;
+  double D[L + 1];
+  if constexpr (SPLIT) {
+    double c = KF;
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+      D[m] = c != 0.0 ? K / c : 0.0;
+      c *= (2 * m + 1) * hp;
+    }
+  }
   double acc[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) acc[k] = 0.0;
-  // the weight carries K (and F_0 in the far field), so acc sums K R^a F_m gV
-  auto point = [&](const double x, const double y, const double z, double w) {
+  auto point = [&](const double x, const double y, const double z, const double w) {
     const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
     const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
-    double F[L + 1];
+    double wm[L + 1];
     if (!__any(p * r2 < fe_tfar(L))) {
       const double ri = fe_rsqrt_pos(r2);
-      fe_boys_far_scaled<L>(hp * (ri * ri), F);
-      w *= KF * ri;
+      if constexpr (SPLIT) {
+        const double u = ri * ri;
+        wm[0] = w * ri;
+#pragma unroll
+        for (int m = 1; m <= L; ++m) wm[m] = wm[m - 1] * u;
+      } else {
+        double F[L + 1];
+        fe_boys_far_scaled<L>(hp * (ri * ri), F);
+        const double wf = w * (KF * ri);
+#pragma unroll
+        for (int m = 0; m <= L; ++m) wm[m] = wf * F[m];
+      }
     } else {
+      double F[L + 1];
       fe_boys<L>(p * r2, F, s_tab);
-      w *= K;
+#pragma unroll
+      for (int m = 0; m <= L; ++m) wm[m] = SPLIT ? (w * F[m]) * D[m] : (w * K) * F[m];
     }
-    fe_dg_accum<L>(Rx, Ry, Rz, F, w, acc);
+    fe_dg_accum<L>(Rx, Ry, Rz, wm, acc);
   };
   // two wave-uniform points per iteration so their scalar loads overlap; the
   // odd tail point repeats the last point with zero weight
@@ -64,6 +92,14 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
     const double w1 = (q + 1 < q1) ? c1[3] : 0.0;
     point(x0, y0, z0, w0);
     point(x1, y1, z1, w1);
+  }
+  if constexpr (SPLIT) {
+    // C_m from the record again (not kept live through the point loop)
+    double C[L + 1];
+    C[0] = r[7];
+#pragma unroll
+    for (int m = 1; m <= L; ++m) C[m] = C[m - 1] * ((2 * m - 1) * r[6]);
+    fe_dg_scale<L>(acc, C);
   }
   if (pair < n_pairs) {
 #pragma unroll

@@ -12,7 +12,8 @@ independently (own implementation of the a-table recurrence of
 * ``fe_contract<L>``: I(R) = sum_k g_k R^alpha_k F_mk with the monomials built
   once and the terms grouped by Boys order m (one FMA per term);
 * ``fe_contract_grad<L>``: the same plus the gradient with respect to C;
-* ``fe_dg_accum<L>``: the per-term products w R^alpha F_m for dL/dG;
+* ``fe_dg_accum<L>``: the per-term products w_m R^alpha for dL/dG (and
+  ``fe_dg_scale<L>``, the per-order factors applied once per pair);
 * ``fe_contract_f32<L>``: the mixed-precision form (FP32 monomials/products).
 
 Run: ``python gen_lclass.py [LMAX]`` (default 6).  Output is deterministic.
@@ -294,7 +295,8 @@ def gen_L(L):
         o.append(f"  G{d} = {e};")
     o.append("}")
 
-    # ---------------- dL/dG accumulation: acc[k] += w * R^a_k * F_mk
+    # ---------------- dL/dG accumulation: acc[k] += w_mk * R^a_k
+    # (w_m = wm[m], the caller's per-order weights.)
     # Each term is one fma(X, Y, acc) with X Y = w_m R^a, X = w_m R^gamma a
     # weighted prefix (gamma an ancestor of a on the parent chain) and Y = R^beta,
     # beta = a - gamma, a monomial of degree min(|a|, s_m).  s_m is chosen per
@@ -302,8 +304,8 @@ def gen_L(L):
     # for every term of a high order would rebuild long monomials once per
     # order; shared monomials alone would need every monomial of the class).
     o.append(f"template <> __device__ __forceinline__ void fe_dg_accum<{L}>("
-             "const double Rx, const double Ry, const double Rz, const double* F, "
-             "const double w, double* __restrict__ acc) {")
+             "const double Rx, const double Ry, const double Rz, const double* wm, "
+             "double* __restrict__ acc) {")
     midx = {mm: i for i, mm in enumerate(order)}
     _, terms, mclos, weighted = dg_plan(L)
     def rname(j):  # a monomial by its schedule index: R_axis at degree 1
@@ -313,7 +315,7 @@ def gen_L(L):
         if sum(mm) >= 2 and mm in mclos:
             o.append(f"  const double m{j} = {rname(parent[j])} * {('Rx', 'Ry', 'Rz')[axis[j]]};")
     for m in ms:
-        o.append(f"  const double w{m} = w * F[{m}];")
+        o.append(f"  const double w{m} = wm[{m}];")
         for j, mm in enumerate(order):  # weighted prefixes in parent-before-child order
             if sum(mm) >= 1 and mm in weighted[m]:
                 src = f"w{m}" if parent[j] == 0 else f"v{m}_{parent[j]}"
@@ -331,6 +333,12 @@ def gen_L(L):
         x = wname(m, gamma)
         o.append(f"  acc[{k}] = " + (f"acc[{k}] + {x};" if sum(beta) == 0
                                      else f"fma({x}, {yname(beta)}, acc[{k}]);"))
+    o.append("}")
+    # acc[k] *= c[m_k]: the per-order factor the weights left out
+    o.append(f"template <> __device__ __forceinline__ void fe_dg_scale<{L}>("
+             "double* __restrict__ acc, const double* c) {")
+    for k, (ax, ay, az, m) in enumerate(keys):
+        o.append(f"  acc[{k}] *= c[{m}];")
     o.append("}")
 
     # ---------------- mixed: FP32 monomials and per-m polynomials, FP64 F and final sum
@@ -407,7 +415,8 @@ def main():
            "template <int L> struct FeL;",
            "template <int L> __device__ __forceinline__ double fe_contract(const double, const double, const double, const double*, fe_gdptr __restrict__);",
            "template <int L> __device__ __forceinline__ void fe_contract_grad(const double, const double, const double, const double*, fe_gdptr __restrict__, const double, double&, double&, double&, double&);",
-           "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, const double, double* __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_dg_accum(const double, const double, const double, const double*, double* __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_dg_scale(double* __restrict__, const double*);",
            "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, fe_gfptr __restrict__);",
            "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, fe_gfptr __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
            "template <int L> __host__ __device__ inline void fe_fill_dcoef(double*);",

@@ -5,7 +5,8 @@ form over the class's keys (a, m):
 
 * fe_contract<L>:       sum_k g_k R^a F_m
 * fe_contract_grad<L>:  the same, and sum_k g_k [d(R^a)/dR F_m - 2p R R^a F_{m+1}]
-* fe_dg_accum<L>:       acc_k += w R^a F_m (the product plan of the generator)
+* fe_dg_accum<L>:       acc_k += w_m R^a (the product plan of the generator)
+* fe_dg_scale<L>:       acc_k *= c_m
 * the FP32 forms fe_contract_f32 / fe_contract_grad_f32, evaluated in FP64
 """
 import re
@@ -91,9 +92,13 @@ def test_generated_contractions(src, L):
         assert e32[name] == pytest.approx(e[name], rel=1e-12, abs=1e-12), name
 
     acc = [0.25] * len(keys)
-    run(body(src, "fe_dg_accum", L), dict(env, w=w, acc=acc))
+    wm = [w * f for f in F]
+    run(body(src, "fe_dg_accum", L), dict(env, wm=wm, acc=acc))
     want_acc = [0.25 + w * mono[i] * F[k[3]] for i, k in enumerate(keys)]
     np.testing.assert_allclose(acc, want_acc, rtol=1e-13, atol=1e-14)
+    c = list(rng.uniform(0.5, 2.0, L + 1))
+    run(body(src, "fe_dg_scale", L), dict(acc=acc, c=c))
+    np.testing.assert_allclose(acc, [a * c[k[3]] for a, k in zip(want_acc, keys)], rtol=1e-13)
 
 
 def test_record_strides_match_the_flop_model(src):
