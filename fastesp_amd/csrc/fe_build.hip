@@ -278,15 +278,6 @@ __global__ __launch_bounds__(256) void k_fill_dcoef(double* __restrict__ rec, co
   if (i < n) fe_fill_dcoef_any(L, rec + (size_t)i * S + 8);
 }
 
-// (Px, Py, Pz, p) of the records idx[0..n) of a candidate buffer, packed
-__global__ __launch_bounds__(256) void k_head(const double* __restrict__ src, const int S,
-                                              const int* __restrict__ idx, const int n,
-                                              double* __restrict__ head) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= 4 * n) return;
-  head[e] = src[(size_t)idx[e >> 2] * S + (e & 3)];
-}
-
 // ---------------------------------------------------------------------------
 namespace {
 struct HostShell {
@@ -539,8 +530,12 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     }
     // original (reference-order) position of every kept record, for export:
     // candidate slot -> global candidate index via its shell pair (slots of
-    // one class are contiguous per shell pair in shell-pair order)
+    // one class are contiguous per shell pair in shell-pair order, primitives
+    // ii-major); with pair ordering on, also each record's (Px, Py, Pz, p)
+    // from the basis on the host
+    const bool order = c->pair_order && idx.size() > 1;
     std::vector<int32_t> orig(idx.size());
+    std::vector<double> head(order ? idx.size() * 4 : 0);
     {
       std::vector<int> sps;
       for (int s = 0; s < nsp; ++s)
@@ -549,9 +544,29 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
       for (size_t k = 0; k < idx.size(); ++k) {
         while (cur + 1 < sps.size() && sp_slot[sps[cur + 1]] <= idx[k]) ++cur;
         const int s = sps[cur];
-        orig[k] = (int32_t)(sp_global[s] + (idx[k] - sp_slot[s]));
+        const int64_t local = idx[k] - sp_slot[s];
+        orig[k] = (int32_t)(sp_global[s] + local);
+        if (order) {
+          const HostShell &A = sh[sp_i[s]], &B = sh[sp_j[s]];
+          const int ii = (int)(local / B.nprim), jj = (int)(local - (int64_t)ii * B.nprim);
+          const double al = pexp[A.prim0 + ii], be = pexp[B.prim0 + jj], pp = al + be;
+          for (int a = 0; a < 3; ++a) head[4 * k + a] = (al * A.c[a] + be * B.c[a]) / pp;
+          head[4 * k + 3] = pp;
+        }
       }
     }
+    if (order) {
+      const std::vector<int> ord = fe_pair_order(head.data(), 4, (int64_t)idx.size());
+      std::vector<int> idx2(idx.size());
+      std::vector<int32_t> orig2(idx.size());
+      for (size_t k = 0; k < idx.size(); ++k) {
+        idx2[k] = idx[ord[k]];
+        orig2[k] = orig[ord[k]];
+      }
+      idx.swap(idx2);
+      orig.swap(orig2);
+    }
+    for (size_t k = 0; k < idx.size(); ++k) s2k[idx[k]] = (int)k;
     t->n_L[L] = (int64_t)idx.size();
     total += (int64_t)idx.size();
     if (hipMalloc(&t->d_slot2kept[L], sizeof(int) * ncand[L]) != hipSuccess) {
@@ -559,57 +574,25 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
       free_cand();
       return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
     }
-    (void)hipMemcpy(t->d_slot2kept[L], s2k.data(), sizeof(int) * ncand[L], hipMemcpyHostToDevice);
+    (void)hipMemcpyAsync(t->d_slot2kept[L], s2k.data(), sizeof(int) * ncand[L],
+                         hipMemcpyHostToDevice, st);
     t->dev_bytes += sizeof(int) * ncand[L];
-    if (idx.empty()) continue;
+    if (idx.empty()) {
+      (void)hipStreamSynchronize(st);  // s2k is a local: the copy must finish first
+      continue;
+    }
     const int S = fe_host::STRIDE[L];
     int* d_idx = nullptr;
     if (hipMalloc(&t->d_rec[L], (size_t)idx.size() * S * sizeof(double)) != hipSuccess ||
         hipMalloc(&d_idx, sizeof(int) * idx.size()) != hipSuccess) {
       (void)hipGetLastError();
+      (void)hipStreamSynchronize(st);
       if (d_idx) (void)hipFree(d_idx);
       free_cand();
       return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: table allocation failed"));
     }
     t->dev_bytes += (size_t)idx.size() * S * sizeof(double);
-    (void)hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
-    if (c->pair_order && idx.size() > 1) {
-      // record headers of the kept candidates -> host order -> permuted gather list
-      const int nk = (int)idx.size();
-      std::vector<double> head((size_t)nk * 4);
-      double* d_head = nullptr;
-      if (hipMalloc(&d_head, head.size() * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(d_idx);
-        free_cand();
-        return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
-      }
-      hipLaunchKernelGGL(k_head, dim3((unsigned)((4 * nk + 255) / 256)), dim3(256), 0, st,
-                         cand[L], S, d_idx, nk, d_head);
-      const hipError_t he =
-          hipMemcpyAsync(head.data(), d_head, head.size() * sizeof(double), hipMemcpyDeviceToHost,
-                         st);
-      const hipError_t hs = hipStreamSynchronize(st);
-      (void)hipFree(d_head);
-      if (he != hipSuccess || hs != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(d_idx);
-        free_cand();
-        return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: record header copy failed"));
-      }
-      const std::vector<int> ord = fe_pair_order(head.data(), 4, nk);
-      std::vector<int> idx2(nk);
-      std::vector<int32_t> orig2(nk);
-      for (int k = 0; k < nk; ++k) {
-        idx2[k] = idx[ord[k]];
-        orig2[k] = orig[ord[k]];
-      }
-      idx.swap(idx2);
-      orig.swap(orig2);
-      (void)hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
-    }
-    for (size_t k = 0; k < idx.size(); ++k) s2k[idx[k]] = (int)k;
-    (void)hipMemcpy(t->d_slot2kept[L], s2k.data(), sizeof(int) * ncand[L], hipMemcpyHostToDevice);
+    (void)hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, st);
     const int64_t ne = (int64_t)idx.size() * S;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, cand[L], S,
                        d_idx, (int)idx.size(), t->d_rec[L]);

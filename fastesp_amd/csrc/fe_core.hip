@@ -335,48 +335,80 @@ std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n) {
   for (int64_t i = 0; i < n; ++i) order[i] = (int)i;
   if (n < 2) return order;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  double lpmin = INFINITY, lpmax = -INFINITY;
+  double pmin = INFINITY, pmax = -INFINITY;
   for (int64_t i = 0; i < n; ++i) {
     const double* r = rec4 + i * stride;
     for (int a = 0; a < 3; ++a) {
       lo[a] = std::min(lo[a], r[a]);
       hi[a] = std::max(hi[a], r[a]);
     }
-    const double lp = std::log(r[3]);
-    lpmin = std::min(lpmin, lp);
-    lpmax = std::max(lpmax, lp);
+    pmin = std::min(pmin, r[3]);
+    pmax = std::max(pmax, r[3]);
   }
   const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-12});
-  const double lpw = std::max(lpmax - lpmin, 1e-12);
+  // exponent bucket b = floor(16 (ln p - ln pmin) / (ln pmax - ln pmin)), clamped
+  // to 15, as the count of the 15 bucket edges p_k = pmin (pmax/pmin)^(k/16) <= p
+  const double lpmin = std::log(pmin), lpw = std::max(std::log(pmax) - lpmin, 1e-12);
+  double edge[15];
+  for (int k = 0; k < 15; ++k) edge[k] = std::exp(lpmin + lpw * (k + 1) / 16.0);
+  auto spread10 = [](uint64_t x) {  // bit b of x to bit 3b
+    x &= 0x3ff;
+    x = (x | (x << 16)) & 0x030000ffull;
+    x = (x | (x << 8)) & 0x0300f00full;
+    x = (x | (x << 4)) & 0x030c30c3ull;
+    x = (x | (x << 2)) & 0x09249249ull;
+    return x;
+  };
   std::vector<uint64_t> key(n);
   for (int64_t i = 0; i < n; ++i) {
     const double* r = rec4 + i * stride;
     uint64_t code = 0;
-    uint32_t q[3];
     for (int a = 0; a < 3; ++a)
-      q[a] = (uint32_t)std::min(1023.0, std::max(0.0, (r[a] - lo[a]) / ext * 1023.0));
-    for (int b = 0; b < 10; ++b)
-      for (int a = 0; a < 3; ++a) code |= (uint64_t)((q[a] >> b) & 1u) << (3 * b + a);
-    const uint64_t bucket =
-        (uint64_t)std::min(15.0, std::floor((std::log(r[3]) - lpmin) / lpw * 16.0));
+      code |= spread10((uint64_t)std::min(1023.0, std::max(0.0, (r[a] - lo[a]) / ext * 1023.0)))
+              << a;
+    const uint64_t bucket = (uint64_t)(std::upper_bound(edge, edge + 15, r[3]) - edge);
     key[i] = (bucket << 30) | code;
   }
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+  // stable LSD radix sort of the 34-bit keys, 12 bits a pass
+  std::vector<int> tmp(n);
+  std::vector<int64_t> cnt(1 << 12);
+  for (int shift = 0; shift < 34; shift += 12) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t i = 0; i < n; ++i) ++cnt[(key[i] >> shift) & 0xfff];
+    int64_t sum = 0;
+    for (auto& c : cnt) {
+      const int64_t v = c;
+      c = sum;
+      sum += v;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      const int o = order[i];
+      tmp[cnt[(key[o] >> shift) & 0xfff]++] = o;
+    }
+    order.swap(tmp);
+  }
   return order;
 }
 
 int fe_table_index(fe_table* t) {
-  std::vector<std::tuple<int32_t, int, int>> order;
-  order.reserve(t->n_pairs);
+  // records in `orig` order (unique indices): term counts scattered by orig,
+  // an exclusive prefix sum, and each record's offset read back
+  int32_t omax = -1;
   for (int L = 0; L <= FE_LMAX; ++L)
-    for (int64_t k = 0; k < t->n_L[L]; ++k) order.emplace_back(t->orig[L][k], L, (int)k);
-  std::sort(order.begin(), order.end());
-  std::vector<std::vector<int>> off(FE_LMAX + 1);
-  for (int L = 0; L <= FE_LMAX; ++L) off[L].resize(t->n_L[L]);
+    for (const int32_t o : t->orig[L]) omax = std::max(omax, o);
+  std::vector<int32_t> at((size_t)omax + 1, 0);
+  for (int L = 0; L <= FE_LMAX; ++L)
+    for (const int32_t o : t->orig[L]) at[o] = fe_host::NTERM[L];
   int32_t toff = 0;
-  for (auto& e : order) {
-    off[std::get<1>(e)][std::get<2>(e)] = toff;
-    toff += fe_host::NTERM[std::get<1>(e)];
+  for (auto& v : at) {
+    const int32_t n = v;
+    v = toff;
+    toff += n;
+  }
+  std::vector<std::vector<int>> off(FE_LMAX + 1);
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    off[L].resize(t->n_L[L]);
+    for (int64_t k = 0; k < t->n_L[L]; ++k) off[L][k] = at[t->orig[L][k]];
   }
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (!t->n_L[L]) continue;
