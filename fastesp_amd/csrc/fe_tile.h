@@ -63,15 +63,10 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
       } else {
         fe_boys<M>(p * r2[q], F, s_tab);
       }
-      if constexpr (GRAD && L == 0 && !MIXED) {
-        // class 0: V += w g F_0, field += -2p g w F_1 R, the weight folded in
-        // before the products (three fewer per point than the general form)
-        const double g0 = r[8];
-        av[q] = fma(g0, w * F[0], av[q]);
-        const double s1 = (-2.0 * p * g0) * (w * F[1]);
-        ax[q] = fma(s1, Rx[q], ax[q]);
-        ay[q] = fma(s1, Ry[q], ay[q]);
-        az[q] = fma(s1, Rz[q], az[q]);
+      if constexpr (GRAD && L <= 2 && !MIXED) {
+        // the weight folded into F before the contraction: fewer operations
+        // than weighting its four results for the light classes
+        fe_contract_grad_acc<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, w, av[q], ax[q], ay[q], az[q]);
       } else if constexpr (GRAD) {
         double I, Gx, Gy, Gz;
         if (fp64)

@@ -294,6 +294,35 @@ def gen_L(L):
                 e = f"fma(d{d}{m}, F[{m}], {e})"
         o.append(f"  G{d} = {e};")
     o.append("}")
+    # the same, accumulated: av += w I, a{x,y,z} += w G with w folded into
+    # F first (wF_m = w F_m): L + 2 products instead of the 4 weighted
+    # accumulations and the separate results, fewer operations for L <= 2
+    o.append(f"template <> __device__ __forceinline__ void fe_contract_grad_acc<{L}>("
+             "const double Rx, const double Ry, const double Rz, const double* F, "
+             "fe_gdptr __restrict__ g, const double p, const double w, double& av, double& ax, "
+             "double& ay, double& az) {")
+    emit_monos(o, L, order, parent, axis)
+    o.extend(lines)
+    for m in range(L + 2):
+        o.append(f"  const double wF{m} = w * F[{m}];")
+    e = "av"
+    for m in ms:
+        e = f"fma(a{m}, wF{m}, {e})"
+    o.append(f"  av = {e};")
+    if L == 0:  # a_0 = g_0 is wave-uniform: -2p g_0 is one product per pair
+        o.append("  const double s1 = (-2.0 * p * a0) * wF1;")
+    else:
+        s_expr = f"a{ms[0]} * wF{ms[0] + 1}"
+        for m in ms[1:]:
+            s_expr = f"fma(a{m}, wF{m + 1}, {s_expr})"
+        o.append(f"  const double s1 = -2.0 * p * ({s_expr});")
+    for d in "xyz":
+        e = f"fma(s1, R{d}, a{d})"
+        for m in ms:
+            if f"d{d}{m}" in init:
+                e = f"fma(d{d}{m}, wF{m}, {e})"
+        o.append(f"  a{d} = {e};")
+    o.append("}")
 
     # ---------------- dL/dG accumulation: acc[k] += w_mk * R^a_k
     # (w_m = wm[m], the caller's per-order weights.)
@@ -420,6 +449,7 @@ def main():
            "template <int L> __device__ __forceinline__ double fe_contract_f32(const float, const float, const float, const double*, fe_gfptr __restrict__);",
            "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, fe_gfptr __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
            "template <int L> __host__ __device__ inline void fe_fill_dcoef(double*);",
+           "template <int L> __device__ __forceinline__ void fe_contract_grad_acc(const double, const double, const double, const double*, fe_gdptr __restrict__, const double, const double, double&, double&, double&, double&);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, nds, koff, moff = [], [], [], [0], [0]

@@ -5,6 +5,7 @@ form over the class's keys (a, m):
 
 * fe_contract<L>:       sum_k g_k R^a F_m
 * fe_contract_grad<L>:  the same, and sum_k g_k [d(R^a)/dR F_m - 2p R R^a F_{m+1}]
+* fe_contract_grad_acc<L>: both, weighted and added to running sums
 * fe_dg_accum<L>:       acc_k += w_m R^a (the product plan of the generator)
 * fe_dg_scale<L>:       acc_k *= c_m
 * the FP32 forms fe_contract_f32 / fe_contract_grad_f32, evaluated in FP64
@@ -82,6 +83,13 @@ def test_generated_contractions(src, L):
                 gd += g[i] * a[d] * np.prod(R ** lo) * F[k[3]]
             gd -= g[i] * 2 * p * R[d] * mono[i] * F[k[3] + 1]
         assert e[name] == pytest.approx(gd, rel=1e-12, abs=1e-12), name
+
+    acc4 = dict(av=0.5, ax=-0.25, ay=0.125, az=1.0)
+    e4 = run(body(src, "fe_contract_grad_acc", L), dict(env, w=w, **acc4))
+    assert e4["av"] == pytest.approx(acc4["av"] + w * want, rel=1e-12, abs=1e-12)
+    for name in ("x", "y", "z"):
+        assert e4["a" + name] == pytest.approx(acc4["a" + name] + w * e["G" + name],
+                                               rel=1e-12, abs=1e-12), name
 
     # the mixed-precision forms (FP32 in the device code; the algebra is the same)
     got32 = run(body(src, "fe_contract_f32", L), env)
