@@ -24,6 +24,11 @@
 #ifndef FE_DG_MINW_L
 #define FE_DG_MINW_L 1, 1, 1, 1, 1, 2, 1
 #endif
+// Highest class whose value + field evaluation folds the weight into F before
+// the contraction (fe_contract_grad_acc) instead of weighting its four results.
+#ifndef FE_GRAD_ACC_LMAX
+#define FE_GRAD_ACC_LMAX 2
+#endif
 __host__ __device__ constexpr int fe_eval_minw(int L, int ppl) {
   constexpr int t[7] = {FE_EVAL_MINW_L};
   return ppl == 2 ? t[L] : 1;  // the table is for 2 points per lane

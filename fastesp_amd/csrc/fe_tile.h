@@ -63,7 +63,7 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
       } else {
         fe_boys<M>(p * r2[q], F, s_tab);
       }
-      if constexpr (GRAD && L <= 2 && !MIXED) {
+      if constexpr (GRAD && L <= FE_GRAD_ACC_LMAX && !MIXED) {
         // the weight folded into F before the contraction: fewer operations
         // than weighting its four results for the light classes
         fe_contract_grad_acc<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, w, av[q], ax[q], ay[q], az[q]);
