@@ -11,7 +11,9 @@
 #define FE_LMAX 6
 #endif
 
+#ifndef FE_WG
 #define FE_WG 256  // threads per workgroup of the evaluation kernels (4 waves)
+#endif
 
 // Minimum resident waves per SIMD requested from the compiler for the
 // evaluation (k_eval, k_batch_eval; 2 points per lane) and dL/dG (k_dG,
