@@ -12,7 +12,9 @@
 //           (three lanes, one axis each), etab over the |t| <= L simplex (lanes
 //           over t), G = (2 pi/p)(2p)^m M0 etab (lanes over terms), the
 //           screening bound (wave reduction), reff and the sum-rule partial.
-//  kernel : k_screen, one lane per candidate, points streamed through LDS:
+//  kernel : k_screen, one lane per candidate, points streamed through LDS
+//           (k_screen_part + k_screen_decide split the points over blocks
+//           when there are few candidates per point):
 //           rule 0 (NumPy, table.py:190-194) keep = !(K bound/max(dmin,0.05) < tol)
 //           rule 1 (C++, build_pairs.cpp:318-328) keep = !(K bound/max(dmin-reff,0.05) < tol)
 //           rule 2 (ESPEngine, esp.py:265-270)    keep = !(K bound/max(dmax-reff,0.05) < tol)
@@ -206,22 +208,14 @@ __global__ __launch_bounds__(64) void k_build_pairs(const BuildArgs a) {
   if (tid == 0) a.sp_srule[sp] = srule;
 }
 
-// One lane per candidate; dmin over all points with the points staged in LDS.
-__global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, const int n_cand,
-                                                const int S, const double* __restrict__ pts,
-                                                const int n_pts, const double tol, const int rule,
-                                                int* __restrict__ keep) {
-  __shared__ double s_p[3 * 1024];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int ii = min(i, max(n_cand - 1, 0));
-  const double* r = rec + (size_t)ii * S;
-  const double Px = r[0], Py = r[1], Pz = r[2];
-  // rule 2 screens with the largest distance to any point, the others with
-  // the smallest (the block-uniform rule picks one reduction)
-  const bool use_max = rule == 2;
+// Smallest (rule 0, 1) or largest (rule 2) |C - P|^2 over the points
+// [q0, q1), staged through `s_p` 1024 at a time by the whole block.
+__device__ __forceinline__ double fe_screen_d2(const double Px, const double Py, const double Pz,
+                                               const double* __restrict__ pts, const int q0,
+                                               const int q1, const bool use_max, double* s_p) {
   double d2sel = use_max ? -INFINITY : INFINITY;
-  for (int base = 0; base < n_pts; base += 1024) {
-    const int m = min(1024, n_pts - base);
+  for (int base = q0; base < q1; base += 1024) {
+    const int m = min(1024, q1 - base);
     __syncthreads();
     for (int q = threadIdx.x; q < 3 * m; q += 256) s_p[q] = pts[3 * (size_t)base + q];
     __syncthreads();
@@ -237,7 +231,15 @@ __global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, 
       }
     }
   }
-  if (i >= n_cand) return;
+  return d2sel;
+}
+
+// The screen's decision for candidate record r given the selected squared
+// distance: 1 kept, 0 a candidate the screen dropped, -1 no candidate
+// (K < 1e-30 or an empty density block: not in the reference's primitive-pair
+// list).
+__device__ __forceinline__ int fe_screen_keep(const double* __restrict__ r, const double d2sel,
+                                              const double tol, const int rule) {
   const int flags = (int)r[7];
   int k = flags & 1;
   if (k) {
@@ -252,9 +254,57 @@ __global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, 
     if (dd < 0.05) dd = 0.05;
     if (kb / dd < tol) k = 0;
   }
-  // 1 kept, 0 a candidate the screen dropped, -1 no candidate (K < 1e-30 or an
-  // empty density block: not in the reference's primitive-pair list)
-  keep[i] = (flags & 1) ? k : -1;
+  return (flags & 1) ? k : -1;
+}
+
+// One lane per candidate, every point (small point sets).
+__global__ __launch_bounds__(256) void k_screen(const double* __restrict__ rec, const int n_cand,
+                                                const int S, const double* __restrict__ pts,
+                                                const int n_pts, const double tol, const int rule,
+                                                int* __restrict__ keep) {
+  __shared__ double s_p[3 * 1024];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const double* r = rec + (size_t)min(i, max(n_cand - 1, 0)) * S;
+  const double d2sel = fe_screen_d2(r[0], r[1], r[2], pts, 0, n_pts, rule == 2, s_p);
+  if (i < n_cand) keep[i] = fe_screen_keep(r, d2sel, tol, rule);
+}
+
+// One lane per candidate over the point slice blockIdx.y: the slice's
+// selected squared distance is combined into d2[i] by an integer atomic min
+// (max for rule 2) on its bit pattern, which orders like the value for
+// non-negative doubles.  d2 starts at +inf (0 for rule 2).  Splitting the
+// points over blocks fills the device for the classes with few candidates.
+__global__ __launch_bounds__(256) void k_screen_part(const double* __restrict__ rec,
+                                                     const int n_cand, const int S,
+                                                     const double* __restrict__ pts,
+                                                     const int n_pts, const int slice,
+                                                     const int rule,
+                                                     unsigned long long* __restrict__ d2) {
+  __shared__ double s_p[3 * 1024];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const double* r = rec + (size_t)min(i, max(n_cand - 1, 0)) * S;
+  const int q0 = blockIdx.y * slice, q1 = min(n_pts, q0 + slice);
+  const double v = fe_screen_d2(r[0], r[1], r[2], pts, q0, q1, rule == 2, s_p);
+  if (i >= n_cand || q0 >= q1) return;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  if (rule == 2) atomicMax(d2 + i, b);
+  else atomicMin(d2 + i, b);
+}
+
+__global__ __launch_bounds__(256) void k_screen_decide(const double* __restrict__ rec,
+                                                       const int n_cand, const int S,
+                                                       const unsigned long long* __restrict__ d2,
+                                                       const double tol, const int rule,
+                                                       int* __restrict__ keep) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_cand) return;
+  keep[i] = fe_screen_keep(rec + (size_t)i * S, __longlong_as_double((long long)d2[i]), tol, rule);
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict__ a, const int n,
+                                                  const unsigned long long v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ src, const int S,
@@ -451,12 +501,14 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   double* cand[FE_LMAX + 1] = {};
   int* keep[FE_LMAX + 1] = {};
   double* d_srule = nullptr;
+  unsigned long long* d_d2 = nullptr;
   auto free_cand = [&]() {
     for (int L = 0; L <= FE_LMAX; ++L) {
       if (cand[L]) (void)hipFree(cand[L]);
       if (keep[L]) (void)hipFree(keep[L]);
     }
     if (d_srule) (void)hipFree(d_srule);
+    if (d_d2) (void)hipFree(d_d2);
     free_tmp();
   };
   for (int L = 0; L <= FE_LMAX; ++L) {
@@ -469,7 +521,12 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
       return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: candidate allocation failed"));
     }
   }
-  if (hipMalloc(&d_srule, (size_t)std::max(nsp, 1) * sizeof(double)) != hipSuccess) {
+  // per-candidate selected squared distances of the point-split screen
+  int64_t cand_off[FE_LMAX + 2] = {};
+  for (int L = 0; L <= FE_LMAX; ++L) cand_off[L + 1] = cand_off[L] + ncand[L];
+  if (hipMalloc(&d_d2, (size_t)std::max<int64_t>(cand_off[FE_LMAX + 1], 1) *
+                           sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&d_srule, (size_t)std::max(nsp, 1) * sizeof(double)) != hipSuccess) {
     (void)hipGetLastError();
     free_cand();
     return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
@@ -487,9 +544,23 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   if (nsp > 0) hipLaunchKernelGGL(k_build_pairs, dim3(nsp), dim3(64), 0, st, ba);
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (!ncand[L]) continue;
-    hipLaunchKernelGGL(k_screen, dim3((unsigned)((ncand[L] + 255) / 256)), dim3(256), 0, st,
-                       cand[L], (int)ncand[L], fe_host::STRIDE[L], d_pts, n_points, tol, rule,
-                       keep[L]);
+    const unsigned nb = (unsigned)((ncand[L] + 255) / 256);
+    // point slices of at least 1024 points, enough for about 2048 blocks
+    const int max_split = (n_points + 1023) / 1024;
+    const int ns = std::min<int>(max_split, (int)std::max<unsigned>(1, (2048 + nb - 1) / nb));
+    if (ns < 2) {
+      hipLaunchKernelGGL(k_screen, dim3(nb), dim3(256), 0, st, cand[L], (int)ncand[L],
+                         fe_host::STRIDE[L], d_pts, n_points, tol, rule, keep[L]);
+      continue;
+    }
+    const int slice = (n_points + ns - 1) / ns;
+    unsigned long long* d2 = d_d2 + cand_off[L];
+    hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, st, d2, (int)ncand[L],
+                       rule == 2 ? 0ull : 0x7FF0000000000000ull);
+    hipLaunchKernelGGL(k_screen_part, dim3(nb, (unsigned)ns), dim3(256), 0, st, cand[L],
+                       (int)ncand[L], fe_host::STRIDE[L], d_pts, n_points, slice, rule, d2);
+    hipLaunchKernelGGL(k_screen_decide, dim3(nb), dim3(256), 0, st, cand[L], (int)ncand[L],
+                       fe_host::STRIDE[L], d2, tol, rule, keep[L]);
   }
   if (hipGetLastError() != hipSuccess) {
     free_cand();

@@ -78,6 +78,34 @@ def test_build_cfg1_300_matches_golden():
     assert t.stats()["sum_rule"] == pytest.approx(float(g["sum_rule"]), abs=1e-9)
 
 
+@pytest.mark.parametrize("rule,tol", [("numpy", 1e-12), ("numpy", 1e-8), ("engine", 1e-7)])
+def test_point_split_screen_matches_oracle(rule, tol):
+    """More than 1024 points and few candidates: the screen splits the points
+    over blocks and combines the per-slice minima (maxima for the engine rule)."""
+    from fastesp_amd.builder import SCREEN_ENGINE
+    w = synth.config0(n_points=5000)
+    say(f"cfg0 5000 points build rule={rule} tol={tol}")
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=tol,
+                        rule=SCREEN_ENGINE if rule == "engine" else 0)
+    tab = engine.build_table(w.wfn, w.basis, w.density, w.points, tol, rule=rule)
+    assert 0 < tab.n_pairs
+    _compare_tables(export_flat(t), tab)
+
+
+def test_point_split_screen_cpp_rule_matches_port():
+    from oracle.cpu_port import CpuTable
+    w = synth.config0(n_points=5000)
+    say("cfg0 5000 points build rule=cpp tol=1e-9")
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-9, rule=SCREEN_CPP)
+    ref = CpuTable(w.wfn, w.basis, w.density, w.points, tol=1e-9, rule=1)
+    p6, _ = ref.export()
+    flat = export_flat(t)
+    assert flat.n_pairs == ref.n_pairs
+    mine = np.stack([flat.pp_exp, flat.pp_cx, flat.pp_cy, flat.pp_cz], 1)
+    order = lambda a: a[np.lexsort(a.T[::-1])]  # noqa: E731
+    np.testing.assert_allclose(order(mine), order(p6[:, :4]), rtol=1e-13, atol=1e-13)
+
+
 @pytest.mark.skipif(not REF_BUILDER.exists(), reason="reference builder not compiled")
 def test_cpp_screen_rule_matches_reference_builder(tmp_path):
     """Rule 1 reproduces build_pairs' kept set at a tolerance that drops pairs."""
