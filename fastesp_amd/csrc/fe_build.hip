@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "fe_boys.h"
@@ -461,34 +462,53 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   t->b.sp_i = (const int*)(bm + o_spi); t->b.sp_j = (const int*)(bm + o_spj);
   t->b.sp_slot = (const int*)(bm + o_sps);
 
+  // ---- scratch in the context's build workspace (grown when needed and kept
+  // for the next build, so a build allocates only its table): density and
+  // points of host inputs, candidates, keep flags, screen distances, sum-rule
+  // partials and gather indices; the keep flags come back through pinned
+  // host memory, so their copies do not hold up the launches
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  int64_t cand_off[FE_LMAX + 2] = {};
+  for (int L = 0; L <= FE_LMAX; ++L) cand_off[L + 1] = cand_off[L] + ncand[L];
+  const int64_t ncand_all = cand_off[FE_LMAX + 1];
+  const bool host_in = ptr_kind == FE_PTR_HOST;
+  size_t wo = 0;
+  const size_t o_dens = wo;
+  if (host_in) wo += al((size_t)n_func * n_func * sizeof(double));
+  const size_t o_pts = wo;
+  if (host_in) wo += al((size_t)n_points * 3 * sizeof(double));
+  size_t o_cand[FE_LMAX + 1], o_keep[FE_LMAX + 1], o_idx[FE_LMAX + 1];
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    o_cand[L] = wo;
+    wo += al((size_t)ncand[L] * fe_host::STRIDE[L] * sizeof(double));
+    o_keep[L] = wo;
+    wo += al((size_t)ncand[L] * sizeof(int));
+    o_idx[L] = wo;
+    wo += al((size_t)ncand[L] * sizeof(int));
+  }
+  const size_t o_d2 = wo;
+  wo += al((size_t)std::max<int64_t>(ncand_all, 1) * sizeof(unsigned long long));
+  const size_t o_srule = wo;
+  wo += al((size_t)std::max(nsp, 1) * sizeof(double));
+  if (fe_ctx_reserve(c, FE_WS_BUILD, wo) != FE_OK ||
+      fe_ctx_reserve_pinned(c, (size_t)std::max<int64_t>(ncand_all, 1) * sizeof(int)) != FE_OK)
+    return fail(FE_ERR_ALLOC);
+  char* wb = (char*)c->ws[FE_WS_BUILD];
+  int* h_keep = (int*)c->h_pin;
+
   // ---- density and points
   const double* d_dens = dens;
   const double* d_pts = pts;
-  double* tmp_dens = nullptr;
-  double* tmp_pts = nullptr;
-  if (ptr_kind == FE_PTR_HOST) {
-    const size_t db = (size_t)n_func * n_func * sizeof(double);
-    if (hipMalloc(&tmp_dens, db) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: density allocation failed"));
-    }
-    (void)hipMemcpyAsync(tmp_dens, dens, db, hipMemcpyHostToDevice, st);
-    d_dens = tmp_dens;
+  if (host_in) {
+    (void)hipMemcpyAsync(wb + o_dens, dens, (size_t)n_func * n_func * sizeof(double),
+                         hipMemcpyHostToDevice, st);
+    d_dens = (const double*)(wb + o_dens);
     if (n_points) {
-      if (hipMalloc(&tmp_pts, (size_t)n_points * 3 * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipFree(tmp_dens);
-        return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: points allocation failed"));
-      }
-      (void)hipMemcpyAsync(tmp_pts, pts, (size_t)n_points * 3 * sizeof(double),
+      (void)hipMemcpyAsync(wb + o_pts, pts, (size_t)n_points * 3 * sizeof(double),
                            hipMemcpyHostToDevice, st);
-      d_pts = tmp_pts;
+      d_pts = (const double*)(wb + o_pts);
     }
   }
-  auto free_tmp = [&]() {
-    if (tmp_dens) (void)hipFree(tmp_dens);
-    if (tmp_pts) (void)hipFree(tmp_pts);
-  };
 
   // ---- candidates
   static bool consts_done[64] = {};
@@ -500,36 +520,25 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   }
   double* cand[FE_LMAX + 1] = {};
   int* keep[FE_LMAX + 1] = {};
-  double* d_srule = nullptr;
-  unsigned long long* d_d2 = nullptr;
+  double* d_srule = (double*)(wb + o_srule);
+  unsigned long long* d_d2 = (unsigned long long*)(wb + o_d2);
+  hipEvent_t ev[FE_LMAX + 1] = {};  // class L's keep flags are on the host
+  // every exit after the first launch has synchronised the stream first
   auto free_cand = [&]() {
-    for (int L = 0; L <= FE_LMAX; ++L) {
-      if (cand[L]) (void)hipFree(cand[L]);
-      if (keep[L]) (void)hipFree(keep[L]);
-    }
-    if (d_srule) (void)hipFree(d_srule);
-    if (d_d2) (void)hipFree(d_d2);
-    free_tmp();
+    for (int L = 0; L <= FE_LMAX; ++L)
+      if (ev[L]) (void)hipEventDestroy(ev[L]);
   };
   for (int L = 0; L <= FE_LMAX; ++L) {
     t->n_cand_L[L] = ncand[L];
+    cand[L] = (double*)(wb + o_cand[L]);
+    keep[L] = (int*)(wb + o_keep[L]);
     if (!ncand[L]) continue;
-    if (hipMalloc(&cand[L], (size_t)ncand[L] * fe_host::STRIDE[L] * sizeof(double)) != hipSuccess ||
-        hipMalloc(&keep[L], (size_t)ncand[L] * sizeof(int)) != hipSuccess) {
+    if (hipEventCreateWithFlags(&ev[L], hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
+      (void)hipStreamSynchronize(st);
       free_cand();
-      return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: candidate allocation failed"));
+      return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: event creation failed"));
     }
-  }
-  // per-candidate selected squared distances of the point-split screen
-  int64_t cand_off[FE_LMAX + 2] = {};
-  for (int L = 0; L <= FE_LMAX; ++L) cand_off[L + 1] = cand_off[L] + ncand[L];
-  if (hipMalloc(&d_d2, (size_t)std::max<int64_t>(cand_off[FE_LMAX + 1], 1) *
-                           sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&d_srule, (size_t)std::max(nsp, 1) * sizeof(double)) != hipSuccess) {
-    (void)hipGetLastError();
-    free_cand();
-    return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
   }
   BuildArgs ba{};
   ba.sh_atom = t->b.atom; ba.sh_l = t->b.l; ba.sh_fn0 = t->b.fn0; ba.sh_nfn = t->b.nfn;
@@ -542,6 +551,9 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   ba.sp_srule = d_srule;
   ba.lt = c->ltab;
   if (nsp > 0) hipLaunchKernelGGL(k_build_pairs, dim3(nsp), dim3(64), 0, st, ba);
+  // each class's keep flags go to the host as soon as its screen ends
+  int* hk[FE_LMAX + 1];
+  for (int L = 0; L <= FE_LMAX; ++L) hk[L] = h_keep + cand_off[L];
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (!ncand[L]) continue;
     const unsigned nb = (unsigned)((ncand[L] + 255) / 256);
@@ -551,53 +563,47 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     if (ns < 2) {
       hipLaunchKernelGGL(k_screen, dim3(nb), dim3(256), 0, st, cand[L], (int)ncand[L],
                          fe_host::STRIDE[L], d_pts, n_points, tol, rule, keep[L]);
-      continue;
+    } else {
+      const int slice = (n_points + ns - 1) / ns;
+      unsigned long long* d2 = d_d2 + cand_off[L];
+      hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, st, d2, (int)ncand[L],
+                         rule == 2 ? 0ull : 0x7FF0000000000000ull);
+      hipLaunchKernelGGL(k_screen_part, dim3(nb, (unsigned)ns), dim3(256), 0, st, cand[L],
+                         (int)ncand[L], fe_host::STRIDE[L], d_pts, n_points, slice, rule, d2);
+      hipLaunchKernelGGL(k_screen_decide, dim3(nb), dim3(256), 0, st, cand[L], (int)ncand[L],
+                         fe_host::STRIDE[L], d2, tol, rule, keep[L]);
     }
-    const int slice = (n_points + ns - 1) / ns;
-    unsigned long long* d2 = d_d2 + cand_off[L];
-    hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, st, d2, (int)ncand[L],
-                       rule == 2 ? 0ull : 0x7FF0000000000000ull);
-    hipLaunchKernelGGL(k_screen_part, dim3(nb, (unsigned)ns), dim3(256), 0, st, cand[L],
-                       (int)ncand[L], fe_host::STRIDE[L], d_pts, n_points, slice, rule, d2);
-    hipLaunchKernelGGL(k_screen_decide, dim3(nb), dim3(256), 0, st, cand[L], (int)ncand[L],
-                       fe_host::STRIDE[L], d2, tol, rule, keep[L]);
+    (void)hipMemcpyAsync(hk[L], keep[L], sizeof(int) * ncand[L], hipMemcpyDeviceToHost, st);
+    (void)hipEventRecord(ev[L], st);
   }
   if (hipGetLastError() != hipSuccess) {
+    (void)hipStreamSynchronize(st);
     free_cand();
     return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: kernel launch failed"));
   }
-  // ---- compaction in candidate order (host prefix over the keep flags)
   std::vector<double> srule(std::max(nsp, 1));
-  if (hipMemcpyAsync(srule.data(), d_srule, sizeof(double) * std::max(nsp, 1),
-                     hipMemcpyDeviceToHost, st) != hipSuccess) {
-    free_cand();
-    return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: readback failed"));
-  }
-  std::vector<std::vector<int>> hk(FE_LMAX + 1);
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    if (!ncand[L]) continue;
-    hk[L].resize(ncand[L]);
-    (void)hipMemcpyAsync(hk[L].data(), keep[L], sizeof(int) * ncand[L], hipMemcpyDeviceToHost, st);
-  }
-  if (hipStreamSynchronize(st) != hipSuccess) {
-    free_cand();
-    return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: build kernels failed: %s",
-                             hipGetErrorString(hipGetLastError())));
-  }
-  double sr = 0.0;
-  for (int s = 0; s < nsp; ++s) sr += srule[s];
-  t->sum_rule = sr;
-  int64_t total = 0;
+  (void)hipMemcpyAsync(srule.data(), d_srule, sizeof(double) * std::max(nsp, 1),
+                       hipMemcpyDeviceToHost, st);
+  // ---- compaction in candidate order, one host thread per class, each
+  // started when its keep flags have arrived (the later classes are still
+  // being screened); the device copies and gathers follow in one pass
   // global candidate order: shell pairs in order, primitives ii-major
   std::vector<int64_t> sp_global(nsp + 1, 0);
   for (int s = 0; s < nsp; ++s)
     sp_global[s + 1] = sp_global[s] + (int64_t)sh[sp_i[s]].nprim * sh[sp_j[s]].nprim;
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    if (!ncand[L]) continue;
-    std::vector<int> idx, s2k(ncand[L], -1);
+  struct ClassOut {
+    std::vector<int> idx, s2k;
+    std::vector<int32_t> orig;
+    int64_t n_cand = 0;
+  };
+  std::vector<ClassOut> co(FE_LMAX + 1);
+  auto compact = [&](const int L) {
+    std::vector<int>& idx = co[L].idx;
+    std::vector<int>& s2k = co[L].s2k;
+    s2k.assign(ncand[L], -1);
     for (int64_t q = 0; q < ncand[L]; ++q) {
       if (hk[L][q] > 0) idx.push_back((int)q);
-      if (hk[L][q] >= 0) ++t->n_candidates;
+      if (hk[L][q] >= 0) ++co[L].n_cand;
     }
     // original (reference-order) position of every kept record, for export:
     // candidate slot -> global candidate index via its shell pair (slots of
@@ -638,27 +644,56 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
       orig.swap(orig2);
     }
     for (size_t k = 0; k < idx.size(); ++k) s2k[idx[k]] = (int)k;
+    co[L].orig = std::move(orig);
+  };
+  std::vector<std::thread> workers;
+  bool screened = true;
+  for (int L = 0; L <= FE_LMAX && screened; ++L) {
+    if (!ncand[L]) continue;
+    if (hipEventSynchronize(ev[L]) != hipSuccess) {
+      screened = false;
+      break;
+    }
+    try {
+      workers.emplace_back(compact, L);
+    } catch (...) {
+      compact(L);  // no thread available: compact on this one
+    }
+  }
+  for (auto& w : workers) w.join();
+  if (!screened || hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    free_cand();
+    return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: build kernels failed: %s",
+                             hipGetErrorString(hipGetLastError())));
+  }
+  double sr = 0.0;
+  for (int s = 0; s < nsp; ++s) sr += srule[s];
+  t->sum_rule = sr;
+  int64_t total = 0;
+  // the copies below read the host vectors of co: synchronise before leaving
+  auto free_idx = [&]() { (void)hipStreamSynchronize(st); };
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    if (!ncand[L]) continue;
+    const std::vector<int>& idx = co[L].idx;
+    t->n_candidates += co[L].n_cand;
     t->n_L[L] = (int64_t)idx.size();
     total += (int64_t)idx.size();
     if (hipMalloc(&t->d_slot2kept[L], sizeof(int) * ncand[L]) != hipSuccess) {
       (void)hipGetLastError();
+      free_idx();
       free_cand();
       return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: allocation failed"));
     }
-    (void)hipMemcpyAsync(t->d_slot2kept[L], s2k.data(), sizeof(int) * ncand[L],
+    (void)hipMemcpyAsync(t->d_slot2kept[L], co[L].s2k.data(), sizeof(int) * ncand[L],
                          hipMemcpyHostToDevice, st);
     t->dev_bytes += sizeof(int) * ncand[L];
-    if (idx.empty()) {
-      (void)hipStreamSynchronize(st);  // s2k is a local: the copy must finish first
-      continue;
-    }
+    if (idx.empty()) continue;
     const int S = fe_host::STRIDE[L];
-    int* d_idx = nullptr;
-    if (hipMalloc(&t->d_rec[L], (size_t)idx.size() * S * sizeof(double)) != hipSuccess ||
-        hipMalloc(&d_idx, sizeof(int) * idx.size()) != hipSuccess) {
+    int* d_idx = (int*)(wb + o_idx[L]);
+    if (hipMalloc(&t->d_rec[L], (size_t)idx.size() * S * sizeof(double)) != hipSuccess) {
       (void)hipGetLastError();
-      (void)hipStreamSynchronize(st);
-      if (d_idx) (void)hipFree(d_idx);
+      free_idx();
       free_cand();
       return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: table allocation failed"));
     }
@@ -670,10 +705,9 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     if (fe_host::NDCOEF[L])
       hipLaunchKernelGGL(k_fill_dcoef, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0,
                          st, t->d_rec[L], S, (int)idx.size(), L);
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(d_idx);
-    t->orig[L] = std::move(orig);
   }
+  free_idx();
+  for (int L = 0; L <= FE_LMAX; ++L) t->orig[L] = std::move(co[L].orig);
   t->n_pairs = (int)total;
   int64_t nterms = 0, nvalid = 0;
   for (int L = 0; L <= FE_LMAX; ++L) nterms += t->n_L[L] * fe_host::NTERM[L];

@@ -41,6 +41,7 @@ __host__ __device__ constexpr int fe_dg_minw(int L) {
 }
 
 int fe_set_error(int code, const char* fmt, ...);
+int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes);
 
 #define FE_HIPCHK(x)                                                                         \
   do {                                                                                       \
@@ -62,6 +63,7 @@ enum {
   FE_WS_AUX2,
   FE_WS_AUX3,
   FE_WS_SORT,
+  FE_WS_BUILD,  // scratch of fe_table_build (kept for the next build)
   FE_WS_SLOTS
 };
 
@@ -108,6 +110,8 @@ struct fe_ctx {
   LTabDev ltab{};
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
+  void* h_pin = nullptr;  // pinned host staging of the table build (hipHostMalloc)
+  size_t h_pin_bytes = 0;
   int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 2};  // points per lane, per class
   int target_wg = 4096; // workgroups per class launch
   bool sort_points = true;  // Morton-order the points before evaluation
