@@ -31,7 +31,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .basis import Z_OF, cart_order
+from .basis import Z_OF
 
 HEADER = struct.Struct("<8i")
 
@@ -126,8 +126,8 @@ def read_pairs(path) -> FlatTable:
 def basis_payload(wfn, basis, density, points) -> dict:
     """Flat basis + density + points arrays (``fastesp/table.py:268-293``)."""
     fn_nprim = np.array([len(e) for e in basis.exps], np.int32)
-    shell_ids = np.concatenate([np.full(len(cart_order(sh.l)), i, np.int32)
-                                for i, sh in enumerate(wfn.shells)])
+    shell_ids = np.repeat(np.arange(len(wfn.shells), dtype=np.int32),
+                          [(sh.l + 1) * (sh.l + 2) // 2 for sh in wfn.shells])
     points = np.asarray(points, np.float64).reshape(-1, 3)
     return dict(
         n_func=basis.n, n_prim=int(fn_nprim.sum()), n_points=len(points),
