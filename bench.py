@@ -149,6 +149,19 @@ def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, 
     return out
 
 
+def probe_valu_rate(device: int, precision: int) -> float:
+    """Sustained FMA rate (TFLOP/s) of FP64 (0) or non-packed FP32 (1) chains,
+    from the benchmark's calibration library (not the product library)."""
+    import ctypes
+    lib = ctypes.CDLL(str(ROOT / "fastesp_amd" / "libfastesp_probe.so"))
+    lib.fe_probe_valu_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.fe_probe_last_error.restype = ctypes.c_char_p
+    v = ctypes.c_double()
+    if lib.fe_probe_valu_rate(device, precision, ctypes.byref(v)) != 0:
+        raise RuntimeError(lib.fe_probe_last_error().decode())
+    return v.value
+
+
 def cpu_baseline(w, budget_s: float) -> dict:
     """C++/OpenMP port (oracle/cpu_port) on this host: build + evaluate a sample."""
     sys.path.insert(0, str(ROOT))
@@ -582,8 +595,8 @@ def main() -> int:
     # sustained FMA rates of this device (independent chains): the FP32 the
     # mixed contraction can use is non-packed, so its per-instruction rate is
     # what bounds the mixed-precision gain
-    valu_rate = {"fp64_tflops": round(ctx.valu_rate(_lib.FE_PREC_FP64), 2),
-                 "fp32_nonpacked_tflops": round(ctx.valu_rate(_lib.FE_PREC_MIXED), 2)}
+    valu_rate = {"fp64_tflops": round(probe_valu_rate(0, 0), 2),
+                 "fp32_nonpacked_tflops": round(probe_valu_rate(0, 1), 2)}
     log(f"measured VALU FMA rate: {valu_rate}")
 
     batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None

@@ -16,6 +16,7 @@ loaded on first use and a missing library or device raises HipBackendError.
 from .basis import Basis, Wavefunction, cart_order, overlap_matrix
 from .esp import ESPEngine
 from .formats import basis_payload, write_basis_payload
+from .ints import boys, primitive_pair_integral
 from .molden import read_molden
 from .table import PairTable, build_table
 
@@ -24,6 +25,7 @@ __version__ = "0.1.0"
 __all__ = [
     "Wavefunction", "read_molden",
     "Basis", "cart_order", "overlap_matrix",
+    "boys", "primitive_pair_integral",
     "ESPEngine", "PairTable", "build_table",
     "basis_payload", "write_basis_payload",
     "__version__",

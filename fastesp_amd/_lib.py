@@ -46,6 +46,7 @@ SIGNATURES = {
     "fe_last_error": [],
     "fe_init": [C.c_int, C.POINTER(_vp)],
     "fe_finalize": [_vp],
+    "fe_ctx_trim": [_vp],
     "fe_table_from_pairs": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip,
                             _ip, _ip, _ip, _ip, _dp, C.c_int, _dp, _dp, C.POINTER(_vp)],
     "fe_table_free": [_vp],
@@ -83,7 +84,6 @@ SIGNATURES = {
     "fe_density_set_tuning": [_vp, C.c_int, C.c_int64],
     "fe_density_free": [_vp],
     "fe_vdw_candidates": [_vp, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _dp, _ip],
-    "fe_valu_rate": [_vp, C.c_int, _dp],
 }
 _RESTYPE = {"fe_last_error": C.c_char_p}
 
@@ -159,6 +159,10 @@ class Context:
     def set_tuning(self, key: int, value: int) -> None:
         check(self.lib.fe_set_tuning(self.handle, key, int(value)), "fe_set_tuning")
 
+    def trim(self) -> None:
+        """Release the context's workspace and pinned staging (``fe_ctx_trim``)."""
+        check(self.lib.fe_ctx_trim(self.handle), "fe_ctx_trim")
+
     def profile_start(self) -> None:
         check(self.lib.fe_profile(self.handle, 1, None, None, 0), "fe_profile")
 
@@ -168,13 +172,6 @@ class Context:
         cnt = (C.c_int64 * 32)()
         check(self.lib.fe_profile(self.handle, 0, ms, cnt, 32), "fe_profile")
         return {i: (ms[i], int(cnt[i])) for i in range(32) if cnt[i]}
-
-    def valu_rate(self, precision: int) -> float:
-        """Measured FMA rate (TFLOP/s) of FP64 (FE_PREC_FP64) or non-packed FP32
-        (FE_PREC_MIXED) vector code on this device."""
-        v = C.c_double()
-        check(self.lib.fe_valu_rate(self.handle, precision, C.byref(v)), "fe_valu_rate")
-        return v.value
 
     def boys(self, T, M: int) -> np.ndarray:
         T = np.ascontiguousarray(T, np.float64).ravel()

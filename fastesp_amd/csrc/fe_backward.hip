@@ -62,6 +62,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
     return fe_set_error(FE_ERR_ARG, "fe_esp_forward_grad: bad precision");
   if (n == 0) return FE_OK;
   fe_ctx* c = t->ctx;
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   const double* d_pts = nullptr;
@@ -251,13 +252,17 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     return fe_set_error(FE_ERR_STATE,
                         "fe_esp_backward: dL/dP needs a table built from a basis (fe_table_build)");
   fe_ctx* c = t->ctx;
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
-  // staging: points + weights packed as (x, y, z, gV)
+  // staging: points + weights packed as (x, y, z, gV); with no points nothing
+  // is read from pts or gV (either may be NULL) and dL/dG is zero
   const double* d_pts = nullptr;
-  int rc = fe_stage_points(c, pts, std::max<int64_t>(n, 1), ptr_kind, st, &d_pts);
-  if (n == 0) d_pts = nullptr;
-  if (rc && n > 0) return rc;
+  int rc = FE_OK;
+  if (n > 0) {
+    rc = fe_stage_points(c, pts, n, ptr_kind, st, &d_pts);
+    if (rc) return rc;
+  }
   const DGPlan d = plan_dG(t, std::max<int64_t>(n, 1), c->target_wg);
   const size_t pw_bytes = (size_t)std::max<int64_t>(n, 1) * 4 * sizeof(double);
   rc = fe_ctx_reserve(c, FE_WS_AUX0, pw_bytes + (size_t)std::max<int64_t>(n, 1) * sizeof(double));

@@ -372,6 +372,7 @@ extern "C" int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables,
     return fe_set_error(FE_ERR_ARG, "fe_batch_create: %lld points in total is too many",
                         (long long)tot);
   if (tot > 0 && !pts) return fe_set_error(FE_ERR_ARG, "fe_batch_create: null points");
+  FE_CTX_LOCK(ctx);
   FE_HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = (hipStream_t)stream;
   fe_batch* b = new fe_batch();
@@ -800,6 +801,7 @@ static int check_prec(int precision, const char* fn) {
 extern "C" int fe_batch_forward(fe_batch* b, int precision, int flags, void* stream) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_forward: null batch");
   if (int rc = check_prec(precision, "fe_batch_forward")) return rc;
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = (hipStream_t)stream;
   if (int rc = plan_fwd(b, flags & FE_BATCH_FIELD, precision == FE_PREC_MIXED, st)) return rc;
@@ -810,6 +812,7 @@ extern "C" int fe_batch_forward(fe_batch* b, int precision, int flags, void* str
 
 extern "C" int fe_batch_loss(fe_batch* b, void* stream) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_loss: null batch");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   launch_loss(b, (hipStream_t)stream);
   FE_HIPCHK(hipGetLastError());
@@ -820,6 +823,7 @@ extern "C" int fe_batch_backward(fe_batch* b, const double* gV, int ptr_kind, vo
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_backward: null batch");
   if (gV && ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
     return fe_set_error(FE_ERR_ARG, "fe_batch_backward: bad ptr_kind");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = (hipStream_t)stream;
   if (int rc = plan_bwd(b)) return rc;
@@ -849,6 +853,7 @@ static void launch_step(fe_batch* b, int flags, hipStream_t st) {
 
 extern "C" int fe_batch_step(fe_batch* b, int precision, int flags, void* stream) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_step: null batch");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = (hipStream_t)stream;
   if (int rc = prepare_step(b, precision, flags, st)) return rc;
@@ -859,6 +864,7 @@ extern "C" int fe_batch_step(fe_batch* b, int precision, int flags, void* stream
 
 extern "C" int fe_batch_capture(fe_batch* b, int precision, int flags) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_capture: null batch");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   if (int rc = prepare_step(b, precision, flags, b->cs)) return rc;
   FE_HIPCHK(hipStreamSynchronize(b->cs));
@@ -889,6 +895,7 @@ extern "C" int fe_batch_launch(fe_batch* b, void* stream) {
   if (!b) return fe_set_error(FE_ERR_ARG, "fe_batch_launch: null batch");
   if (!b->exec)
     return fe_set_error(FE_ERR_STATE, "fe_batch_launch: no captured step (fe_batch_capture)");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   FE_HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
   b->has_field = b->f_grad;
@@ -899,6 +906,7 @@ extern "C" int fe_batch_set_points(fe_batch* b, const double* pts, int ptr_kind,
   if (!b || (b->n_tot && !pts)) return fe_set_error(FE_ERR_ARG, "fe_batch_set_points: null argument");
   if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
     return fe_set_error(FE_ERR_ARG, "fe_batch_set_points: bad ptr_kind");
+  FE_CTX_LOCK(b->ctx);
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = (hipStream_t)stream;
   if (b->n_tot)

@@ -39,7 +39,8 @@
 #include "fe_internal.h"
 #include "fe_reduce.h"
 
-#define FE_VERSION 100
+#define FE_VERSION 101
+
 
 namespace {
 thread_local std::string g_err;
@@ -208,6 +209,7 @@ void fe_prof_end(fe_ctx* c, int slot, hipStream_t st) {
 
 extern "C" int fe_profile(fe_ctx* c, int op, double* ms, int64_t* count, int n_phase) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_profile: null context");
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   if (op == 1) {
     c->prof_n = 0;
@@ -287,8 +289,26 @@ extern "C" int fe_finalize(fe_ctx* c) {
   return FE_OK;
 }
 
+extern "C" int fe_ctx_trim(fe_ctx* c) {
+  if (!c) return fe_set_error(FE_ERR_ARG, "fe_ctx_trim: null context");
+  FE_CTX_LOCK(c);
+  FE_HIPCHK(hipSetDevice(c->device));
+  // work still queued on any stream may read the slots
+  FE_HIPCHK(hipDeviceSynchronize());
+  for (int s = 0; s < FE_WS_SLOTS; ++s) {
+    if (c->ws[s]) FE_HIPCHK(hipFree(c->ws[s]));
+    c->ws[s] = nullptr;
+    c->ws_bytes[s] = 0;
+  }
+  if (c->h_pin) FE_HIPCHK(hipHostFree(c->h_pin));
+  c->h_pin = nullptr;
+  c->h_pin_bytes = 0;
+  return FE_OK;
+}
+
 extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_set_tuning: null context");
+  FE_CTX_LOCK(c);
   switch (key) {
     case 0:  // points per lane of every class
       if (value != 1 && value != 2 && value != 4)
@@ -731,6 +751,7 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
   if (parts < 1 || parts > 3) return fe_set_error(FE_ERR_ARG, "fe_esp_forward: bad parts %d", parts);
   if (n == 0) return FE_OK;
   fe_ctx* c = t->ctx;
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   const double* d_pts = nullptr;
@@ -779,6 +800,7 @@ extern "C" int fe_boys_eval(fe_ctx* c, const double* T, int64_t n, int M, int pt
   if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
     return fe_set_error(FE_ERR_ARG, "fe_boys_eval: bad ptr_kind");
   if (n == 0) return FE_OK;
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   const double* dT = T;

@@ -420,6 +420,7 @@ extern "C" int fe_density_eval(fe_density* d, const double* pts, int64_t n_point
     return fe_set_error(FE_ERR_ARG, "fe_density_eval: bad ptr_kind");
   if (n_points == 0) return FE_OK;
   fe_ctx* c = d->ctx;
+  FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   // points padded to 128 so both tile widths divide every chunk

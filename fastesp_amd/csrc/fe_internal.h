@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fastesp_hip.h"
@@ -41,6 +42,7 @@ __host__ __device__ constexpr int fe_dg_minw(int L) {
 }
 
 int fe_set_error(int code, const char* fmt, ...);
+
 int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes);
 
 #define FE_HIPCHK(x)                                                                         \
@@ -103,6 +105,11 @@ struct FeProfEv {
 };
 
 struct fe_ctx {
+  // serialises the entry points that use the workspace slots below (calls on
+  // one context from several host threads queue up here; device-pointer calls
+  // on different streams must still be ordered by the caller, because the
+  // slots are shared by every stream)
+  std::recursive_mutex mu;
   int device = 0;
   int n_cu = 256;
   double* d_boys = nullptr;  // (FE_BOYS_MMAX+1) x NR x COLS
@@ -125,6 +132,8 @@ struct fe_ctx {
   std::vector<FeProfEv> prof;
   size_t prof_n = 0;
 };
+
+#define FE_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> fe_ctx_lock_((c)->mu)
 
 // start / end of a timed phase on stream st (no-ops unless profiling)
 int fe_prof_begin(fe_ctx* c, int phase, hipStream_t st);

@@ -20,29 +20,19 @@ import math
 
 import numpy as np
 
-from .basis import Basis, Shell, Wavefunction, cart_order
 
 
 def pair_case(A, la, al, B, lb, be):
     """(wfn, basis, P): two unit-coefficient primitive shells and a density
     whose only non-zero entry couples the components la (on A) and lb (on B)."""
-    shells = [Shell(0, sum(la), np.array([al]), np.array([1.0])),
-              Shell(1, sum(lb), np.array([be]), np.array([1.0]))]
-    wfn = Wavefunction(symbols=["H", "H"], coords=np.array([A, B], float), shells=shells)
-    basis = Basis(wfn)
-    P = np.zeros((basis.n, basis.n))
-    i = cart_order(sum(la)).index(tuple(la))
-    j = len(cart_order(sum(la))) + cart_order(sum(lb)).index(tuple(lb))
-    P[i, j] = 1.0
-    return wfn, basis, P
+    from .ints import pair_case as _case
+    return _case(A, la, al, B, lb, be)
 
 
 def _pair_integral(A, la, al, B, lb, be, C) -> float:
     """<a|1/|r-C||b> of two unit-coefficient Cartesian primitives, on the GPU."""
-    from .builder import build_table_hip
-    wfn, basis, P = pair_case(A, la, al, B, lb, be)
-    C = np.asarray(C, float).reshape(1, 3)
-    return float(build_table_hip(wfn, basis, P, C, tol=0.0).electron(C)[0])
+    from .ints import primitive_pair_integral
+    return primitive_pair_integral(al, A, la, be, B, lb, np.asarray(C, float).reshape(3))
 
 
 def quad(C, A, la, al, B, lb, be, R=14.0, ngl=240, ncos=220, nphi=440) -> float:

@@ -97,6 +97,41 @@ class PairTable:
         except Exception:
             pass
 
+    # -- the reference's flat-table fields (fastesp/table.py:41-62) ---------
+    # Exported from the device on first use (fe_table_export, reference pair
+    # order) and returned as read-only arrays, like the frombuffer views of a
+    # table read from pairs.bin (fastesp/table.py:150-154).
+    def _field(self, name: str) -> np.ndarray:
+        v = getattr(self.to_flat(), name).view()
+        v.flags.writeable = False
+        return v
+
+    pp_exp = property(lambda self: self._field("pp_exp"), doc="product exponents p")
+    pp_cx = property(lambda self: self._field("pp_cx"), doc="product centre x")
+    pp_cy = property(lambda self: self._field("pp_cy"), doc="product centre y")
+    pp_cz = property(lambda self: self._field("pp_cz"), doc="product centre z")
+    pp_K = property(lambda self: self._field("pp_K"), doc="exp(-ab/p |A-B|^2)")
+    pp_nt = property(lambda self: self._field("pp_nt"), doc="terms per pair")
+    pp_off = property(lambda self: self._field("pp_off"), doc="first term of each pair")
+    pp_nmono = property(lambda self: self._field("pp_nmono"), doc="monomials per pair")
+    pp_monooff = property(lambda self: self._field("pp_monooff"), doc="first monomial of each pair")
+    mo_parent = property(lambda self: self._field("mo_parent"), doc="monomial schedule: parent")
+    mo_axis = property(lambda self: self._field("mo_axis"), doc="monomial schedule: axis")
+    tm_aidx = property(lambda self: self._field("tm_aidx"), doc="monomial index of each term")
+    tm_m = property(lambda self: self._field("tm_m"), doc="Boys order of each term")
+    tm_g = property(lambda self: self._field("tm_g"), doc="coefficient of each term")
+
+    @property
+    def max_mono(self) -> int:
+        """Largest monomial count of a pair (header field of pairs.bin)."""
+        return int(self.to_flat().max_mono)
+
+    def screen(self, tol: float) -> "PairTable":
+        """``PairTable.screen`` (``fastesp/table.py:107-116``): the reference's
+        method keeps every pair and returns the table itself; the screen that
+        drops pairs runs inside the device build (``tol`` of ``build_table``)."""
+        return self
+
     # -- info --------------------------------------------------------------
     def info(self) -> dict:
         v = (C.c_int64 * 16)()

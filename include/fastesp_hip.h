@@ -20,8 +20,11 @@
  *    copies through device staging buffers and synchronises the stream before
  *    returning) or FE_PTR_DEVICE (device pointers; work is only enqueued on
  *    `stream`, which is a hipStream_t passed as void*, NULL = default stream).
- *  - One fe_ctx per device.  Calls on one ctx (and the tables made from it)
- *    must be serialised by the caller.
+ *  - One fe_ctx per device.  The context's entry points take a lock, so
+ *    calls from several host threads queue up; but the context's workspace is
+ *    shared by every stream, so device-pointer calls that enqueue work on
+ *    different streams must be ordered by the caller (host-pointer calls
+ *    synchronise their stream before returning and are safe).
  *  - Units: Bohr in, Hartree/e out (fastesp/table.py:73-104).
  */
 #ifndef FASTESP_HIP_H
@@ -68,6 +71,14 @@ int fe_init(int device, fe_ctx** ctx);
 
 /* Free every device buffer of the context (tables must be freed first). */
 int fe_finalize(fe_ctx* ctx);
+
+/* Release the context's workspace (the scratch that table builds and
+ * evaluations grow on demand and keep for the next call: about 0.1 GB after a
+ * 1634-function build, 0.7 GB after a 4538-function one) and its pinned host
+ * staging buffer.  Waits for all work on the device first.  The next call
+ * allocates again what it needs.  No counterpart in the reference, whose
+ * processes free everything at exit (fastesp/cli.py:143-163). */
+int fe_ctx_trim(fe_ctx* ctx);
 
 /* Upload an existing flat pair table (the pairs.bin fields of
  * fastesp/table.py:11-24, as written by build_pairs.cpp:386-421 or
@@ -189,12 +200,6 @@ int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
  * 16+L dL/dG of class L, 24+L dL/dG chunk reduction of class L.  Launches
  * recorded into a hipGraph are not timed. */
 int fe_profile(fe_ctx* ctx, int op, double* ms, int64_t* count, int n_phase);
-
-/* Sustained VALU FMA rate of the device in TFLOP/s, measured with independent
- * FMA chains: FE_PREC_FP64 for FP64, FE_PREC_MIXED for FP32 as this library is
- * compiled (non-packed).  No counterpart in the reference: it calibrates the
- * roofline peaks of the benchmark. */
-int fe_valu_rate(fe_ctx* ctx, int precision, double* tflops);
 
 /* ---- batched conformers ------------------------------------------------
  * No counterpart in the reference (it evaluates one table per esp_gpu

@@ -60,7 +60,6 @@ def test_errors_are_reported_not_fatal():
     assert lib.fe_density_eval(None, None, 1, 0, None, None) == -1
     assert lib.fe_density_free(None) == 0
     assert lib.fe_vdw_candidates(None, 1, None, None, None, None, None) == -1
-    assert lib.fe_valu_rate(None, 0, None) == -1
 
 
 def test_no_device_is_a_clean_error():

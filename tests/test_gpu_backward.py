@@ -148,3 +148,33 @@ def test_backward_without_points_is_zero():
     assert not dG.any() and not dP.any()
     V, F = esp_with_grad(t, np.zeros((0, 3)))
     assert V.shape == (0,) and F.shape == (0, 3)
+
+
+def test_backward_without_points_accepts_null_arrays():
+    """n = 0 with NULL points and weights, host outputs: nothing is read from
+    them and dL/dG, dL/dP come back zero."""
+    import ctypes as C
+    from fastesp_amd import _lib
+    wfn, basis, P, _ = cf_toy()
+    t = build_table_hip(wfn, basis, P, np.zeros((4, 3)) + 3.0, tol=0.0)
+    n_terms = t.n_terms
+    dG = np.full(n_terms, np.nan)
+    dP = np.full((basis.n, basis.n), np.nan)
+    lib = _lib.load()
+    say("backward n=0 with NULL arrays")
+    rc = lib.fe_esp_backward(t._h, None, 0, None, _lib.FE_PREC_FP64, _lib.FE_PTR_HOST, None,
+                             dG.ctypes.data, dP.ctypes.data)
+    assert rc == 0, lib.fe_last_error()
+    assert not dG.any() and not dP.any()
+
+
+def test_trim_releases_workspace_and_next_call_works():
+    from fastesp_amd import _lib
+    wfn, basis, P, _ = cf_toy()
+    pts = np.array([[2.0, 1.0, 3.0], [-2.5, 0.5, 1.0]])
+    t = build_table_hip(wfn, basis, P, pts, tol=0.0)
+    V0, F0 = esp_with_grad(t, pts)
+    say("fe_ctx_trim")
+    _lib.Context.get().trim()
+    V1, F1 = esp_with_grad(t, pts)
+    assert np.array_equal(V0, V1) and np.array_equal(F0, F1)
