@@ -27,6 +27,7 @@ FE_PART_ELECTRON = 2
 FE_PART_ESP = 3
 FE_LMAX = 6
 FE_BOYS_MMAX = 8
+FE_NOALLOC_MAX_POINTS = 65536
 FE_BATCH_FIELD = 1
 FE_BATCH_DLDP = 2
 FE_BOUT_V, FE_BOUT_FIELD, FE_BOUT_DLDP, FE_BOUT_LOSS, FE_BOUT_GV = range(5)
@@ -52,6 +53,7 @@ SIGNATURES = {
     "fe_table_free": [_vp],
     "fe_table_info": [_vp, C.POINTER(C.c_int64), C.c_int],
     "fe_esp_forward": [_vp, _vp, C.c_int64, C.c_int, C.c_int, C.c_int, _vp, _vp],
+    "fe_esp_forward_noalloc": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp],
     "fe_boys_eval": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp],
     "fe_set_tuning": [_vp, C.c_int, C.c_int64],
     "fe_table_build": [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp,

@@ -269,6 +269,12 @@ extern "C" int fe_init(int device, fe_ctx** out) {
     return rc;
   }
   c->target_wg = 16 * c->n_cu;
+  if (int rc = fe_noalloc_init(c)) {
+    (void)hipFree(c->d_ltab_mem);
+    (void)hipFree(c->d_boys);
+    delete c;
+    return rc;
+  }
   *out = c;
   return FE_OK;
 }
@@ -680,6 +686,13 @@ __global__ __launch_bounds__(256) void k_esp_reduce(const double* __restrict__ p
                          ? fe_nuclear(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], atoms, n_atoms)
                          : 0.0;
   V[o] = (parts & FE_PART_ELECTRON) ? nuc - e : nuc;
+}
+
+void fe_launch_esp_reduce(const double* partial, int n_rows, int ld, const double* pts, int n,
+                          const fe_table* t, int parts, const int32_t* perm, double* V,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, partial,
+                     n_rows, ld, pts, n, t->d_atoms, t->n_atoms, parts, perm, V);
 }
 
 template <int M>

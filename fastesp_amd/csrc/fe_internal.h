@@ -131,6 +131,9 @@ struct fe_ctx {
   bool prof_on = false;     // record phase events (fe_profile)
   std::vector<FeProfEv> prof;
   size_t prof_n = 0;
+  // device address of this device's copy of the file-scope scratch array of
+  // fe_esp_forward_noalloc (from hipGetSymbolAddress at fe_init)
+  char* noalloc_scratch = nullptr;
 };
 
 #define FE_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> fe_ctx_lock_((c)->mu)
@@ -222,8 +225,13 @@ FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg);
   } while (0)
 int fe_stage_points(fe_ctx* c, const double* pts, int64_t n, int ptr_kind, hipStream_t st,
                     const double** d_pts);
+// Morton-sorted copy of the points and the permutation (sorted -> caller
+// index), in the context's FE_WS_SORT slot, or in `buf` (buf_bytes long) when
+// it is given: then nothing is allocated and FE_ERR_ARG comes back if it is too
+// small.
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
-                   const double** d_sorted, const int32_t** d_perm);
+                   const double** d_sorted, const int32_t** d_perm, void* buf = nullptr,
+                   size_t buf_bytes = 0);
 // one work item of the batched evaluation: pairs [p0, p1) of class L of
 // conformer `conf` against the sorted points [base, min(base + PPL*FE_WG, end));
 // the partial sums go to partial[out + ...] (planes `plane` apart for the field)
@@ -247,6 +255,12 @@ struct FeBatchEvalArgs {
 };
 
 int fe_ensure_g32(fe_table* t, hipStream_t st);
+// V = [nuclear] - [sum of the first n_rows partial rows] (k_esp_reduce)
+void fe_launch_esp_reduce(const double* partial, int n_rows, int ld, const double* pts, int n,
+                          const fe_table* t, int parts, const int32_t* perm, double* V,
+                          hipStream_t st);
+// takes this device's address of the no-allocation scratch array (fe_init)
+int fe_noalloc_init(fe_ctx* c);
 int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,
                    size_t plane, bool grad, bool mixed, hipStream_t st);
 // launches the batched evaluation of class L (a != nullptr), or returns its

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_gather_pts(const double* __restrict__ p
 // nuclei's bounding box grown by 16 Bohr (points outside are clamped, which
 // only costs locality, never correctness).
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
-                   const double** d_sorted, const int32_t** d_perm) {
+                   const double** d_sorted, const int32_t** d_perm, void* buf, size_t buf_bytes) {
   double lo[3] = {-16.0, -16.0, -16.0}, hi[3] = {16.0, 16.0, 16.0};
   if (t->n_atoms > 0) {
     for (int a = 0; a < 3; ++a) {
@@ -85,9 +85,16 @@ int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hip
   auto up = [&](size_t b) { return (b + al - 1) / al * al; };
   const size_t b_key = up((size_t)n * 4), b_pts = up((size_t)n * 3 * sizeof(double));
   const size_t need = 4 * b_key + b_pts + up(temp_bytes);
-  int rc = fe_ctx_reserve(c, FE_WS_SORT, need);
-  if (rc) return rc;
-  char* base = (char*)c->ws[FE_WS_SORT];
+  char* base = (char*)buf;
+  if (buf) {
+    if (buf_bytes < need)
+      return fe_set_error(FE_ERR_ARG, "point sort: %zu bytes of scratch needed, %zu given", need,
+                          buf_bytes);
+  } else {
+    int rc = fe_ctx_reserve(c, FE_WS_SORT, need);
+    if (rc) return rc;
+    base = (char*)c->ws[FE_WS_SORT];
+  }
   uint32_t* k_in = (uint32_t*)base;
   uint32_t* k_out = (uint32_t*)(base + b_key);
   int32_t* v_in = (int32_t*)(base + 2 * b_key);

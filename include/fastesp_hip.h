@@ -173,6 +173,28 @@ int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const double* gV,
                     int precision, int ptr_kind, void* stream, double* dLdG,
                     double* dLdP);
 
+/* fe_esp_forward on device buffers that allocates nothing: no hipMalloc,
+ * hipFree or allocating library call, so it can run where allocation is not
+ * allowed (under HIP graph capture, in tight serving loops, in harnesses that
+ * forbid it).  No counterpart in the reference (esp_gpu allocates per run,
+ * fastesp/native/esp_gpu.cu:251-277); it is for those callers.
+ *  - pts (n x 3) and V (n) are device pointers; work is enqueued on `stream`.
+ *  - n <= FE_NOALLOC_MAX_POINTS, else FE_ERR_ARG.
+ *  - Its scratch (point sort + partial sums, FE_NOALLOC_SCRATCH_MIB MiB) is a
+ *    fixed device array that every GPU holds for the life of the process.
+ *    Every call on one GPU shares that GPU's array and keeps the last call's
+ *    contents: calls that overlap on one GPU (two streams, or two threads with
+ *    different contexts of the same device) race and must be ordered by the
+ *    caller.
+ *  - FE_PREC_MIXED needs the table's FP32 coefficients, which the first
+ *    fe_esp_forward with FE_PREC_MIXED builds (FE_ERR_STATE otherwise).
+ *  - Same parts, precision and results as fe_esp_forward; the pair-chunk
+ *    count may be lower (fewer partial rows), which can change the last bits. */
+#define FE_NOALLOC_MAX_POINTS 65536
+#define FE_NOALLOC_SCRATCH_MIB 32
+int fe_esp_forward_noalloc(fe_table* t, const double* pts, int64_t n, int precision,
+                           int parts, void* stream, double* V);
+
 /* Boys functions F_0..F_M(T) for n arguments with the device algorithm
  * (F is (n, M+1) f64, row-major).  Replaces boys_all
  * (fastesp/native/esp_gpu.cu:36-66) / fastesp.ints.boys (fastesp/ints.py:35-40)
