@@ -60,6 +60,7 @@ SIGNATURES = {
                        _vp, C.c_int, _vp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, _vp,
                        C.POINTER(_vp)],
     "fe_table_stats": [_vp, _dp, C.c_int],
+    "fe_table_rebuild": [_vp, _vp, C.c_int, _vp, C.c_double, C.c_int, C.c_int, _vp],
     "fe_esp_forward_grad": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, _vp, _vp],
     "fe_esp_backward": [_vp, _vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, _vp, _vp],
     "fe_table_export": [_vp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip,

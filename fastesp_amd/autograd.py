@@ -5,9 +5,11 @@
     loss = (w * (V - target) ** 2).sum()
     loss.backward()                      # points.grad = gV * dV/dC, density.grad = dL/dP
 
-Forward builds the pair table from ``density`` on the device
-(``fe_table_build``, pointers handed over directly, no host round trip) and
-evaluates V -- fused with the field dV/dC when ``points`` requires a gradient
+Forward builds the pair table from ``density`` on the device: the first call
+with ``fe_table_build`` (pointers handed over directly), later calls in place
+with ``fe_table_rebuild``, which neither allocates nor synchronises with the
+host, so a whole step (build, forward, backward) can be recorded as a graph.
+It then evaluates V -- fused with the field dV/dC when ``points`` requires a gradient
 (``fe_esp_forward_grad``).  Backward returns gV * dV/dC and dL/dP
 (``fe_esp_backward``: dL/dG per pair with the table's G recomputed from the
 stored Hermite data rather than stored per point, then the adjoint of the
@@ -52,20 +54,33 @@ def _prec(p) -> int:
 
 
 class ESPModel:
-    """Holds the basis; ``__call__(points, density)`` is differentiable in both."""
+    """Holds the basis; ``__call__(points, density)`` is differentiable in both.
+
+    ``reuse_table=True`` (default) keeps one device table and rebuilds it in
+    place on every call after the first (``fe_table_rebuild``); a backward must
+    then run before the next forward (it checks, and raises otherwise).
+    ``reuse_table=False`` builds a fresh table per call."""
 
     def __init__(self, wfn, basis, tol: float = 1e-12, precision="fp64", rule: int = 0,
-                 device: int | None = None):
+                 device: int | None = None, reuse_table: bool = True):
         self.ctx = _lib.Context.get(device)
         self.arr = _BasisArrays(wfn, basis)
         self.tol = float(tol)
         self.precision = _prec(precision)
         self.rule = int(rule)
+        self.reuse_table = bool(reuse_table)
         self.last_table: PairTable | None = None
+        self.generation = 0  # number of builds of the held table
 
     def build(self, points: torch.Tensor, density: torch.Tensor) -> PairTable:
         a = self.arr
         stream = torch.cuda.current_stream(points.device).cuda_stream
+        if self.reuse_table and self.last_table is not None:
+            _lib.check(self.ctx.lib.fe_table_rebuild(
+                self.last_table._h, density.data_ptr(), points.shape[0], points.data_ptr(),
+                self.tol, self.rule, _lib.FE_PTR_DEVICE, stream), "fe_table_rebuild")
+            self.generation += 1
+            return self.last_table
         ip = lambda k: _lib.ptr(a.i32[k], C.c_int32)  # noqa: E731
         h = C.c_void_p()
         _lib.check(self.ctx.lib.fe_table_build(
@@ -77,6 +92,8 @@ class ESPModel:
             C.byref(h)), "fe_table_build")
         t = PairTable(h, self.ctx, np.zeros((0, 3)), a.f64["Z"], a.f64["xyz"])
         t.precision = self.precision
+        self.last_table = t
+        self.generation += 1
         return t
 
     def __call__(self, points: torch.Tensor, density: torch.Tensor) -> torch.Tensor:
@@ -98,7 +115,7 @@ class ESPFunction(torch.autograd.Function):
         pts = _check(points.detach(), "points", (n, 3))
         dens = _check(density.detach(), "density", (model.arr.n_func, model.arr.n_func))
         table = model.build(pts, dens)
-        model.last_table = table
+        ctx.generation = model.generation
         stream = torch.cuda.current_stream(pts.device).cuda_stream
         lib = model.ctx.lib
         V = torch.empty(n, dtype=torch.float64, device=pts.device)
@@ -121,6 +138,9 @@ class ESPFunction(torch.autograd.Function):
     def backward(ctx, gV):
         pts, field = ctx.saved_tensors
         model = ctx.model
+        if model.reuse_table and ctx.generation != model.generation:
+            raise RuntimeError("ESPModel: the table was rebuilt by a later forward; run each "
+                               "backward before the next forward, or use reuse_table=False")
         g_pts = g_dens = None
         gV = gV.detach().to(torch.float64).contiguous()
         if ctx.needs_input_grad[0]:

@@ -23,7 +23,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 SOURCES = ["fe_core.hip", "fe_eval.hip", "fe_mixed.hip", "fe_backward.hip", "fe_build.hip",
-           "fe_sort.hip", "fe_batch.hip", "fe_density.hip", "fe_noalloc.hip"]
+           "fe_sort.hip", "fe_batch.hip", "fe_density.hip", "fe_noalloc.hip", "fe_prim.hip"]
 LIB = PKG / "libfastesp_hip.so"
 PROBE_SOURCES = ["probe/fe_probe.hip"]
 PROBE_LIB = PKG / "libfastesp_probe.so"

@@ -118,19 +118,23 @@ __global__ __launch_bounds__(256) void k_pack_pw(const double* __restrict__ pts,
 }
 
 // one lane per pair of class L; points (x, y, z, gV) of chunk blockIdx.y are
-// wave-uniform; partial[(chunk * NT + k) * n_pairs + pair]
+// wave-uniform; partial[(chunk * NT + k) * stride + pair].  The kept count is
+// read on the device; `stride` (>= it) is the count the launch was planned for.
 template <int L>
-__global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG(const double* __restrict__ rec, const int n_pairs,
+__global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG(const double* __restrict__ rec,
+                                            const int* __restrict__ n_pairs_p, const int stride,
                                             const double* __restrict__ pw, const int n_pts,
                                             const int ppc, const double* __restrict__ boys_tab,
                                             double* __restrict__ partial) {
+  const int n_pairs = min(*n_pairs_p, stride);
+  if ((int)(blockIdx.x * 256) >= n_pairs) return;  // block-uniform
   constexpr int NTAB = fe_boys_rows(L) * FE_BOYS_COLS;
   __shared__ double s_tab[NTAB];
   for (int i = threadIdx.x; i < NTAB; i += 256) s_tab[i] = boys_tab[i];
   __syncthreads();
   const int q0 = blockIdx.y * ppc;
   fe_dg_block<L>(rec, n_pairs, blockIdx.x * 256 + threadIdx.x, pw, q0, min(n_pts, q0 + ppc),
-                 s_tab, partial + (size_t)blockIdx.y * FeL<L>::NT * n_pairs, n_pairs);
+                 s_tab, partial + (size_t)blockIdx.y * FeL<L>::NT * stride, stride);
 }
 
 __global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, const int64_t n,
@@ -146,16 +150,20 @@ __global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, const int6
 // combined in a fixed order in LDS (bitwise reproducible).
 template <int L>
 __global__ __launch_bounds__(256) void k_dG_reduce(const double* __restrict__ partial,
-                                                   const int n_pairs, const int n_chunks,
-                                                   double* __restrict__ dG,
+                                                   const int n_pairs, const int* __restrict__ kept,
+                                                   const int n_chunks, double* __restrict__ dG,
                                                    const int* __restrict__ flatoff,
                                                    double* __restrict__ flat) {
+  // n_pairs: the planned count (layout stride); pairs from the device kept
+  // count on are not part of the table
   constexpr int NT = FeL<L>::NT;
   __shared__ double s_red[4][64];
   const int lane = threadIdx.x & 63, sl = threadIdx.x >> 6;
   const int64_t e = (int64_t)blockIdx.x * 64 + lane;
-  const bool valid = e < (int64_t)n_pairs * NT;
-  const int k = valid ? (int)(e / n_pairs) : 0;
+  const int nk = min(*kept, n_pairs);
+  const int k0 = e < (int64_t)n_pairs * NT ? (int)(e / n_pairs) : 0;
+  const bool valid = e < (int64_t)n_pairs * NT && (int)(e - (int64_t)k0 * n_pairs) < nk;
+  const int k = valid ? k0 : 0;
   const int pair = valid ? (int)(e - (int64_t)k * n_pairs) : 0;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   if (valid) {
@@ -191,7 +199,7 @@ struct DGPlan {
 static DGPlan plan_dG(const fe_table* t, int64_t n, int target_wg) {
   DGPlan d{};
   for (int L = 0; L <= FE_LMAX; ++L) {
-    const int64_t nL = t->n_L[L];
+    const int64_t nL = fe_plan_n(t, L);
     d.part_off[L] = d.part_total;
     d.dg_off[L] = d.dg_total;
     if (!nL) continue;
@@ -216,17 +224,18 @@ static DGPlan plan_dG(const fe_table* t, int64_t n, int target_wg) {
 template <int L>
 static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, int n,
                         double* part, double* dG, double* flat, hipStream_t st) {
-  const int64_t nL = t->n_L[L];
+  const int64_t nL = fe_plan_n(t, L);
   if (!nL) return;
   const double* tab = t->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS;
+  const int* kept = t->d_stats + FE_ST_KEPT + L;
   const int pg = fe_prof_begin(t->ctx, FE_PH_DG0 + L, st);
   hipLaunchKernelGGL(k_dG<L>, dim3((unsigned)((nL + 255) / 256), d.chunks[L]), dim3(256), 0, st,
-                     t->d_rec[L], (int)nL, pw, n, d.ppc[L], tab, part + d.part_off[L]);
+                     t->d_rec[L], kept, (int)nL, pw, n, d.ppc[L], tab, part + d.part_off[L]);
   fe_prof_end(t->ctx, pg, st);
   const int64_t ne = nL * FeL<L>::NT;
   const int pr = fe_prof_begin(t->ctx, FE_PH_DGRED0 + L, st);
   hipLaunchKernelGGL(k_dG_reduce<L>, dim3((unsigned)((ne + 63) / 64)), dim3(256), 0, st,
-                     part + d.part_off[L], (int)nL, d.chunks[L], dG + d.dg_off[L],
+                     part + d.part_off[L], (int)nL, kept, d.chunks[L], dG + d.dg_off[L],
                      t->d_flatoff[L], flat);
   fe_prof_end(t->ctx, pr, st);
 }
@@ -251,6 +260,9 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
   if (dLdP && !t->has_basis)
     return fe_set_error(FE_ERR_STATE,
                         "fe_esp_backward: dL/dP needs a table built from a basis (fe_table_build)");
+  // the flat dL/dG layout (n_terms values) needs the host-side counts
+  if (dLdG)
+    if (int rc = fe_table_sync_counts(t)) return rc;
   fe_ctx* c = t->ctx;
   FE_CTX_LOCK(c);
   FE_HIPCHK(hipSetDevice(c->device));

@@ -366,6 +366,8 @@ extern "C" int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables,
     if (tables[i]->ctx != ctx)
       return fe_set_error(FE_ERR_ARG, "fe_batch_create: table %d belongs to another context", i);
     if (n_pts[i] < 0) return fe_set_error(FE_ERR_ARG, "fe_batch_create: n_pts[%d] < 0", i);
+    // work items are cut from the host-side counts
+    if (int rc = fe_table_sync_counts(tables[i])) return rc;
     tot += n_pts[i];
   }
   if (tot > (int64_t)INT32_MAX / 8)

@@ -358,18 +358,69 @@ static int class_of_nterm(int nt) {
   return -1;
 }
 
-int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec) {
+int fe_table_layout(fe_table* t, const int64_t* cap, int64_t n_slots) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t o_stats = off;
+  off += al(FE_ST_SUMRULE_BYTE + sizeof(double));
+  size_t o_rec[FE_LMAX + 1], o_off[FE_LMAX + 1], o_orig[FE_LMAX + 1], o_s2k[FE_LMAX + 1];
   for (int L = 0; L <= FE_LMAX; ++L) {
-    t->n_L[L] = rec[L].size() / fe_host::STRIDE[L];
-    if (t->n_L[L] == 0) continue;
-    const size_t b = rec[L].size() * sizeof(double);
-    if (hipMalloc(&t->d_rec[L], b) != hipSuccess) {
-      (void)hipGetLastError();
-      return fe_set_error(FE_ERR_ALLOC, "table: allocation of %zu bytes failed", b);
-    }
-    FE_HIPCHK(hipMemcpy(t->d_rec[L], rec[L].data(), b, hipMemcpyHostToDevice));
-    t->dev_bytes += b;
+    o_rec[L] = off;
+    off += al((size_t)cap[L] * fe_host::STRIDE[L] * sizeof(double));
+    o_off[L] = off;
+    off += al((size_t)cap[L] * sizeof(int));
+    o_orig[L] = off;
+    off += al((size_t)cap[L] * sizeof(int32_t));
   }
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    o_s2k[L] = off;
+    off += n_slots ? al((size_t)t->n_cand_L[L] * sizeof(int)) : 0;
+  }
+  void* m = nullptr;
+  if (hipMalloc(&m, off) != hipSuccess) {
+    (void)hipGetLastError();
+    return fe_set_error(FE_ERR_ALLOC, "table: allocation of %zu bytes failed", off);
+  }
+  if (t->d_mem) {
+    (void)hipFree(t->d_mem);
+    t->dev_bytes -= t->mem_bytes;
+  }
+  t->d_mem = m;
+  t->mem_bytes = off;
+  t->dev_bytes += off;
+  char* base = (char*)m;
+  t->d_stats = (int*)(base + o_stats);
+  t->d_sum_rule = (double*)(base + o_stats + FE_ST_SUMRULE_BYTE);
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    t->cap_L[L] = cap[L];
+    t->d_rec[L] = (double*)(base + o_rec[L]);
+    t->d_flatoff[L] = (int*)(base + o_off[L]);
+    t->d_orig[L] = (int32_t*)(base + o_orig[L]);
+    t->d_slot2kept[L] = n_slots ? (int*)(base + o_s2k[L]) : nullptr;
+  }
+  return FE_OK;
+}
+
+int fe_table_sync_counts(fe_table* t) {
+  if (!t->counts_stale) return FE_OK;
+  FE_HIPCHK(hipSetDevice(t->ctx->device));
+  FE_HIPCHK(hipStreamSynchronize(t->count_stream));
+  int st[FE_ST_N];
+  double sr = 0.0;
+  FE_HIPCHK(hipMemcpy(st, t->d_stats, sizeof st, hipMemcpyDeviceToHost));
+  FE_HIPCHK(hipMemcpy(&sr, t->d_sum_rule, sizeof sr, hipMemcpyDeviceToHost));
+  int64_t np = 0, nt = 0, nc = 0;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    t->n_L[L] = st[FE_ST_KEPT + L];
+    np += st[FE_ST_KEPT + L];
+    nt += (int64_t)st[FE_ST_KEPT + L] * fe_host::NTERM[L];
+    nc += st[FE_ST_VALID + L];
+  }
+  t->n_pairs = (int)np;
+  t->n_terms = (int)nt;
+  t->n_candidates = nc;
+  t->sum_rule = sr;
+  t->counts_stale = false;
   return FE_OK;
 }
 
@@ -431,39 +482,6 @@ std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n) {
     order.swap(tmp);
   }
   return order;
-}
-
-int fe_table_index(fe_table* t) {
-  // records in `orig` order (unique indices): term counts scattered by orig,
-  // an exclusive prefix sum, and each record's offset read back
-  int32_t omax = -1;
-  for (int L = 0; L <= FE_LMAX; ++L)
-    for (const int32_t o : t->orig[L]) omax = std::max(omax, o);
-  std::vector<int32_t> at((size_t)omax + 1, 0);
-  for (int L = 0; L <= FE_LMAX; ++L)
-    for (const int32_t o : t->orig[L]) at[o] = fe_host::NTERM[L];
-  int32_t toff = 0;
-  for (auto& v : at) {
-    const int32_t n = v;
-    v = toff;
-    toff += n;
-  }
-  std::vector<std::vector<int>> off(FE_LMAX + 1);
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    off[L].resize(t->n_L[L]);
-    for (int64_t k = 0; k < t->n_L[L]; ++k) off[L][k] = at[t->orig[L][k]];
-  }
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    if (!t->n_L[L]) continue;
-    if (hipMalloc(&t->d_flatoff[L], sizeof(int) * t->n_L[L]) != hipSuccess) {
-      (void)hipGetLastError();
-      return fe_set_error(FE_ERR_ALLOC, "table: index allocation failed");
-    }
-    FE_HIPCHK(hipMemcpy(t->d_flatoff[L], off[L].data(), sizeof(int) * t->n_L[L],
-                        hipMemcpyHostToDevice));
-    t->dev_bytes += sizeof(int) * t->n_L[L];
-  }
-  return FE_OK;
 }
 
 int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz) {
@@ -564,18 +582,35 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
   t->ctx = c;
   t->n_pairs = n_pairs;
   t->n_terms = n_terms;
-  int rc = fe_table_alloc_classes(t, rec);
-  if (rc == FE_OK) rc = fe_table_set_atoms(t, n_atoms, Z, xyz);
-  if (rc != FE_OK) {
+  auto fail = [&](int rc) {
     fe_table_free(t);
     return rc;
+  };
+  // flat term offset of every record: term counts in reference order, scanned
+  std::vector<int32_t> toff(n_pairs + 1, 0);
+  for (int i = 0; i < n_pairs; ++i) toff[i + 1] = toff[i] + pp_nt[i];
+  int64_t cap[FE_LMAX + 1];
+  for (int L = 0; L <= FE_LMAX; ++L) cap[L] = t->n_L[L] = (int64_t)orig[L].size();
+  int rc = fe_table_layout(t, cap, 0);
+  if (rc != FE_OK) return fail(rc);
+  std::vector<int> st(FE_ST_N, 0);
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    st[FE_ST_KEPT + L] = st[FE_ST_VALID + L] = st[FE_ST_KVALID + L] = (int)t->n_L[L];
+    if (!t->n_L[L]) continue;
+    std::vector<int> fo(t->n_L[L]);
+    for (int64_t k = 0; k < t->n_L[L]; ++k) fo[k] = toff[orig[L][k]];
+    FE_HIPCHK(hipMemcpy(t->d_rec[L], rec[L].data(), rec[L].size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+    FE_HIPCHK(hipMemcpy(t->d_flatoff[L], fo.data(), fo.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+    FE_HIPCHK(hipMemcpy(t->d_orig[L], orig[L].data(), orig[L].size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
   }
-  for (int L = 0; L <= FE_LMAX; ++L) t->orig[L] = std::move(orig[L]);
-  rc = fe_table_index(t);
-  if (rc != FE_OK) {
-    fe_table_free(t);
-    return rc;
-  }
+  const double sr = 0.0;
+  FE_HIPCHK(hipMemcpy(t->d_stats, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+  FE_HIPCHK(hipMemcpy(t->d_sum_rule, &sr, sizeof sr, hipMemcpyHostToDevice));
+  rc = fe_table_set_atoms(t, n_atoms, Z, xyz);
+  if (rc != FE_OK) return fail(rc);
   *out = t;
   return FE_OK;
 }
@@ -583,12 +618,11 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
 extern "C" int fe_table_free(fe_table* t) {
   if (!t) return fe_set_error(FE_ERR_ARG, "fe_table_free: null table");
   (void)hipSetDevice(t->ctx->device);
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    if (t->d_rec[L]) (void)hipFree(t->d_rec[L]);
+  // a rebuild may still be running on its stream
+  if (t->counts_stale) (void)hipStreamSynchronize(t->count_stream);
+  for (int L = 0; L <= FE_LMAX; ++L)
     if (t->d_rec32[L]) (void)hipFree(t->d_rec32[L]);
-    if (t->d_slot2kept[L]) (void)hipFree(t->d_slot2kept[L]);
-    if (t->d_flatoff[L]) (void)hipFree(t->d_flatoff[L]);
-  }
+  if (t->d_mem) (void)hipFree(t->d_mem);
   if (t->d_atoms) (void)hipFree(t->d_atoms);
   if (t->d_basis_mem) (void)hipFree(t->d_basis_mem);
   delete t;
@@ -597,6 +631,7 @@ extern "C" int fe_table_free(fe_table* t) {
 
 extern "C" int fe_table_info(const fe_table* t, int64_t* info, int n_info) {
   if (!t || !info) return fe_set_error(FE_ERR_ARG, "fe_table_info: null argument");
+  if (int rc = fe_table_sync_counts(const_cast<fe_table*>(t))) return rc;
   int64_t v[16] = {0};
   v[0] = t->n_pairs;
   v[1] = t->n_terms;
@@ -616,6 +651,7 @@ extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx,
   if (!t || !pp_exp || !pp_cx || !pp_cy || !pp_cz || !pp_K || !pp_nt || !pp_off || !pp_nmono ||
       !pp_monooff || !mo_parent || !mo_axis || !tm_aidx || !tm_m || !tm_g)
     return fe_set_error(FE_ERR_ARG, "fe_table_export: null argument");
+  if (int rc = fe_table_sync_counts(const_cast<fe_table*>(t))) return rc;
   FE_HIPCHK(hipSetDevice(t->ctx->device));
   // (orig position, class, index within class), sorted by original position
   std::vector<std::tuple<int32_t, int, int>> order;
@@ -626,7 +662,10 @@ extern "C" int fe_table_export(const fe_table* t, double* pp_exp, double* pp_cx,
     rec[L].resize((size_t)t->n_L[L] * fe_host::STRIDE[L]);
     FE_HIPCHK(hipMemcpy(rec[L].data(), t->d_rec[L], rec[L].size() * sizeof(double),
                         hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < t->n_L[L]; ++k) order.emplace_back(t->orig[L][k], L, (int)k);
+    std::vector<int32_t> orig(t->n_L[L]);
+    FE_HIPCHK(hipMemcpy(orig.data(), t->d_orig[L], orig.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < t->n_L[L]; ++k) order.emplace_back(orig[k], L, (int)k);
   }
   std::sort(order.begin(), order.end());
   int32_t toff = 0, moff = 0;
@@ -715,7 +754,7 @@ FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
   pl.ld = (int)((n + 1023) / 1024 * 1024);
   int rows = 0;
   for (int L = 0; L <= FE_LMAX; ++L) {
-    const int64_t nL = t->n_L[L];
+    const int64_t nL = fe_plan_n(t, L);
     pl.ppl[L] = (ppl[L] == 4 || ppl[L] == 2) ? ppl[L] : 1;
     const int per_block = FE_WG * pl.ppl[L];
     pl.n_pb[L] = (int)((n + per_block - 1) / per_block);

@@ -8,8 +8,9 @@
 
 template <int L, int PPL, bool GRAD, bool MIXED>
 __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_eval(const double* __restrict__ rec,
-                                                const float* __restrict__ g32, const int n_pairs,
-                                                const int ppc, const double* __restrict__ pts,
+                                                const float* __restrict__ g32,
+                                                const int* __restrict__ n_pairs_p,
+                                                const double* __restrict__ pts,
                                                 const int n_pts,
                                                 const double* __restrict__ boys_tab,
                                                 double* __restrict__ partial, const int nadd,
@@ -19,6 +20,10 @@ __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_eval(const doub
   __shared__ double s_tab[NTAB];
   for (int i = threadIdx.x; i < NTAB; i += FE_WG) s_tab[i] = boys_tab[i];
   __syncthreads();
+  // the kept count is read on the device (a rebuild may have changed it since
+  // the launch was planned); the chunks split it evenly
+  const int n_pairs = *n_pairs_p;
+  const int ppc = (n_pairs + (int)gridDim.y - 1) / (int)gridDim.y;
   const int chunk = blockIdx.y;
   const int p0 = chunk * ppc;
   const int p1 = min(n_pairs, p0 + ppc);
@@ -71,7 +76,7 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
   const int ph = fe_prof_begin(t->ctx, FE_PH_EVAL0 + L, st);
 #define FE_K(G, X)                                                                            \
   hipLaunchKernelGGL((k_eval<L, PPL, G, X>), grid, dim3(FE_WG), 0, st, t->d_rec[L], g32,      \
-                     (int)t->n_L[L], pl.ppc[L], pts, n, tab, partial, pl.nadd[L], pl.ld,      \
+                     t->d_stats + FE_ST_KEPT + L, pts, n, tab, partial, pl.nadd[L], pl.ld,    \
                      plane, tau)
   if (grad) {
     if (mixed) FE_K(true, true); else FE_K(true, false);

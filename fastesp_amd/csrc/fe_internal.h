@@ -43,6 +43,17 @@ __host__ __device__ constexpr int fe_dg_minw(int L) {
 
 int fe_set_error(int code, const char* fmt, ...);
 
+// Device primitives (fe_prim.hip): deterministic, no memsets, no host
+// synchronisation.  Stable LSD radix sort of (key, value) pairs on the low
+// key_bits bits, in place, with fe_radix_sort_scratch(n) bytes of scratch;
+// exclusive prefix sums (out may equal in) with fe_scan_scratch(n) bytes.
+size_t fe_radix_sort_scratch(int64_t n);
+int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scratch,
+                  hipStream_t st);
+size_t fe_scan_scratch(int64_t n);
+int fe_scan_excl_i32(const int* in, int* out, int n, void* scratch, hipStream_t st);
+int fe_scan_excl_u64(const unsigned long long* in, unsigned long long* out, int n,
+                     void* scratch, hipStream_t st);
 int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes);
 
 #define FE_HIPCHK(x)                                                                         \
@@ -142,16 +153,39 @@ struct fe_ctx {
 int fe_prof_begin(fe_ctx* c, int phase, hipStream_t st);
 void fe_prof_end(fe_ctx* c, int slot, hipStream_t st);
 
+// Table statistics on the device (FE_ST_* ints, then the sum rule as a double)
+enum {
+  FE_ST_KEPT = 0,                    // + L: kept pairs of class L (what the kernels read)
+  FE_ST_VALID = FE_LMAX + 1,         // + L: candidates of the reference's pair list
+  FE_ST_KVALID = 2 * (FE_LMAX + 1),  // + L: primitive pairs with K >= 1e-30 (rebuild capacity)
+  FE_ST_N = 3 * (FE_LMAX + 1),
+  FE_ST_SUMRULE_BYTE = 128           // byte offset of the sum rule (a double)
+};
+
 struct fe_table {
   fe_ctx* ctx = nullptr;
   int n_pairs = 0;
   int n_terms = 0;
   int n_atoms = 0;
   int64_t n_candidates = 0;
-  double* d_rec[FE_LMAX + 1] = {};    // per-L records (FP64)
+  // One allocation holds the statistics, the per-class records, their flat
+  // term offsets and reference-order indices, and (built tables) the slot map.
+  void* d_mem = nullptr;
+  size_t mem_bytes = 0;
+  int* d_stats = nullptr;             // FE_ST_N ints (kernels read the kept counts here)
+  double* d_sum_rule = nullptr;
+  double* d_rec[FE_LMAX + 1] = {};    // per-L records (FP64), cap_L[L] of them
   float* d_rec32[FE_LMAX + 1] = {};   // per-L g in FP32 (mixed precision), lazily built
-  int64_t n_L[FE_LMAX + 1] = {};
-  std::vector<int32_t> orig[FE_LMAX + 1];  // flat-table index of every record
+  int64_t n_L[FE_LMAX + 1] = {};      // kept pairs per class (host copy, see counts_stale)
+  int64_t cap_L[FE_LMAX + 1] = {};    // records allocated per class
+  int32_t* d_orig[FE_LMAX + 1] = {};  // reference-order (flat-table) index of every record
+  int* d_flatoff[FE_LMAX + 1] = {};   // term offset of every record in the flat layout
+  // A rebuild (fe_table_rebuild) leaves the new counts on the device only:
+  // host-side plans then size launches by cap_L and the kernels read the true
+  // counts; host queries read them back first (fe_table_sync_counts).
+  bool counts_stale = false;
+  bool capacity_layout = false;       // records for every K-valid candidate
+  hipStream_t count_stream = nullptr;
   double* d_atoms = nullptr;          // (x, y, z, Z) per atom
   std::vector<double> h_atoms;        // host copy of d_atoms
   size_t dev_bytes = 0;
@@ -166,14 +200,33 @@ struct fe_table {
     const int* atom; const int* l; const int* fn0; const int* nfn; const int* prim0;
     const int* nprim; const double* c; const double* pexp; const double* pcoef;
     const int* fn_lx; const int* fn_ly; const int* fn_lz;
-    const int* sp_i; const int* sp_j; const int* sp_slot;
+    const int* sp_i; const int* sp_j; const int* sp_slot; const int* sp_gidx;
   } b{};
-  int* d_flatoff[FE_LMAX + 1] = {};   // term offset of every record in the flat layout
-  int64_t n_cand_L[FE_LMAX + 1] = {};
+  int64_t n_cand_L[FE_LMAX + 1] = {}; // candidate slots (every primitive pair) per class
+  int64_t kvalid_L[FE_LMAX + 1] = {}; // K-valid slots per class
   int* d_slot2kept[FE_LMAX + 1] = {}; // candidate slot -> compact record (-1 dropped)
   std::vector<int> h_sp_i, h_sp_j;    // shell pair list (host)
   std::vector<int64_t> h_sp_slot;
+  // pair-order key frame: bounding box of the shell centres, and per class the
+  // log range of the candidate exponents p
+  double key_lo[3] = {0, 0, 0};
+  double key_scale = 0.0;
+  double key_lpmin[FE_LMAX + 1] = {}, key_lpinv[FE_LMAX + 1] = {};
 };
+
+// pairs of class L that launches are sized for: the kept count when the host
+// knows it, else the allocated capacity (the kernels read the kept count)
+inline int64_t fe_plan_n(const fe_table* t, int L) {
+  return t->counts_stale ? t->cap_L[L] : t->n_L[L];
+}
+// one device allocation for a table: statistics, records (cap[L] per class),
+// flat offsets and reference-order indices, plus n_slots ints of slot map
+int fe_table_layout(fe_table* t, const int64_t* cap, int64_t n_slots);
+// read the device statistics back after a rebuild (synchronises its stream)
+int fe_table_sync_counts(fe_table* t);
+// the FP32 coefficients again after the records changed (no-op without them)
+int fe_refresh_g32(fe_table* t, hipStream_t st);
+
 
 // Partial sums: pair chunk c of every class adds into row c.  The classes run
 // in the order 0..FE_LMAX on one stream; class L writes rows [nadd[L],
@@ -192,8 +245,6 @@ struct FwdPlan {
 
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);
 int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* xyz);
-int fe_table_alloc_classes(fe_table* t, const std::vector<double>* rec);
-int fe_table_index(fe_table* t);  // flat term offsets of the records (from orig order)
 // Record order within a class: 16 buckets of log p over the class's range, then
 // the 30-bit Morton code of P in the bounding box of the records.  Pairs whose
 // near-field radii are alike then share a wavefront in the pair-per-lane

@@ -36,28 +36,42 @@ __global__ __launch_bounds__(256) void k_make_g32(const double* __restrict__ rec
 
 
 // ---------------------------------------------------------------------------
+// FP32 copies of every allocated record of class L (records past the kept
+// count are converted too and never read)
+static void launch_make_g32(fe_table* t, int L, hipStream_t st) {
+  const int S32 = FE_S32(fe_host::NTERM[L] + fe_host::NDCOEF[L]);
+  const int64_t ne = t->cap_L[L] * S32;
+  const dim3 grid((unsigned)((ne + 255) / 256));
+  switch (L) {
+#define FE_G32(l)                                                                             \
+  case l:                                                                                     \
+    hipLaunchKernelGGL(k_make_g32<l>, grid, dim3(256), 0, st, t->d_rec[l], (int)t->cap_L[l], \
+                       t->d_rec32[l]);                                                        \
+    break;
+    FE_G32(0) FE_G32(1) FE_G32(2) FE_G32(3) FE_G32(4) FE_G32(5) FE_G32(6)
+#undef FE_G32
+  }
+}
+
 int fe_ensure_g32(fe_table* t, hipStream_t st) {
   for (int L = 0; L <= FE_LMAX; ++L) {
-    if (!t->n_L[L] || t->d_rec32[L]) continue;
+    if (!t->cap_L[L] || t->d_rec32[L]) continue;
     const int S32 = FE_S32(fe_host::NTERM[L] + fe_host::NDCOEF[L]);
-    const size_t b = (size_t)t->n_L[L] * S32 * sizeof(float);
+    const size_t b = (size_t)t->cap_L[L] * S32 * sizeof(float);
     if (hipMalloc(&t->d_rec32[L], b) != hipSuccess) {
       (void)hipGetLastError();
       return fe_set_error(FE_ERR_ALLOC, "mixed: FP32 table allocation failed");
     }
     t->dev_bytes += b;
-    const int64_t ne = t->n_L[L] * S32;
-    const dim3 grid((unsigned)((ne + 255) / 256));
-    switch (L) {
-#define FE_G32(l)                                                                             \
-  case l:                                                                                     \
-    hipLaunchKernelGGL(k_make_g32<l>, grid, dim3(256), 0, st, t->d_rec[l], (int)t->n_L[l],   \
-                       t->d_rec32[l]);                                                        \
-    break;
-      FE_G32(0) FE_G32(1) FE_G32(2) FE_G32(3) FE_G32(4) FE_G32(5) FE_G32(6)
-#undef FE_G32
-    }
+    launch_make_g32(t, L, st);
   }
+  FE_HIPCHK(hipGetLastError());
+  return FE_OK;
+}
+
+int fe_refresh_g32(fe_table* t, hipStream_t st) {
+  for (int L = 0; L <= FE_LMAX; ++L)
+    if (t->cap_L[L] && t->d_rec32[L]) launch_make_g32(t, L, st);
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }

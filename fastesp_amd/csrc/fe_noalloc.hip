@@ -48,7 +48,7 @@ extern "C" int fe_esp_forward_noalloc(fe_table* t, const double* pts, int64_t n,
     return fe_set_error(FE_ERR_STATE, "fe_esp_forward_noalloc: context has no scratch");
   if (precision == FE_PREC_MIXED)
     for (int L = 0; L <= FE_LMAX; ++L)
-      if (t->n_L[L] && !t->d_rec32[L])
+      if (t->cap_L[L] && !t->d_rec32[L])
         return fe_set_error(FE_ERR_STATE,
                             "fe_esp_forward_noalloc: the FP32 coefficients of this table are not "
                             "built yet (run fe_esp_forward once with FE_PREC_MIXED)");

@@ -8,7 +8,6 @@
 // Morton curve makes every wavefront a compact surface patch, so the branch is
 // nearly always uniform.  Results are scattered back to the caller's order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -54,9 +53,11 @@ __global__ __launch_bounds__(256) void k_gather_pts(const double* __restrict__ p
 }
 
 // Sorted copy of the points and the permutation (sorted position -> caller
-// index), both in the context's FE_WS_SORT slot.  The quantisation frame is the
-// nuclei's bounding box grown by 16 Bohr (points outside are clamped, which
-// only costs locality, never correctness).
+// index), in the context's FE_WS_SORT slot or the caller's buffer.  The
+// quantisation frame is the nuclei's bounding box grown by 16 Bohr (points
+// outside are clamped, which only costs locality, never correctness).  The
+// sort is the library's own stable radix sort (fe_prim.hip): no memsets and no
+// host synchronisation, so the sort can be recorded into a hipGraph.
 int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hipStream_t st,
                    const double** d_sorted, const int32_t** d_perm, void* buf, size_t buf_bytes) {
   double lo[3] = {-16.0, -16.0, -16.0}, hi[3] = {16.0, 16.0, 16.0};
@@ -77,14 +78,9 @@ int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hip
   }
   const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
   const double scale = 1023.0 / ext;
-  size_t temp_bytes = 0;
-  FE_HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (uint32_t*)nullptr,
-                                               (uint32_t*)nullptr, (int32_t*)nullptr,
-                                               (int32_t*)nullptr, n, 0, 30, st));
-  const size_t al = 256;
-  auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t b_key = up((size_t)n * 4), b_pts = up((size_t)n * 3 * sizeof(double));
-  const size_t need = 4 * b_key + b_pts + up(temp_bytes);
+  const size_t need = 2 * b_key + b_pts + fe_radix_sort_scratch(n);
   char* base = (char*)buf;
   if (buf) {
     if (buf_bytes < need)
@@ -95,20 +91,17 @@ int fe_sort_points(fe_ctx* c, const fe_table* t, const double* d_pts, int n, hip
     if (rc) return rc;
     base = (char*)c->ws[FE_WS_SORT];
   }
-  uint32_t* k_in = (uint32_t*)base;
-  uint32_t* k_out = (uint32_t*)(base + b_key);
-  int32_t* v_in = (int32_t*)(base + 2 * b_key);
-  int32_t* v_out = (int32_t*)(base + 3 * b_key);
-  double* sorted = (double*)(base + 4 * b_key);
-  void* temp = base + 4 * b_key + b_pts;
+  uint32_t* key = (uint32_t*)base;
+  int32_t* perm = (int32_t*)(base + b_key);
+  double* sorted = (double*)(base + 2 * b_key);
+  void* tmp = base + 2 * b_key + b_pts;
   const dim3 grid((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL(k_morton, grid, dim3(256), 0, st, d_pts, n, lo[0], lo[1], lo[2], scale, k_in,
-                     v_in);
-  FE_HIPCHK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k_in, k_out, v_in, v_out, n, 0,
-                                               30, st));
-  hipLaunchKernelGGL(k_gather_pts, grid, dim3(256), 0, st, d_pts, v_out, n, sorted);
+  hipLaunchKernelGGL(k_morton, grid, dim3(256), 0, st, d_pts, n, lo[0], lo[1], lo[2], scale, key,
+                     perm);
+  if (int rc = fe_radix_sort(key, perm, n, 30, tmp, st)) return rc;
+  hipLaunchKernelGGL(k_gather_pts, grid, dim3(256), 0, st, d_pts, perm, n, sorted);
   FE_HIPCHK(hipGetLastError());
   *d_sorted = sorted;
-  *d_perm = v_out;
+  *d_perm = perm;
   return FE_OK;
 }

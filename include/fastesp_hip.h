@@ -122,6 +122,25 @@ int fe_table_build(fe_ctx* ctx, int n_func, int n_prim, const int32_t* fn_atom,
                    const double* xyz, double tol, int rule, int ptr_kind,
                    void* stream, fe_table** out);
 
+/* Build a table made by fe_table_build again, in place, for a new density
+ * and/or new points (same basis, same rule semantics as fe_table_build).  No
+ * counterpart in the reference, which rebuilds from scratch
+ * (fastesp/cli.py:143-163); it exists so that a build can run where a host
+ * round trip or an allocation cannot: in a differentiable training loop or
+ * under HIP graph capture.
+ *  - The first call resizes the table to hold every primitive pair with
+ *    K >= 1e-30 (a bound for any density and points), which allocates and
+ *    synchronises; later calls allocate nothing and never synchronise with
+ *    the host (device pointers), so they can be captured.
+ *  - The context's build workspace is reused; it grows (allocates) only when
+ *    n_points exceeds every earlier build's.
+ *  - The new counts stay on the device: evaluations read them there, and
+ *    host queries (fe_table_info, fe_table_export, fe_table_stats, a dL/dG
+ *    output of fe_esp_backward, fe_batch_create) wait for the rebuild and read
+ *    them back.  A batch made from the table must not outlive a rebuild. */
+int fe_table_rebuild(fe_table* t, const double* dens, int n_points, const double* pts,
+                     double tol, int rule, int ptr_kind, void* stream);
+
 /* Floating-point statistics of a table: out[0] the sum rule
  * sum_pairs K (pi/p)^{3/2} etab_0 = Tr(P S) over all candidate primitive
  * pairs (printed by build_pairs.cpp:423), out[1] kept pairs, out[2] basis
