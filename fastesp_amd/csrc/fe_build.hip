@@ -274,13 +274,41 @@ __device__ __forceinline__ bool fe_pass(const double d2, const double kb, const 
   return !(kb / dd < tol);
 }
 
+// The largest non-negative double T with fe_pass(T) (-1 if none passes), so
+// that fe_pass(d2) == (d2 <= T) for every d2 >= 0: fe_pass is monotone, and
+// the bit patterns of non-negative doubles are ordered like their values, so
+// a bisection over the patterns finds the switch exactly (63 steps).
+__device__ double fe_pass_threshold(const double kb, const double reff, const double tol,
+                                    const int rule) {
+  if (!fe_pass(0.0, kb, reff, tol, rule)) return -1.0;
+  unsigned long long lo = 0ull, hi = 0x7FF0000000000000ull;  // pass(lo), +inf
+  if (fe_pass(INFINITY, kb, reff, tol, rule)) return INFINITY;
+  while (hi - lo > 1ull) {
+    const unsigned long long mid = lo + (hi - lo) / 2;
+    if (fe_pass(__longlong_as_double((long long)mid), kb, reff, tol, rule)) lo = mid;
+    else hi = mid;
+  }
+  return __longlong_as_double((long long)lo);
+}
+
+// lane l's value of a double (l wave-uniform)
+__device__ __forceinline__ double fe_readlane(const double v, const int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 struct ScreenArgs {
   const double* cand[FE_LMAX + 1];
   ClassInfo ci;
   const int* gidx;
   unsigned long long* gflag;  // per reference-order index: (nterm << 32 | 1) if kept, else 0
-  uint32_t* key;
+  uint32_t* key;              // final pair-order keys and candidate indices
   int32_t* val;
+  uint32_t* skey;             // screening order: keys and candidate indices
+  int32_t* sval;
+  double* thr;                // per candidate: the squared-distance threshold
   const double* pts;  // Morton-sorted
   int n_pts;
   const double* box;
@@ -290,52 +318,127 @@ struct ScreenArgs {
   double lo[3], kscale;  // Morton frame of P (8 bits per axis)
   double lpmin[FE_LMAX + 1], lpinv[FE_LMAX + 1];  // exponent buckets: 16 over the log range
   int* stats;  // FE_ST_* counters, zero before the launch
+  int* n_open;  // candidates left to the box walk (counted by k_screen_prep)
   int64_t n_all;
 };
 
-__global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
-  __shared__ double s_box[FE_BOX_CHUNK * 6];
+__device__ __forceinline__ int fe_class_of(const ClassInfo& ci, const int64_t e) {
+  int L = 0;
+  while (L < FE_LMAX && e >= ci.cand_off[L + 1]) ++L;
+  return L;
+}
+
+__device__ __forceinline__ uint32_t fe_morton24(const double* P, const double* lo,
+                                                const double scale) {
+  uint32_t code = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const uint32_t q = (uint32_t)fmin(fmax((P[ax] - lo[ax]) * scale, 0.0), 255.0);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) code |= ((q >> bit) & 1u) << (3 * bit + ax);
+  }
+  return code;
+}
+
+// The screen's outputs for candidate e of class L: its pair-order key (class,
+// dropped flag, exponent bucket, Morton code of P) and its reference-order
+// flag (term count and 1 when kept, else 0).
+__device__ __forceinline__ void fe_screen_emit(const ScreenArgs& a, const int64_t e, const int L,
+                                               const bool kept, const double* r) {
+  uint32_t k = (uint32_t)L << 29;
+  if (!kept) {
+    k |= 1u << 28;
+  } else {
+    const int bucket = (int)fmin(fmax((log(r[3]) - a.lpmin[L]) * a.lpinv[L], 0.0), 15.0);
+    k |= ((uint32_t)bucket << 24) | fe_morton24(r, a.lo, a.kscale);
+  }
+  a.key[e] = k;
+  a.val[e] = (int32_t)e;
+  a.gflag[a.gidx[e]] = kept ? (((unsigned long long)a.ci.nterm[L] << 32) | 1ull) : 0ull;
+}
+
+// per-block counters of kept candidates, added to the statistics at the end
+__device__ __forceinline__ void fe_flush_counts(int* s_cnt, int* stats) {
+  __syncthreads();
+  if (threadIdx.x < FE_ST_N && s_cnt[threadIdx.x]) atomicAdd(&stats[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+// Every candidate slot: the counters of candidates and K-valid pairs, the
+// squared-distance threshold of the test, and either the final decision (no
+// candidate, rule 1's G/coefficient test, no points, or a threshold that
+// settles every distance) or a screening key that groups candidates with
+// alike thresholds and nearby P, so the box walk of a wavefront is coherent.
+__global__ __launch_bounds__(256) void k_screen_prep(const ScreenArgs a) {
   __shared__ int s_cnt[FE_ST_N];
   if (threadIdx.x < FE_ST_N) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool in = e < a.n_all;
-  int L = 0;
-  while (L < FE_LMAX && e >= a.ci.cand_off[L + 1]) ++L;
-  const double* r = in ? a.cand[L] + (size_t)(e - a.ci.cand_off[L]) * a.ci.stride[L] : nullptr;
-  const int flags = in ? (int)r[7] : 0;
-  const bool valid = flags & 1;
-  double Px = 0.0, Py = 0.0, Pz = 0.0, kb = 0.0, reff = 0.0;
-  if (valid) {
-    Px = r[0];
-    Py = r[1];
-    Pz = r[2];
-    kb = r[5];
-    reff = r[6];
-  }
-  // rules 0, 1: kept iff some point passes (the event is a pass); rule 2: kept
-  // iff every point passes (the event is a fail).  Without an event the
-  // decision is the opposite of the event's.
-  const bool any_mode = a.rule != 2;
-  const double tol = a.tol;
-  const int rule = a.rule;
-  bool done = !valid, keep = false;
-  if (valid) {
-    if (rule == 1 && !((flags & 2) && (flags & 4))) {
-      done = true;
-    } else if (a.n_pts == 0) {
-      done = true;  // as if dmin = +inf (rules 0, 1) or dmax = 0 (rule 2)
-      keep = fe_pass(any_mode ? INFINITY : 0.0, kb, reff, tol, rule);
-    } else {
-      keep = !any_mode;
+  if (e < a.n_all) {
+    const int L = fe_class_of(a.ci, e);
+    const double* r = a.cand[L] + (size_t)(e - a.ci.cand_off[L]) * a.ci.stride[L];
+    const int flags = (int)r[7];
+    const bool valid = flags & 1;
+    if (flags & 8) atomicAdd(&s_cnt[FE_ST_KVALID + L], 1);
+    if (valid) atomicAdd(&s_cnt[FE_ST_VALID + L], 1);
+    const bool any_mode = a.rule != 2;
+    int decided = valid ? -1 : 0;  // -1 open, 0 dropped, 1 kept
+    double thr = -1.0;
+    if (valid) {
+      const double kb = r[5], reff = r[6];
+      if (a.rule == 1 && !((flags & 2) && (flags & 4))) {
+        decided = 0;
+      } else if (a.n_pts == 0) {
+        // as if dmin = +inf (rules 0, 1) or dmax = 0 (rule 2)
+        decided = fe_pass(any_mode ? INFINITY : 0.0, kb, reff, a.tol, a.rule) ? 1 : 0;
+      } else {
+        thr = fe_pass_threshold(kb, reff, a.tol, a.rule);
+        if (thr < 0.0) decided = 0;               // no distance passes
+        else if (thr == INFINITY) decided = 1;    // every distance passes
+      }
     }
+    a.thr[e] = thr;
+    if (decided >= 0) {
+      fe_screen_emit(a, e, L, decided == 1, r);
+      if (decided == 1) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
+      a.skey[e] = 0xFFFFFFFFu;  // sorts after every open candidate
+    } else {
+      atomicAdd(a.n_open, 1);
+      // 254 at most: the key 0xFFFFFFFF marks a decided candidate
+      const int tb = (int)fmin(fmax(floor(log2(thr)) + 128.0, 0.0), 254.0);
+      a.skey[e] = ((uint32_t)tb << 24) | fe_morton24(r, a.lo, a.kscale);
+    }
+    a.sval[e] = (int32_t)e;
   }
-  for (int b0 = 0; b0 < a.n_box; b0 += FE_BOX_CHUNK) {
-    if (__syncthreads_and(done)) break;
-    const int nb = min(FE_BOX_CHUNK, a.n_box - b0);
-    for (int q = threadIdx.x; q < 6 * nb; q += 256) s_box[q] = a.box[6 * (size_t)b0 + q];
-    __syncthreads();
-    for (int j = 0; j < nb && !done; ++j) {
-      const double* bx = s_box + 6 * j;
+  fe_flush_counts(s_cnt, a.stats);
+}
+
+// The open candidates, one wavefront each (persistent: wavefront w takes the
+// sorted positions w, w + W, ... below the open count).  The rules 0, 1 keep
+// a candidate iff some point passes the test (the event is a pass); rule 2 iff
+// every point passes (the event is a fail); without an event the decision is
+// the opposite.  The lanes take 64 boxes at a time: a box whose distance
+// bounds settle the test for all its points decides at once; each lane whose
+// box is ambiguous tests that box's points itself.  So no candidate's work
+// runs on a single lane, and the slowest candidate costs a few box steps.
+__global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
+  __shared__ int s_cnt[FE_ST_N];
+  if (threadIdx.x < FE_ST_N) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int n_open = *a.n_open;
+  const int W = (int)gridDim.x * 4;
+  const bool any_mode = a.rule != 2;
+  for (int i = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < n_open; i += W) {
+    const int64_t e = a.sval[i];
+    const int L = fe_class_of(a.ci, e);
+    const double* r = a.cand[L] + (size_t)(e - a.ci.cand_off[L]) * a.ci.stride[L];
+    const double Px = r[0], Py = r[1], Pz = r[2];
+    const double thr = a.thr[e];  // fe_pass(d2) == (d2 <= thr)
+    bool keep = !any_mode;
+    for (int b0 = 0; b0 < a.n_box; b0 += 64) {
+      const int b = b0 + lane;
+      const bool vb = b < a.n_box;
+      const double* bx = a.box + 6 * (size_t)(vb ? b : 0);
       // bounds of every point's squared distance, widened by 1e-12 relative
       // so that rounding cannot move a point across them
       const double gx = fmax(fmax(bx[0] - Px, Px - bx[3]), 0.0);
@@ -346,57 +449,38 @@ __global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
       const double hz = fmax(Pz - bx[2], bx[5] - Pz);
       const double lo2 = fma(gx, gx, fma(gy, gy, gz * gz)) * (1.0 - 1e-12);
       const double hi2 = fma(hx, hx, fma(hy, hy, hz * hz)) * (1.0 + 1e-12) + 1e-300;
-      const bool pass_far = fe_pass(hi2, kb, reff, tol, rule);  // every point passes
-      const bool pass_near = fe_pass(lo2, kb, reff, tol, rule); // some point may pass
-      if (any_mode ? pass_far : !pass_near) {
+      const bool pass_far = hi2 <= thr;   // every point of the box passes
+      const bool pass_near = lo2 <= thr;  // some point of the box may pass
+      // rules 0/1: a box whose every point passes keeps; rule 2: a box whose
+      // every point fails drops
+      if (__any(vb && (any_mode ? pass_far : !pass_near))) {
         keep = any_mode;
-        done = true;
         break;
       }
-      if (any_mode ? !pass_near : pass_far) continue;
-      const int q0 = (b0 + j) * FE_BOX, q1 = min(a.n_pts, q0 + FE_BOX);
-      for (int q = q0; q < q1; ++q) {
-        const double x = a.pts[3 * (size_t)q] - Px;
-        const double y = a.pts[3 * (size_t)q + 1] - Py;
-        const double z = a.pts[3 * (size_t)q + 2] - Pz;
-        if (fe_pass(fma(x, x, fma(y, y, z * z)), kb, reff, tol, rule) == any_mode) {
-          keep = any_mode;
-          done = true;
-          break;
+      const bool amb = vb && pass_near && !pass_far;
+      if (!__any(amb)) continue;
+      bool hit = false;  // a point of this lane's box passes (rules 0/1) or fails (rule 2)
+      if (amb) {
+        const int q0 = b * FE_BOX, q1 = min(a.n_pts, q0 + FE_BOX);
+#pragma unroll 8
+        for (int q = q0; q < q1; ++q) {
+          const double x = a.pts[3 * (size_t)q] - Px;
+          const double y = a.pts[3 * (size_t)q + 1] - Py;
+          const double z = a.pts[3 * (size_t)q + 2] - Pz;
+          hit |= (fma(x, x, fma(y, y, z * z)) <= thr) == any_mode;
         }
       }
-    }
-  }
-  if (in) {
-    const bool kept = valid && keep;
-    uint32_t k = (uint32_t)L << 29;
-    if (!kept) {
-      k |= 1u << 28;
-    } else {
-      const double p = r[3];
-      const int bucket = (int)fmin(fmax((log(p) - a.lpmin[L]) * a.lpinv[L], 0.0), 15.0);
-      uint32_t code = 0;
-      const double P[3] = {Px, Py, Pz};
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax) {
-        const uint32_t q = (uint32_t)fmin(fmax((P[ax] - a.lo[ax]) * a.kscale, 0.0), 255.0);
-        for (int bit = 0; bit < 8; ++bit) code |= ((q >> bit) & 1u) << (3 * bit + ax);
+      if (__any(hit)) {
+        keep = any_mode;
+        break;
       }
-      k |= ((uint32_t)bucket << 24) | code;
     }
-    a.key[e] = k;
-    a.val[e] = (int32_t)e;
-    a.gflag[a.gidx[e]] = kept ? (((unsigned long long)a.ci.nterm[L] << 32) | 1ull) : 0ull;
+    if (lane == 0) {
+      fe_screen_emit(a, e, L, keep, r);
+      if (keep) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
+    }
   }
-  __syncthreads();
-  if (in) {
-    if (valid && keep) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
-    if (valid) atomicAdd(&s_cnt[FE_ST_VALID + L], 1);
-    if (flags & 8) atomicAdd(&s_cnt[FE_ST_KVALID + L], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < FE_ST_N && s_cnt[threadIdx.x])
-    atomicAdd(&a.stats[threadIdx.x], s_cnt[threadIdx.x]);
+  fe_flush_counts(s_cnt, a.stats);
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, const int64_t n, const int v) {
@@ -404,17 +488,29 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, const int
   if (i < n) a[i] = v;
 }
 
-// sum rule: the shell-pair partials added in a fixed order (one block), and
-// the counters copied into the table's statistics
-__global__ __launch_bounds__(256) void k_build_stats(const double* __restrict__ srule, const int nsp,
+// Sum rule in two fixed-order steps: block b adds the shell-pair partials
+// [b * per, (b + 1) * per) (k_srule_part), then one block adds the block sums
+// and copies the counters into the table's statistics (k_build_stats).
+#define FE_SRULE_BLOCKS 256
+__global__ __launch_bounds__(256) void k_srule_part(const double* __restrict__ srule, const int nsp,
+                                                    double* __restrict__ part) {
+  __shared__ double s_w[4];
+  const int per = (nsp + FE_SRULE_BLOCKS - 1) / FE_SRULE_BLOCKS;
+  const int i0 = blockIdx.x * per, i1 = min(nsp, i0 + per);
+  double s = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) s += srule[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+__global__ __launch_bounds__(256) void k_build_stats(const double* __restrict__ part,
                                                      const int* __restrict__ ws_stats,
                                                      int* __restrict__ stats,
                                                      double* __restrict__ sum_rule) {
   __shared__ double s_w[4];
-  const int per = (nsp + 255) / 256;
-  const int i0 = threadIdx.x * per, i1 = min(nsp, i0 + per);
-  double s = 0.0;
-  for (int i = i0; i < i1; ++i) s += srule[i];
+  double s = threadIdx.x < FE_SRULE_BLOCKS ? part[threadIdx.x] : 0.0;
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -526,17 +622,28 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(int32_t));
   const size_t o_gflag = wo;
   wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(unsigned long long));
+  const size_t o_skey = wo;
+  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(uint32_t));
+  const size_t o_sval = wo;
+  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(int32_t));
+  const size_t o_thr = wo;
+  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(double));
   const size_t o_srule = wo;
   wo += al256((size_t)std::max(nsp, 1) * sizeof(double));
-  const size_t o_stats = wo;
+  const size_t o_stats = wo;  // FE_ST_N counters, the open count, the sum rule
   wo += al256(FE_ST_SUMRULE_BYTE + sizeof(double));
+  const size_t o_spart = wo;
+  wo += al256(FE_SRULE_BLOCKS * sizeof(double));
   const size_t o_box = wo;
   wo += al256((size_t)std::max(n_box, 1) * 6 * sizeof(double));
-  const size_t o_sort = wo;  // point sort, then the candidate sort and scan
+  // the point sort (its sorted copy is read by the screen), then the
+  // candidate sorts and the scan
+  const size_t o_psort = wo;
   const size_t b_key = al256((size_t)n_points * 4), b_pts = al256((size_t)n_points * 24);
   const size_t sort_pts = 2 * b_key + b_pts + fe_radix_sort_scratch(n_points);
-  const size_t sort_cand = std::max(fe_radix_sort_scratch(n_all), fe_scan_scratch(n_all));
-  wo += al256(std::max(sort_pts, sort_cand));
+  wo += al256(sort_pts);
+  const size_t o_sort = wo;
+  wo += al256(std::max(fe_radix_sort_scratch(n_all), fe_scan_scratch(n_all)));
   if (int rc = fe_ctx_reserve(c, FE_WS_BUILD, wo)) return rc;
   char* wb = (char*)c->ws[FE_WS_BUILD];
   int* ws_stats = (int*)(wb + o_stats);
@@ -554,7 +661,7 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
       d_pts = (const double*)(wb + o_pts);
     }
   }
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, ws_stats, (int64_t)FE_ST_N, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, st, ws_stats, (int64_t)FE_ST_N + 1, 0);
 
   // ---- candidates
   BuildArgs ba{};
@@ -576,7 +683,7 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   const double* sorted = nullptr;
   if (n_points > 0) {
     const int32_t* perm = nullptr;
-    if (int rc = fe_sort_points(c, t, d_pts, n_points, st, &sorted, &perm, wb + o_sort, sort_pts))
+    if (int rc = fe_sort_points(c, t, d_pts, n_points, st, &sorted, &perm, wb + o_psort, sort_pts))
       return rc;
     hipLaunchKernelGGL(k_boxes, dim3(n_box), dim3(64), 0, st, sorted, n_points,
                        (double*)(wb + o_box));
@@ -597,6 +704,9 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   sa.gflag = gflag;
   sa.key = key;
   sa.val = val;
+  sa.skey = (uint32_t*)(wb + o_skey);
+  sa.sval = (int32_t*)(wb + o_sval);
+  sa.thr = (double*)(wb + o_thr);
   sa.pts = sorted;
   sa.n_pts = n_points;
   sa.box = (const double*)(wb + o_box);
@@ -606,19 +716,27 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   for (int a = 0; a < 3; ++a) sa.lo[a] = t->key_lo[a];
   sa.kscale = t->key_scale;
   sa.stats = ws_stats;
+  sa.n_open = ws_stats + FE_ST_N;
   sa.n_all = n_all;
   if (n_all > 0) {
-    hipLaunchKernelGGL(k_screen_boxes, dim3((unsigned)((n_all + 255) / 256)), dim3(256), 0, st,
-                       sa);
+    const dim3 g((unsigned)((n_all + 255) / 256));
+    hipLaunchKernelGGL(k_screen_prep, g, dim3(256), 0, st, sa);
+    if (int rc = fe_radix_sort(sa.skey, sa.sval, (int)n_all, 32, wb + o_sort, st)) return rc;
+    // persistent: 16 workgroups (64 wavefronts) per CU walk the open candidates
+    hipLaunchKernelGGL(k_screen_boxes, dim3((unsigned)std::max(1, c->n_cu * 16)), dim3(256), 0,
+                       st, sa);
     if (int rc = fe_radix_sort(key, val, (int)n_all, 32, wb + o_sort, st)) return rc;
     if (int rc = fe_scan_excl_u64(gflag, gflag, (int)n_all, wb + o_sort, st)) return rc;
   }
   FE_HIPCHK(hipGetLastError());
 
+  double* spart = (double*)(wb + o_spart);
+  hipLaunchKernelGGL(k_srule_part, dim3(FE_SRULE_BLOCKS), dim3(256), 0, st, ba.sp_srule, nsp,
+                     spart);
   if (!sized) {
     // the one synchronisation of a build: the counts size the table
-    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(256), 0, st, ba.sp_srule, nsp, ws_stats,
-                       (int*)nullptr, (double*)(wb + o_stats + FE_ST_SUMRULE_BYTE));
+    hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(256), 0, st, spart, ws_stats, (int*)nullptr,
+                       (double*)(wb + o_stats + FE_ST_SUMRULE_BYTE));
     int hs[FE_ST_N];
     double sr = 0.0;
     FE_HIPCHK(hipMemcpyAsync(hs, ws_stats, sizeof hs, hipMemcpyDeviceToHost, st));
@@ -640,8 +758,8 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
     t->sum_rule = sr;
     if (int rc = fe_table_layout(t, cap, n_all)) return rc;
   }
-  hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(256), 0, st, ba.sp_srule, nsp, ws_stats,
-                     t->d_stats, t->d_sum_rule);
+  hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(256), 0, st, spart, ws_stats, t->d_stats,
+                     t->d_sum_rule);
 
   // ---- compaction into the table, in key order
   for (int L = 0; L <= FE_LMAX; ++L) {
@@ -665,25 +783,54 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
 }
 }  // namespace
 
-extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
-                              const int32_t* fn_nprim, const int32_t* fn_off,
-                              const int32_t* fn_lx, const int32_t* fn_ly, const int32_t* fn_lz,
-                              const int32_t* fn_shell, const double* fc, const double* pexp,
-                              const double* pcoef, const double* dens, int n_points,
-                              const double* pts, int n_atoms, const double* Z, const double* xyz,
-                              double tol, int rule, int ptr_kind, void* stream, fe_table** out) {
-  // arguments first (no device needed to reject them)
-  if (out) *out = nullptr;
-  if (n_func <= 0 || n_prim <= 0 || n_atoms < 0)
-    return fe_set_error(FE_ERR_ARG, "fe_table_build: bad sizes");
-  if (!fn_atom || !fn_nprim || !fn_off || !fn_lx || !fn_ly || !fn_lz || !fn_shell || !fc ||
-      !pexp || !pcoef || (n_atoms && (!Z || !xyz)))
-    return fe_set_error(FE_ERR_ARG, "fe_table_build: null array");
-  if (int rc = check_build_args(n_points, pts, dens, tol, rule, ptr_kind, "fe_table_build"))
-    return rc;
-  if (!c || !out) return fe_set_error(FE_ERR_ARG, "fe_table_build: null argument");
+namespace {
+// A basis set up for the device: the shells, the shell pairs (si <= sj in
+// reference order) with their candidate slots and reference-order indices,
+// the pair-order key frame, and one device block holding all of it.  The
+// context keeps the last one, so builds that pass the same basis arrays again
+// (new densities or points) skip the host set-up and the upload.
+struct BasisSetup {
+  int device = 0;
+  std::vector<int32_t> ikey;  // the input arrays: the cache key
+  std::vector<double> dkey;
+  int n_func = 0, n_prim = 0, nsh = 0, nsp = 0;
+  int64_t ncand[FE_LMAX + 1] = {};
+  double lpmin[FE_LMAX + 1] = {}, lpinv[FE_LMAX + 1] = {};
+  double lo[3] = {0, 0, 0}, scale = 0.0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  size_t o_atom = 0, o_l = 0, o_fn0 = 0, o_nfn = 0, o_p0 = 0, o_np = 0, o_c = 0, o_pexp = 0,
+         o_pcoef = 0, o_lx = 0, o_ly = 0, o_lz = 0, o_spi = 0, o_spj = 0, o_sps = 0, o_spg = 0;
+  ~BasisSetup() {
+    if (mem) {
+      (void)hipSetDevice(device);
+      (void)hipFree(mem);
+    }
+  }
+};
 
-  // ---- shells (contiguous runs of equal fn_shell, build_pairs.cpp:170-182)
+void basis_key(int n_func, int n_prim, const int32_t* const* iv, const double* fc,
+               const double* pexp, const double* pcoef, std::vector<int32_t>& ik,
+               std::vector<double>& dk) {
+  ik.clear();
+  dk.clear();
+  ik.reserve((size_t)7 * n_func + 2);
+  ik.push_back(n_func);
+  ik.push_back(n_prim);
+  for (int k = 0; k < 7; ++k) ik.insert(ik.end(), iv[k], iv[k] + n_func);
+  dk.reserve((size_t)3 * n_func + 2 * n_prim);
+  dk.insert(dk.end(), fc, fc + 3 * (size_t)n_func);
+  dk.insert(dk.end(), pexp, pexp + n_prim);
+  dk.insert(dk.end(), pcoef, pcoef + n_prim);
+}
+
+// shells (contiguous runs of equal fn_shell, build_pairs.cpp:170-182), shell
+// pairs and the device upload; FE_ERR_* on a malformed basis
+int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
+                const int32_t* fn_nprim, const int32_t* fn_off, const int32_t* fn_lx,
+                const int32_t* fn_ly, const int32_t* fn_lz, const int32_t* fn_shell,
+                const double* fc, const double* pexp, const double* pcoef,
+                std::shared_ptr<BasisSetup>& out) {
   std::vector<HostShell> sh;
   for (int i = 0; i < n_func;) {
     int j = i;
@@ -717,55 +864,49 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
     i = j;
   }
   const int nsh = (int)sh.size();
-  // ---- shell pairs, candidate slots per class, reference-order indices, and
-  // the log range of the product exponents of each class (pair-order keys)
+  // shell pairs, candidate slots per class, reference-order indices, and the
+  // log range of the product exponents of each class (pair-order keys)
   std::vector<double> emin(nsh), emax(nsh);
   for (int i = 0; i < nsh; ++i) {
     emin[i] = *std::min_element(pexp + sh[i].prim0, pexp + sh[i].prim0 + sh[i].nprim);
     emax[i] = *std::max_element(pexp + sh[i].prim0, pexp + sh[i].prim0 + sh[i].nprim);
   }
-  std::vector<int> sp_i, sp_j, sp_slot, sp_gidx;
-  int64_t ncand[FE_LMAX + 1] = {}, gtot = 0;
+  const size_t nsp_all = (size_t)nsh * (nsh + 1) / 2;
+  std::vector<int> sp_i(nsp_all), sp_j(nsp_all), sp_slot(nsp_all), sp_gidx(nsp_all);
+  auto bs = std::make_shared<BasisSetup>();
+  int64_t gtot = 0;
   double pmin[FE_LMAX + 1], pmax[FE_LMAX + 1];
   for (int L = 0; L <= FE_LMAX; ++L) {
     pmin[L] = INFINITY;
     pmax[L] = 0.0;
   }
+  size_t k = 0;
   for (int i = 0; i < nsh; ++i)
-    for (int j = i; j < nsh; ++j) {
+    for (int j = i; j < nsh; ++j, ++k) {
       const int L = sh[i].l + sh[j].l;
       const int64_t np = (int64_t)sh[i].nprim * sh[j].nprim;
-      sp_i.push_back(i);
-      sp_j.push_back(j);
-      sp_slot.push_back((int)ncand[L]);
-      sp_gidx.push_back((int)gtot);
-      ncand[L] += np;
+      sp_i[k] = i;
+      sp_j[k] = j;
+      sp_slot[k] = (int)bs->ncand[L];
+      sp_gidx[k] = (int)gtot;
+      bs->ncand[L] += np;
       gtot += np;
       pmin[L] = std::min(pmin[L], emin[i] + emin[j]);
       pmax[L] = std::max(pmax[L], emax[i] + emax[j]);
       if (gtot > INT32_MAX / 2)
         return fe_set_error(FE_ERR_UNSUPPORTED, "too many primitive pairs (%lld)", (long long)gtot);
     }
-  const int nsp = (int)sp_i.size();
-
-  FE_CTX_LOCK(c);
-  FE_HIPCHK(hipSetDevice(c->device));
-  hipStream_t st = (hipStream_t)stream;
-  fe_table* t = new fe_table();
-  t->ctx = c;
-  t->has_basis = true;
-  t->n_func = n_func;
-  t->n_shells = nsh;
-  t->n_sp = nsp;
-  t->h_sp_i = sp_i;
-  t->h_sp_j = sp_j;
-  t->h_sp_slot.assign(sp_slot.begin(), sp_slot.end());
+  const int nsp = (int)nsp_all;
+  bs->device = c->device;
+  bs->n_func = n_func;
+  bs->n_prim = n_prim;
+  bs->nsh = nsh;
+  bs->nsp = nsp;
   for (int L = 0; L <= FE_LMAX; ++L) {
-    t->n_cand_L[L] = ncand[L];
     const double lo = std::log(std::isfinite(pmin[L]) ? pmin[L] : 1.0);
     const double w = std::log(std::max(pmax[L], 1e-300)) - lo;
-    t->key_lpmin[L] = lo;
-    t->key_lpinv[L] = 16.0 / std::max(w, 1e-12);
+    bs->lpmin[L] = lo;
+    bs->lpinv[L] = 16.0 / std::max(w, 1e-12);
   }
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (const HostShell& s : sh)
@@ -774,52 +915,119 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
       hi[a] = std::max(hi[a], s.c[a]);
     }
   const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-6});
-  for (int a = 0; a < 3; ++a) t->key_lo[a] = lo[a];
-  t->key_scale = 255.0 / ext;
-  auto fail = [&](int rc) {
-    (void)hipStreamSynchronize(st);
-    fe_table_free(t);
-    return rc;
-  };
-
-  // ---- basis + shell pairs to the device (one buffer)
+  for (int a = 0; a < 3; ++a) bs->lo[a] = lo[a];
+  bs->scale = 255.0 / ext;
+  // one device block
   std::vector<int> v_atom(nsh), v_l(nsh), v_fn0(nsh), v_nfn(nsh), v_p0(nsh), v_np(nsh);
   std::vector<double> v_c(3 * nsh);
   for (int i = 0; i < nsh; ++i) {
     v_atom[i] = sh[i].atom; v_l[i] = sh[i].l; v_fn0[i] = sh[i].fn0; v_nfn[i] = sh[i].nfn;
     v_p0[i] = sh[i].prim0; v_np[i] = sh[i].nprim;
-    for (int k = 0; k < 3; ++k) v_c[3 * i + k] = sh[i].c[k];
+    for (int q = 0; q < 3; ++q) v_c[3 * i + q] = sh[i].c[q];
   }
   std::vector<char> hb;
-  const size_t o_atom = push_bytes(hb, v_atom.data(), nsh), o_l = push_bytes(hb, v_l.data(), nsh),
-               o_fn0 = push_bytes(hb, v_fn0.data(), nsh), o_nfn = push_bytes(hb, v_nfn.data(), nsh),
-               o_p0 = push_bytes(hb, v_p0.data(), nsh), o_np = push_bytes(hb, v_np.data(), nsh),
-               o_c = push_bytes(hb, v_c.data(), 3 * (size_t)nsh),
-               o_pexp = push_bytes(hb, pexp, n_prim), o_pcoef = push_bytes(hb, pcoef, n_prim),
-               o_lx = push_bytes(hb, fn_lx, n_func), o_ly = push_bytes(hb, fn_ly, n_func),
-               o_lz = push_bytes(hb, fn_lz, n_func), o_spi = push_bytes(hb, sp_i.data(), nsp),
-               o_spj = push_bytes(hb, sp_j.data(), nsp),
-               o_sps = push_bytes(hb, sp_slot.data(), nsp),
-               o_spg = push_bytes(hb, sp_gidx.data(), nsp);
-  if (hipMalloc(&t->d_basis_mem, hb.size()) != hipSuccess) {
+  bs->o_atom = push_bytes(hb, v_atom.data(), nsh);
+  bs->o_l = push_bytes(hb, v_l.data(), nsh);
+  bs->o_fn0 = push_bytes(hb, v_fn0.data(), nsh);
+  bs->o_nfn = push_bytes(hb, v_nfn.data(), nsh);
+  bs->o_p0 = push_bytes(hb, v_p0.data(), nsh);
+  bs->o_np = push_bytes(hb, v_np.data(), nsh);
+  bs->o_c = push_bytes(hb, v_c.data(), 3 * (size_t)nsh);
+  bs->o_pexp = push_bytes(hb, pexp, n_prim);
+  bs->o_pcoef = push_bytes(hb, pcoef, n_prim);
+  bs->o_lx = push_bytes(hb, fn_lx, n_func);
+  bs->o_ly = push_bytes(hb, fn_ly, n_func);
+  bs->o_lz = push_bytes(hb, fn_lz, n_func);
+  bs->o_spi = push_bytes(hb, sp_i.data(), nsp);
+  bs->o_spj = push_bytes(hb, sp_j.data(), nsp);
+  bs->o_sps = push_bytes(hb, sp_slot.data(), nsp);
+  bs->o_spg = push_bytes(hb, sp_gidx.data(), nsp);
+  if (hipMalloc(&bs->mem, hb.size()) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(fe_set_error(FE_ERR_ALLOC, "fe_table_build: basis allocation failed"));
+    bs->mem = nullptr;
+    return fe_set_error(FE_ERR_ALLOC, "fe_table_build: basis allocation failed");
   }
+  bs->bytes = hb.size();
   // synchronous: hb is a local buffer
-  if (hipMemcpy(t->d_basis_mem, hb.data(), hb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(bs->mem, hb.data(), hb.size(), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(fe_set_error(FE_ERR_HIP, "fe_table_build: basis upload (hipMemcpy) failed"));
+    return fe_set_error(FE_ERR_HIP, "fe_table_build: basis upload (hipMemcpy) failed");
   }
-  t->dev_bytes += hb.size();
-  const char* bm = (const char*)t->d_basis_mem;
-  t->b.atom = (const int*)(bm + o_atom); t->b.l = (const int*)(bm + o_l);
-  t->b.fn0 = (const int*)(bm + o_fn0); t->b.nfn = (const int*)(bm + o_nfn);
-  t->b.prim0 = (const int*)(bm + o_p0); t->b.nprim = (const int*)(bm + o_np);
-  t->b.c = (const double*)(bm + o_c); t->b.pexp = (const double*)(bm + o_pexp);
-  t->b.pcoef = (const double*)(bm + o_pcoef); t->b.fn_lx = (const int*)(bm + o_lx);
-  t->b.fn_ly = (const int*)(bm + o_ly); t->b.fn_lz = (const int*)(bm + o_lz);
-  t->b.sp_i = (const int*)(bm + o_spi); t->b.sp_j = (const int*)(bm + o_spj);
-  t->b.sp_slot = (const int*)(bm + o_sps); t->b.sp_gidx = (const int*)(bm + o_spg);
+  const int32_t* iv[7] = {fn_atom, fn_nprim, fn_off, fn_lx, fn_ly, fn_lz, fn_shell};
+  basis_key(n_func, n_prim, iv, fc, pexp, pcoef, bs->ikey, bs->dkey);
+  out = bs;
+  return FE_OK;
+}
+}  // namespace
+
+extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
+                              const int32_t* fn_nprim, const int32_t* fn_off,
+                              const int32_t* fn_lx, const int32_t* fn_ly, const int32_t* fn_lz,
+                              const int32_t* fn_shell, const double* fc, const double* pexp,
+                              const double* pcoef, const double* dens, int n_points,
+                              const double* pts, int n_atoms, const double* Z, const double* xyz,
+                              double tol, int rule, int ptr_kind, void* stream, fe_table** out) {
+  // arguments first (no device needed to reject them)
+  if (out) *out = nullptr;
+  if (n_func <= 0 || n_prim <= 0 || n_atoms < 0)
+    return fe_set_error(FE_ERR_ARG, "fe_table_build: bad sizes");
+  if (!fn_atom || !fn_nprim || !fn_off || !fn_lx || !fn_ly || !fn_lz || !fn_shell || !fc ||
+      !pexp || !pcoef || (n_atoms && (!Z || !xyz)))
+    return fe_set_error(FE_ERR_ARG, "fe_table_build: null array");
+  if (int rc = check_build_args(n_points, pts, dens, tol, rule, ptr_kind, "fe_table_build"))
+    return rc;
+  if (!c || !out) return fe_set_error(FE_ERR_ARG, "fe_table_build: null argument");
+
+  FE_CTX_LOCK(c);
+  FE_HIPCHK(hipSetDevice(c->device));
+  // the basis: the cached set-up when the arrays are the same as last time
+  std::shared_ptr<BasisSetup> bs = std::static_pointer_cast<BasisSetup>(c->basis_cache);
+  bool same = false;
+  if (bs && bs->n_func == n_func && bs->n_prim == n_prim) {
+    std::vector<int32_t> ik;
+    std::vector<double> dk;
+    const int32_t* iv[7] = {fn_atom, fn_nprim, fn_off, fn_lx, fn_ly, fn_lz, fn_shell};
+    basis_key(n_func, n_prim, iv, fc, pexp, pcoef, ik, dk);
+    same = ik == bs->ikey && dk.size() == bs->dkey.size() &&
+           memcmp(dk.data(), bs->dkey.data(), dk.size() * sizeof(double)) == 0;
+  }
+  if (!same) {
+    c->basis_cache.reset();
+    if (int rc = basis_setup(c, n_func, n_prim, fn_atom, fn_nprim, fn_off, fn_lx, fn_ly, fn_lz,
+                             fn_shell, fc, pexp, pcoef, bs))
+      return rc;
+    c->basis_cache = bs;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  fe_table* t = new fe_table();
+  t->ctx = c;
+  t->has_basis = true;
+  t->n_func = n_func;
+  t->n_shells = bs->nsh;
+  t->n_sp = bs->nsp;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    t->n_cand_L[L] = bs->ncand[L];
+    t->key_lpmin[L] = bs->lpmin[L];
+    t->key_lpinv[L] = bs->lpinv[L];
+  }
+  for (int a = 0; a < 3; ++a) t->key_lo[a] = bs->lo[a];
+  t->key_scale = bs->scale;
+  t->basis_mem = bs;
+  t->dev_bytes += bs->bytes;
+  const char* bm = (const char*)bs->mem;
+  t->b.atom = (const int*)(bm + bs->o_atom); t->b.l = (const int*)(bm + bs->o_l);
+  t->b.fn0 = (const int*)(bm + bs->o_fn0); t->b.nfn = (const int*)(bm + bs->o_nfn);
+  t->b.prim0 = (const int*)(bm + bs->o_p0); t->b.nprim = (const int*)(bm + bs->o_np);
+  t->b.c = (const double*)(bm + bs->o_c); t->b.pexp = (const double*)(bm + bs->o_pexp);
+  t->b.pcoef = (const double*)(bm + bs->o_pcoef); t->b.fn_lx = (const int*)(bm + bs->o_lx);
+  t->b.fn_ly = (const int*)(bm + bs->o_ly); t->b.fn_lz = (const int*)(bm + bs->o_lz);
+  t->b.sp_i = (const int*)(bm + bs->o_spi); t->b.sp_j = (const int*)(bm + bs->o_spj);
+  t->b.sp_slot = (const int*)(bm + bs->o_sps); t->b.sp_gidx = (const int*)(bm + bs->o_spg);
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(st);
+    fe_table_free(t);
+    return rc;
+  };
   // nuclei (the Morton frame of the point sort)
   if (int rc = fe_table_set_atoms(t, n_atoms, Z, xyz)) return fail(rc);
   if (int rc = build_on_device(t, dens, n_points, pts, tol, rule, ptr_kind, st, false))

@@ -285,6 +285,7 @@ extern "C" int fe_finalize(fe_ctx* c) {
   for (int s = 0; s < FE_WS_SLOTS; ++s)
     if (c->ws[s]) (void)hipFree(c->ws[s]);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  c->basis_cache.reset();
   if (c->d_boys) (void)hipFree(c->d_boys);
   if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
   for (FeProfEv& e : c->prof) {
@@ -309,6 +310,7 @@ extern "C" int fe_ctx_trim(fe_ctx* c) {
   if (c->h_pin) FE_HIPCHK(hipHostFree(c->h_pin));
   c->h_pin = nullptr;
   c->h_pin_bytes = 0;
+  c->basis_cache.reset();
   return FE_OK;
 }
 
@@ -624,7 +626,7 @@ extern "C" int fe_table_free(fe_table* t) {
     if (t->d_rec32[L]) (void)hipFree(t->d_rec32[L]);
   if (t->d_mem) (void)hipFree(t->d_mem);
   if (t->d_atoms) (void)hipFree(t->d_atoms);
-  if (t->d_basis_mem) (void)hipFree(t->d_basis_mem);
+  t->basis_mem.reset();
   delete t;
   return FE_OK;
 }

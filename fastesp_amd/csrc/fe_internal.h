@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -145,6 +146,9 @@ struct fe_ctx {
   // device address of this device's copy of the file-scope scratch array of
   // fe_esp_forward_noalloc (from hipGetSymbolAddress at fe_init)
   char* noalloc_scratch = nullptr;
+  // the last basis fe_table_build set up (host layout + device copy), reused
+  // when the next build passes the same basis arrays
+  std::shared_ptr<void> basis_cache;
 };
 
 #define FE_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> fe_ctx_lock_((c)->mu)
@@ -195,7 +199,9 @@ struct fe_table {
   int n_func = 0;
   int n_shells = 0;
   int n_sp = 0;
-  void* d_basis_mem = nullptr;        // shells, primitives, components, shell pairs
+  // device copy of the shells, primitives, components and shell pairs, shared
+  // with the context's basis cache and every table built from the same basis
+  std::shared_ptr<void> basis_mem;
   struct {
     const int* atom; const int* l; const int* fn0; const int* nfn; const int* prim0;
     const int* nprim; const double* c; const double* pexp; const double* pcoef;
@@ -205,8 +211,6 @@ struct fe_table {
   int64_t n_cand_L[FE_LMAX + 1] = {}; // candidate slots (every primitive pair) per class
   int64_t kvalid_L[FE_LMAX + 1] = {}; // K-valid slots per class
   int* d_slot2kept[FE_LMAX + 1] = {}; // candidate slot -> compact record (-1 dropped)
-  std::vector<int> h_sp_i, h_sp_j;    // shell pair list (host)
-  std::vector<int64_t> h_sp_slot;
   // pair-order key frame: bounding box of the shell centres, and per class the
   // log range of the candidate exponents p
   double key_lo[3] = {0, 0, 0};

@@ -71,3 +71,30 @@ def test_no_device_is_a_clean_error():
     rc = lib.fe_init(0, ctypes.byref(h))
     assert rc in (-1, -2)
     assert lib.fe_last_error()
+
+
+def test_table_build_rejects_bad_arguments_without_a_device():
+    """fe_table_build and fe_table_rebuild check their arguments before they
+    touch a device: each bad argument comes back as FE_ERR_ARG with a message
+    naming it."""
+    import numpy as np
+    lib = _lib.load()
+    i = np.zeros(1, np.int32)
+    d = np.zeros(3)
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+
+    def build(n_func=1, n_points=1, tol=1e-12, rule=0, ptr_kind=0, dens=True):
+        out = ctypes.c_void_p()
+        return lib.fe_table_build(None, n_func, 1, ip(i), ip(i), ip(i), ip(i), ip(i), ip(i), ip(i),
+                                  dp(d), dp(d), dp(d), d.ctypes.data if dens else None, n_points,
+                                  d.ctypes.data, 0, None, None, tol, rule, ptr_kind, None,
+                                  ctypes.byref(out))
+
+    for kw, msg in [({"rule": 5}, b"rule"), ({"tol": -1.0}, b"tol"), ({"n_points": -1}, b"point"),
+                    ({"ptr_kind": 7}, b"ptr_kind"), ({"n_func": 0}, b"sizes"),
+                    ({"dens": False}, b"null"), ({}, b"null argument")]:
+        assert build(**kw) == -1, kw
+        assert msg in lib.fe_last_error(), (kw, lib.fe_last_error())
+    assert lib.fe_table_rebuild(None, None, 0, None, 1e-12, 0, 0, None) == -1
+    assert b"null table" in lib.fe_last_error()
