@@ -299,18 +299,27 @@ def _conformer_orbitals(args):
         return conformer(*args, orbitals=True)
 
 
-def conformer_set(n: int, seed: int = 1004, workers: int = 0) -> list:
-    """conformer(0..n-1) generated in worker processes (fork: call this before
-    the process touches the GPU).  Workers return the occupied factor C, a few
-    MB instead of the n_fn^2 density; P = 2 C C^T is formed here."""
+def conformer_set(n: int, seed: int = 1004, workers: int = 0, threads: bool = False) -> list:
+    """conformer(0..n-1) generated in parallel.  Workers return the occupied
+    factor C, a few MB instead of the n_fn^2 density; P = 2 C C^T is formed
+    here.  Processes (fork) by default: call this before the process touches
+    the GPU.  ``threads=True`` uses a thread pool instead (single-threaded
+    BLAS in each, so the result is the same), which is safe in a process that
+    already uses the GPU."""
     import multiprocessing as mp
     import os
     if workers <= 0:
         workers = max(1, min(16, len(os.sched_getaffinity(0))))
     if workers == 1 or n < 4:
         return [conformer(i, seed) for i in range(n)]
-    with mp.get_context("fork").Pool(workers) as pool:
-        out = pool.map(_conformer_orbitals, [(i, seed) for i in range(n)], chunksize=1)
+    if threads:
+        from concurrent.futures import ThreadPoolExecutor
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(1), ThreadPoolExecutor(workers) as ex:
+            out = list(ex.map(lambda i: conformer(i, seed, orbitals=True), range(n)))
+    else:
+        with mp.get_context("fork").Pool(workers) as pool:
+            out = pool.map(_conformer_orbitals, [(i, seed) for i in range(n)], chunksize=1)
     for w in out:
         w.density = density_from_orbitals(w.density)
     return out

@@ -13,6 +13,11 @@ Fixtures:
   cfg1_300.npz    config 1 (1634 functions, 300 points): V at tol 0 and 1e-12
   cfg1_30k.npz    config 1 table screened against its 30000 points (tol 1e-12),
                   V at 256 sampled points
+  cfg1_3k.npz     config 1 at 3000 points (tol 1e-12): V at every point (C++
+                  restatement), pair counts and the sum rule
+  cfg4.npz        8 conformers of the 256-conformer set (10-80 atoms, 517-19988
+                  points): V at up to 256 sampled points (C++ restatement), the
+                  field at 64 of them (NumPy oracle), pair counts, sum rules
   grad_toy.npz    field and dL/dP of the C+O toy (all classes L <= 6)
   cfg2_sub.npz    config-2 loss on a 400-point subset: V, field, dL/dG checksum,
                   dL/dP (1634 x 1634, float64, compressed)
@@ -193,8 +198,63 @@ def make_cfg3():
                         sum_rule=ct.sum_rule, n_func=w.basis.n)
 
 
+# conformers of the 256-set in the cfg4 fixture: the fewest and most atoms,
+# the fewest and most points, and four between
+CFG4_CONFORMERS = (0, 11, 52, 77, 117, 120, 132, 160)
+
+
+def make_cfg1_3k():
+    """Config 1 at 3000 points: V at every point from the C++ restatement
+    (NumPy screen rule), checked against the NumPy oracle at 64 points."""
+    from oracle.cpu_port import CpuTable
+    w = synth.config1(3000)
+    ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, rule=0)
+    V = ct.esp(w.points)
+    tab = _port_table(ct, w)
+    Vnp = par_esp(tab, w.points[:64])
+    print(f"cfg1_3k: NumPy V over the port's table vs the port: {np.abs(Vnp - V[:64]).max():.2e}")
+    assert np.abs(Vnp - V[:64]).max() < 1e-11
+    np.savez_compressed(OUT / "cfg1_3k.npz", hash=w.input_hash(), fp=w.fingerprint(), V=V,
+                        npairs=ct.n_pairs, n_candidates=ct.n_candidates, sum_rule=ct.sum_rule)
+
+
+def make_cfg4():
+    """Eight conformers of the batched set (synth.conformer_set(256), seed 1004),
+    regenerated exactly as the set makes them (occupied factor, then P = 2 C C^T).
+    V at up to 256 sampled points from the C++ restatement; the field at the
+    first 64 of them from the NumPy oracle over the same table (whose V must
+    agree with the restatement's)."""
+    from oracle.cpu_port import CpuTable
+    out = dict(conformers=np.array(CFG4_CONFORMERS))
+    for c in CFG4_CONFORMERS:
+        w = synth.conformer(c, 1004, orbitals=True)
+        w.density = synth.density_from_orbitals(w.density)
+        n = len(w.points)
+        rng = np.random.default_rng([404, c])
+        idx = np.sort(rng.choice(n, min(n, 256), replace=False))
+        ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, rule=0)
+        V = ct.esp(w.points[idx])
+        tab = _port_table(ct, w)
+        chunks = np.array_split(w.points[idx[:64]], 8)
+        with ProcessPoolExecutor(8) as ex:
+            res = list(ex.map(_field_chunk, [(tab, ch) for ch in chunks]))
+        field = np.concatenate([r[0] for r in res])
+        Vnp = np.concatenate([r[1] for r in res])
+        err = np.abs(Vnp - V[:64]).max()
+        print(f"cfg4 conformer {c}: {w.meta['n_atoms']} atoms, {w.basis.n} functions, {n} points, "
+              f"{ct.n_pairs} pairs; NumPy vs port V {err:.2e}", flush=True)
+        assert err < 1e-11
+        for k, v in dict(hash=np.frombuffer(w.input_hash().encode(), np.uint8), fp=w.fingerprint(),
+                         idx=idx, V=V, field=field, npairs=ct.n_pairs,
+                         n_candidates=ct.n_candidates, sum_rule=ct.sum_rule,
+                         n_points=n).items():
+            out[f"c{c}_{k}"] = v
+    np.savez_compressed(OUT / "cfg4.npz", **out)
+
+
 JOBS = dict(boys=make_boys, cfg0=make_cfg0, cfg1_300=make_cfg1_300, cfg1_30k=make_cfg1_30k,
-            grad_toy=make_grad_toy, cfg2_sub=make_cfg2_sub, cfg3=make_cfg3)
+            grad_toy=make_grad_toy, cfg2_sub=make_cfg2_sub, cfg3=make_cfg3, cfg1_3k=make_cfg1_3k,
+            cfg4=make_cfg4)
 
 
 def main():
