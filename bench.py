@@ -7,26 +7,35 @@
 Workload: the 1634-function protein-fragment-sized synthetic molecule (54
 atoms, C/N/O/H + 2 S, TZ-like s/p/d/f basis, seed 1001) with 30000 points on
 its 1.4 x vdW surface.  The pair table is built on the GPU (timed and reported
-separately); a step is one pass of the evaluation over all 30000 points with
-the table and the points already resident in HBM.  `value` is ms per 1e4
-points of a step (lower is better).
+separately); a step is one pass of forward + field + loss + backward to dL/dP
+over all 30000 points (the config-2 loss) with the table and the points
+already resident in HBM.  `value` is ms per 1e4 points of the FP64 step
+(lower is better); the mixed-precision step is reported beside it.
 
 Printed JSON (one line): the driver's contract fields plus
-  roofline     -- algorithmic FP64 FLOPs of the evaluation kernels per step /
-                  their HIP-event time, against the FP64 vector peak;
-  cpu_baseline -- the C++/OpenMP port of the reference path (table build +
-                  evaluation) timed on this host's cores on a bounded point
-                  sample, scaled to ms per 1e4 points;
-  accuracy     -- max |dV| against the FP64 oracle at 256 sampled points;
-  batched      -- the 256-conformer graph-captured step;
-  config3      -- 4538 functions, 200000 points: FP64 and mixed fwd+bwd with
-                  their errors against the FP64 oracle;
-  density      -- the electron density at the same points on the FP64 matrix
-                  cores, its roofline and its NumPy baseline.
+  roofline      -- the dominant launch of the headline step: its performed
+                   FLOPs (each (point, pair) priced by the path it took, counted
+                   by fe_eval_work; fastesp_amd.flops) over its HIP-event time,
+                   against the FP64 vector peak; the model and executed
+                   (hardware counter) figures beside it;
+  roofline_step -- the same for the whole step;
+  cpu_baseline  -- the NumPy restatement of the reference evaluation plus its
+                   adjoint (forward + field + dL/dG), one core, on a fixed
+                   point sample scaled to 1e4 points; cpu_baselines holds the
+                   NumPy forward, the C++/OpenMP port on every core and the
+                   reference's own build_pairs;
+  build_plus_fwd -- table build + FP64 forward at 300 / 3000 / 30000 points,
+                   the reference's published measurement;
+  autograd      -- the differentiable step through ESPModel (rebuild in place,
+                   forward + field, autograd backward) and the same step as a
+                   recorded graph;
+  batched, config3, density -- the 256-conformer graph-captured step, the
+                   200000-point configuration, the electron density.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -39,6 +48,10 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# published build + FP64 forward of the reference (RTX 5090 + 16 CPU cores),
+# ms per 1e4 points (benchmarks/README.md:50-52)
+REFERENCE_BUILD_PLUS_FWD = {300: 8959.0, 3000: 1598.0, 30000: 819.0}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -47,9 +60,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=30000)
     ap.add_argument("--mode", choices=["fwd", "fwdbwd"], default="fwdbwd")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the C++/OpenMP port's evaluation sample")
+    ap.add_argument("--numpy-points", type=int, default=300,
+                    help="fixed point sample of the NumPy forward baseline")
+    ap.add_argument("--numpy-adjoint-points", type=int, default=20,
+                    help="fixed point sample of the NumPy forward + adjoint baseline")
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0 = library default)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="skip the launch-shape autotuner (runs before the timed region)")
@@ -72,6 +90,10 @@ def parse():
                     help="timed evaluations of the electron-density leg (0 = skip it)")
     ap.add_argument("--config3-steps", type=int, default=3,
                     help="timed fwd+bwd steps of the config-3 leg, FP64 and mixed (0 = skip it)")
+    ap.add_argument("--build-legs", default="300,3000,30000",
+                    help="point counts of the build + FP64 forward legs ('' = skip)")
+    ap.add_argument("--autograd-steps", type=int, default=10,
+                    help="timed steps of the autograd / graph-captured leg (0 = skip it)")
     return ap.parse_args()
 
 
@@ -87,59 +109,78 @@ PHASE_NAMES.update({16 + L: f"dG_L{L}" for L in range(7)})
 PHASE_NAMES.update({24 + L: f"dG_reduce_L{L}" for L in range(7)})
 
 
-def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, flops,
-                             traffic_file, counters_file):
-    """Roofline of the launch that takes the most time in the timed region.
+def work_counts(lib, tab, pts_ptr, n, grad, sh) -> dict:
+    """fe_eval_work: far-field / tabulated (point, pair) evaluations per class of
+    the evaluation and dL/dG launches, with the context's current tuning."""
+    from fastesp_amd import _lib
+    c = (ctypes.c_int64 * 28)()
+    _lib.check(lib.fe_eval_work(tab._h, pts_ptr, n, int(grad), _lib.FE_PTR_DEVICE, sh, c),
+               "fe_eval_work")
+    v = list(c)
+    return dict(eval_far=v[0:7], eval_near=v[7:14], dg_far=v[14:21], dg_near=v[21:28])
 
-    phases: {phase id: (total ms, launches)} from fe_profile over the timed
-    steps.  Candidates are the per-class evaluation and dL/dG launches; the
-    achieved rate is that launch's algorithmic FLOPs (flops.kernel_flops) over
-    its own average event-timed duration."""
+
+def launch_flops(counts: dict, grad: bool) -> dict:
+    """Performed FLOPs of every class launch: {phase id: flops}."""
+    from fastesp_amd import flops
+    out = {}
+    kind = "fwd_grad" if grad else "fwd"
+    for L in range(7):
+        out[8 + L] = flops.performed_flops(kind, L, counts["eval_far"][L], counts["eval_near"][L])
+        if grad:
+            out[16 + L] = flops.performed_flops("dg", L, counts["dg_far"][L], counts["dg_near"][L])
+    return out
+
+
+def profile_values(path: Path, kernel: str) -> dict:
+    """key = value lines of a committed rocprofv3 summary, if it covers `kernel`."""
+    if not path.exists():
+        return {}
+    kv = dict((k.strip(), v.strip()) for k, v in
+              (line.split("=", 1) for line in path.read_text().splitlines()
+               if "=" in line and not line.startswith(("#", "`", " ", "|"))))
+    return kv if kv.get("kernel") == kernel else {}
+
+
+def roofline(phases, counts, grad, ppl_L, fp32_L, prec, pairs_per_L, n, traffic_file,
+             counters_file) -> dict:
+    """The launch with the most time in the timed region: its performed FLOPs
+    per launch over its own average event-timed duration, against the FP64
+    vector peak (FP32 classes issue non-packed FP32, at the FP64 rate)."""
+    from fastesp_amd import flops
+    fl = launch_flops(counts, grad)
     best = None
     for pid, (ms, cnt) in phases.items():
-        if 8 <= pid < 15 or 16 <= pid < 23:
-            if best is None or ms > best[1]:
-                best = (pid, ms, cnt)
+        if pid in fl and (best is None or ms > best[1]):
+            best = (pid, ms, cnt)
     if best is None:
         return None
     pid, ms, cnt = best
-    grad = mode == "fwdbwd"
-    fp32 = False
+    avg_ms = ms / cnt
     if pid < 16:
         L = pid - 8
-        kind = "fwd_grad" if grad else "fwd"
         fp32 = prec == "mixed" and bool(fp32_L[L])
         name = f"k_eval<{L}, {ppl_L[L]}, {str(grad).lower()}, {str(fp32).lower()}>"
+        far, near = counts["eval_far"][L], counts["eval_near"][L]
+        model = flops.kernel_flops("fwd_grad" if grad else "fwd", L, pairs_per_L[L], n)
     else:
         L = pid - 16
-        kind = "dg"
         name = f"k_dG<{L}>"
-    avg_ms = ms / cnt
-    fl = flops.kernel_flops(kind, L, pairs_per_L[L], n)
-    achieved = fl / (avg_ms * 1e-3)
-    if fp32:
-        f_all = {"fwd": flops.f_fwd, "fwd_grad": flops.f_eval_grad}[kind](L)
-        share32 = flops.f32_part(L, grad) / f_all
-        peak = 1.0 / ((1 - share32) / flops.PEAK_FP64_VALU + share32 / flops.PEAK_FP32_VALU)
-    else:
-        peak = flops.PEAK_FP64_VALU
-    def profile_values(path):
-        if not path.exists():
-            return {}
-        kv = dict((k.strip(), v.strip()) for k, v in
-                  (line.split("=", 1) for line in path.read_text().splitlines()
-                   if "=" in line and not line.startswith(("#", "`", " ", "|"))))
-        return kv if kv.get("kernel") == name else {}
-
-    tv = profile_values(traffic_file)
+        far, near = counts["dg_far"][L], counts["dg_near"][L]
+        model = flops.kernel_flops("dg", L, pairs_per_L[L], n)
+    achieved = fl[pid] / (avg_ms * 1e-3)
+    peak = flops.PEAK_FP64_VALU
+    tv = profile_values(traffic_file, name)
     traffic = int(float(tv["traffic_bytes_per_launch"])) if "traffic_bytes_per_launch" in tv else None
     out = dict(bound="valu", achieved=round(achieved / 1e12, 3), peak=round(peak / 1e12, 2),
                unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic, kernel=name,
-               launches=cnt, avg_launch_ms=round(avg_ms, 4), flops_per_launch=fl,
-               flop_model=f"{kind}: N x pairs(L={L}) x {flops.__name__}."
-                          + {"fwd": "f_fwd", "fwd_grad": "f_eval_grad", "dg": "f_dg"}[kind])
-    # executed-instruction cross-check (rocprofv3 SQ_INSTS_VALU_* summary of the same launch)
-    cv = profile_values(counters_file)
+               launches=cnt, avg_launch_ms=round(avg_ms, 4), flops_per_launch=fl[pid],
+               far_evaluations=int(far), tabulated_evaluations=int(near),
+               flop_model="performed work: far and tabulated (point, pair) evaluations "
+                          "(fe_eval_work) x fastesp_amd.flops.f_eval_path / f_dg_path",
+               model_flops_per_launch=model,
+               model_frac=round(model / (avg_ms * 1e-3) / peak, 4))
+    cv = profile_values(counters_file, name)
     if "executed_f64_flops_per_launch" in cv:
         ex = float(cv["executed_f64_flops_per_launch"]) + float(cv.get("executed_f32_flops_per_launch", 0))
         out.update(executed_flops_per_launch=ex,
@@ -152,7 +193,6 @@ def dominant_kernel_roofline(phases, pairs_per_L, n, mode, prec, ppl_L, fp32_L, 
 def probe_valu_rate(device: int, precision: int) -> float:
     """Sustained FMA rate (TFLOP/s) of FP64 (0) or non-packed FP32 (1) chains,
     from the benchmark's calibration library (not the product library)."""
-    import ctypes
     lib = ctypes.CDLL(str(ROOT / "fastesp_amd" / "libfastesp_probe.so"))
     lib.fe_probe_valu_rate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.fe_probe_last_error.restype = ctypes.c_char_p
@@ -162,15 +202,39 @@ def probe_valu_rate(device: int, precision: int) -> float:
     return v.value
 
 
-def cpu_baseline(w, budget_s: float) -> dict:
-    """C++/OpenMP port (oracle/cpu_port) on this host: build + evaluate a sample."""
-    sys.path.insert(0, str(ROOT))
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(w, budget_s: float, n_np: int, n_adj: int, V_gpu) -> dict:
+    """The CPU paths on this host, timed on fixed point samples and scaled
+    linearly to 1e4 points (the evaluation is linear in the point count for a
+    fixed table):
+      numpy_fwd     -- the NumPy restatement of the reference evaluation
+                       (fastesp/table.py:73-104), one process, one thread;
+      numpy_fwdbwd  -- the same plus the NumPy adjoint (field and dL/dG; the
+                       point-independent dL/dG -> dL/dP is left out);
+      cpp_fwd       -- the C++/OpenMP port (builder of build_pairs.cpp, the
+                       reference evaluation) on every core of the host;
+      reference_build_pairs_s -- the reference's own builder, compiled here.
+    The NumPy legs evaluate the C++ port's table (the NumPy screen rule; the
+    two builders keep the same pairs)."""
+    from threadpoolctl import threadpool_limits
+    from oracle import evaluate
     from oracle.cpu_port import CpuTable, threads
     nt = threads()
+    out = dict(cores_available=nt, cpu_model=cpu_model(),
+               omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
     t0 = time.perf_counter()
     ct = CpuTable(w.wfn, w.basis, w.density, w.points, 1e-12, nthreads=nt)
     t_build = time.perf_counter() - t0
-    # calibrate the sample so the evaluation takes about `budget_s`
+    # C++ port: calibrate the sample so the evaluation takes about budget_s
     n = 32
     t0 = time.perf_counter()
     ct.esp(w.points[:n], nthreads=nt)
@@ -179,12 +243,39 @@ def cpu_baseline(w, budget_s: float) -> dict:
     t0 = time.perf_counter()
     ct.esp(w.points[:n2], nthreads=nt)
     dt2 = time.perf_counter() - t0
-    ms_per_1e4 = dt2 * 1e3 / n2 * 1e4
-    out = dict(value=round(ms_per_1e4, 3), unit="ms/1e4 pts", cores=nt, kind="port",
-               sample=f"C++/OpenMP port (oracle/cpu_port.cpp): FP64 forward on the first "
-                      f"{n2} of {len(w.points)} points with its own table "
-                      f"({ct.n_pairs} pairs, built in {t_build:.2f} s), scaled linearly to 1e4",
-               build_s=round(t_build, 3))
+    out["cpp_fwd"] = dict(value=round(dt2 * 1e3 / n2 * 1e4, 3), unit="ms/1e4 pts", cores=nt,
+                          kind="port", build_s=round(t_build, 3),
+                          sample=f"C++/OpenMP port (oracle/cpu_port.cpp): FP64 forward on the "
+                                 f"first {n2} of {len(w.points)} points with its own table "
+                                 f"({ct.n_pairs} pairs), scaled linearly to 1e4")
+    tab = ct.oracle_table(w.wfn, w.points)
+    rng = np.random.default_rng(300)
+    sel = np.sort(rng.choice(len(w.points), n_np, replace=False))
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        Vn = evaluate.esp(tab, w.points[sel])
+        dt = time.perf_counter() - t0
+        out["numpy_fwd"] = dict(value=round(dt * 1e3 / n_np * 1e4, 1), unit="ms/1e4 pts",
+                                cores=1, kind="port", seconds=round(dt, 2),
+                                sample=f"NumPy restatement (oracle/evaluate.py) of "
+                                       f"PairTable.esp: FP64 forward on a fixed {n_np}-point "
+                                       f"subset (seed 300) of the {len(w.points)} points, one "
+                                       f"thread, scaled linearly to 1e4")
+        if V_gpu is not None:
+            out["numpy_fwd"]["max_abs_dV_vs_gpu_au"] = float(np.abs(Vn - V_gpu[sel]).max())
+        sel2 = sel[:n_adj]
+        gV = np.ones(n_adj)
+        t0 = time.perf_counter()
+        evaluate.esp(tab, w.points[sel2])
+        evaluate.field(tab, w.points[sel2])
+        evaluate.grad_G(tab, w.points[sel2], gV)
+        dt = time.perf_counter() - t0
+        out["numpy_fwdbwd"] = dict(value=round(dt * 1e3 / n_adj * 1e4, 1), unit="ms/1e4 pts",
+                                   cores=1, kind="port", seconds=round(dt, 2),
+                                   sample=f"NumPy restatement: FP64 forward + field + dL/dG "
+                                          f"(oracle/evaluate.py esp, field, grad_G) on the first "
+                                          f"{n_adj} points of the same subset, one thread, "
+                                          f"scaled linearly to 1e4 (dL/dG -> dL/dP not included)")
     ref = ROOT / "oracle" / "_ref" / "build_pairs"
     if ref.exists():
         from fastesp_amd.formats import basis_payload, write_basis_payload
@@ -197,6 +288,150 @@ def cpu_baseline(w, budget_s: float) -> dict:
         out["reference_build_pairs_rc"] = r.returncode
         for f in (b, p):
             f.unlink(missing_ok=True)
+    return out
+
+
+def build_plus_fwd_legs(counts, torch, ctx, stream, reps: int = 5) -> dict:
+    """The reference's published measurement (benchmarks/benchmark.py:107-141):
+    table build + FP64 forward at each point count on the config-1 molecule.
+    Build: the public API from host arrays (build_table), fastest of `reps`
+    (the first build of the basis, which also sets it up on the device, is
+    reported too); forward: device-resident points, fastest of `reps` HIP-event
+    timed runs, as the reference takes the minimum over its runs."""
+    from fastesp_amd import _lib, synth
+    from fastesp_amd.table import build_table
+    out = {}
+    sh = stream.cuda_stream
+    for n in counts:
+        w = synth.config1(n)
+        ctx.trim()  # no basis cache from earlier legs: the first build sets it up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+        first = time.perf_counter() - t0
+        bt = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+            bt.append(time.perf_counter() - t0)
+        pts = torch.from_numpy(np.ascontiguousarray(w.points)).to("cuda:0")
+        V = torch.empty(n, dtype=torch.float64, device="cuda:0")
+        ft = []
+        for r in range(reps + 1):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _lib.check(ctx.lib.fe_esp_forward(tab._h, pts.data_ptr(), n, _lib.FE_PREC_FP64,
+                                              _lib.FE_PART_ESP, _lib.FE_PTR_DEVICE, sh,
+                                              V.data_ptr()), "fe_esp_forward")
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if r:
+                ft.append(e0.elapsed_time(e1))
+        ms = min(bt) * 1e3 + min(ft)
+        leg = dict(n_points=n, n_pairs=int(tab.n_pairs), build_ms=round(min(bt) * 1e3, 3),
+                   first_build_ms=round(first * 1e3, 3), fwd_ms=round(min(ft), 4),
+                   build_plus_fwd_ms=round(ms, 3), ms_per_1e4_pts=round(ms / (n / 1e4), 3),
+                   reference_ms_per_1e4_pts=REFERENCE_BUILD_PLUS_FWD.get(n))
+        if leg["reference_ms_per_1e4_pts"]:
+            leg["speedup_vs_reference"] = round(leg["reference_ms_per_1e4_pts"] / leg["ms_per_1e4_pts"], 1)
+        out[str(n)] = leg
+        log(f"build + fwd at {n} points: build {leg['build_ms']} ms (first {leg['first_build_ms']}),"
+            f" fwd {leg['fwd_ms']} ms, {leg['ms_per_1e4_pts']} ms/1e4 pts")
+        tab.close()
+    return out
+
+
+def autograd_leg(w, torch, ctx, stream, steps: int) -> dict:
+    """Config 2 through the differentiable interface: ESPModel (table rebuilt
+    in place from the density and points on every call, forward + field) and
+    the autograd backward to points.grad and density.grad, eager; then the same
+    step (rebuild -> forward + field -> loss -> dL/dP) recorded as one graph
+    through the C ABI and replayed."""
+    from fastesp_amd import _lib
+    from fastesp_amd.autograd import ESPModel
+    dev = torch.device("cuda:0")
+    n, nf = len(w.points), w.basis.n
+    model = ESPModel(w.wfn, w.basis, tol=1e-12)
+    pts = torch.from_numpy(np.ascontiguousarray(w.points)).to(dev).requires_grad_()
+    P = torch.from_numpy(np.ascontiguousarray(w.density)).to(dev).requires_grad_()
+    rng = np.random.default_rng(2002)
+    wts = torch.from_numpy(rng.uniform(0.5, 1.0, n)).to(dev)
+    with torch.no_grad():
+        V0 = model(pts.detach(), P.detach())
+    target = V0 + torch.from_numpy(rng.normal(0, 1e-3, n)).to(dev)
+
+    def step():
+        pts.grad = None
+        P.grad = None
+        V = model(pts, P)
+        loss = (wts * (V - target) ** 2).sum()
+        loss.backward()
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    out = dict(n_points=n, eager_ms_per_step=round(ms, 3),
+               eager_ms_per_1e4_pts=round(ms / (n / 1e4), 4),
+               eager_step="ESPModel: fe_table_rebuild + fe_esp_forward_grad, loss, "
+                          "autograd backward (points.grad = gV dV/dC, density.grad = dL/dP)")
+    # the raw step as a graph
+    lib = ctx.lib
+    t = model.last_table
+    Pd, pd = P.detach(), pts.detach()
+    V = torch.empty(n, dtype=torch.float64, device=dev)
+    F = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    dP = torch.empty((nf, nf), dtype=torch.float64, device=dev)
+
+    def raw():
+        sh = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.fe_table_rebuild(t._h, Pd.data_ptr(), n, pd.data_ptr(), 1e-12, 0,
+                                        _lib.FE_PTR_DEVICE, sh), "fe_table_rebuild")
+        _lib.check(lib.fe_esp_forward_grad(t._h, pd.data_ptr(), n, _lib.FE_PREC_FP64,
+                                           _lib.FE_PTR_DEVICE, sh, V.data_ptr(), F.data_ptr()),
+                   "fe_esp_forward_grad")
+        gV = 2.0 * wts * (V - target)
+        _lib.check(lib.fe_esp_backward(t._h, pd.data_ptr(), n, gV.data_ptr(), _lib.FE_PREC_FP64,
+                                       _lib.FE_PTR_DEVICE, sh, None, dP.data_ptr()),
+                   "fe_esp_backward")
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        raw()
+        raw()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        raw()
+    g.replay()
+    torch.cuda.synchronize()
+    Vg, dPg = V.clone(), dP.clone()
+    raw()
+    torch.cuda.synchronize()
+    bitwise = bool(torch.equal(Vg, V) and torch.equal(dPg, dP))
+    e0.record(stream)
+    for _ in range(steps):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / steps
+    out.update(graph_ms_per_step=round(gms, 3), graph_ms_per_1e4_pts=round(gms / (n / 1e4), 4),
+               graph_replay_equals_eager=bitwise,
+               graph_step="one hipGraph: fe_table_rebuild -> fe_esp_forward_grad -> loss -> "
+                          "fe_esp_backward (dL/dP)")
+    log(f"autograd step {ms:.2f} ms, graph-captured rebuild+fwd+bwd {gms:.2f} ms "
+        f"(replay == eager: {bitwise})")
     return out
 
 
@@ -448,24 +683,61 @@ def main() -> int:
         a.tune = False
         for L, v in enumerate(int(x) for x in a.fp32_classes.split(",")):
             ctx.set_tuning(20 + L, v)
+    dev = torch.device("cuda:0")
+    n = a.points
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    lib = ctx.lib
     # --- table build on the GPU (reported, not part of the step) ----------
-    build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)  # warm-up
+    # public API from host arrays (first build of the basis, then with the
+    # basis set-up cached), and from device-resident density and points
+    t0 = time.perf_counter()
+    build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    t_first = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)
     t_build = time.perf_counter() - t0
     info = tab.info()
-    log(f"table: {info['n_pairs']} pairs, per L {info['pairs_per_L']}, build {t_build*1e3:.1f} ms")
-
-    dev = torch.device("cuda:0")
-    n = a.points
+    log(f"table: {info['n_pairs']} pairs, per L {info['pairs_per_L']}, build {t_build*1e3:.2f} ms "
+        f"(first {t_first*1e3:.2f} ms)")
     pts = torch.from_numpy(np.ascontiguousarray(w.points)).to(dev)
+    Pd = torch.from_numpy(np.ascontiguousarray(w.density)).to(dev)
+    from fastesp_amd.autograd import ESPModel
+    mdl = ESPModel(w.wfn, w.basis, tol=1e-12, reuse_table=False)
+    mdl.build(pts, Pd)
+    torch.cuda.synchronize()
+    dt = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        tb = mdl.build(pts, Pd)
+        torch.cuda.synchronize()
+        dt.append(time.perf_counter() - t0)
+        tb.close()
+    # in-place rebuild (device inputs, no host synchronisation): HIP events
+    _lib.check(lib.fe_table_rebuild(tab._h, Pd.data_ptr(), n, pts.data_ptr(), 1e-12, 0,
+                                    _lib.FE_PTR_DEVICE, sh), "fe_table_rebuild")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        _lib.check(lib.fe_table_rebuild(tab._h, Pd.data_ptr(), n, pts.data_ptr(), 1e-12, 0,
+                                        _lib.FE_PTR_DEVICE, sh), "fe_table_rebuild")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    build = dict(host_api_ms=round(t_build * 1e3, 3), host_api_first_ms=round(t_first * 1e3, 3),
+                 device_inputs_ms=round(min(dt) * 1e3, 3),
+                 rebuild_in_place_ms=round(e0.elapsed_time(e1) / 5, 3),
+                 note="host_api: build_table with NumPy inputs (21 MB density over the host "
+                      "link), basis set-up cached after the first; device_inputs: fe_table_build "
+                      "on HBM-resident density and points; rebuild_in_place: fe_table_rebuild, "
+                      "event-timed, no host synchronisation")
+    log(f"build: {build}")
+    tab = build_table(w.wfn, w.basis, w.density, w.points, tol=1e-12)  # exact-size table again
+
     V = torch.empty(n, dtype=torch.float64, device=dev)
     dVdC = torch.empty((n, 3), dtype=torch.float64, device=dev)
     dLdP = torch.empty((w.basis.n, w.basis.n), dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream(dev)
-    sh = stream.cuda_stream
-    lib = ctx.lib
     PREC = {"fp64": _lib.FE_PREC_FP64, "mixed": _lib.FE_PREC_MIXED}
     # config-2 loss: L = sum w (V - T)^2, w ~ U[0.5, 1], T = V_fp64 + N(0, 1e-3)
     rng = np.random.default_rng(2002)
@@ -493,8 +765,20 @@ def main() -> int:
 
     gold = ROOT / "tests" / "golden" / "cfg1_30k.npz"
     g = np.load(gold) if (gold.exists() and n == 30000) else None
-
+    from fastesp_amd import autotune
     tuned = {}
+
+    def current_shape():
+        return dict(ppl_L=[ctx_ppl[L] for L in range(7)], target_wg=ctx_wg[0],
+                    fp32_classes=[ctx_fp32[L] for L in range(7)])
+
+    ctx_ppl = [a.ppl or 2] * 7
+    if a.ppl_classes:
+        ctx_ppl = [int(x) for x in a.ppl_classes.split(",")]
+    ctx_wg = [a.wg or 16 * 256]
+    ctx_fp32 = [0, 0, 0, 1, 1, 1, 1]
+    if a.fp32_classes:
+        ctx_fp32 = [int(x) for x in a.fp32_classes.split(",")]
 
     def timed(mode, prec):
         force32 = prec == "mixed32"
@@ -503,15 +787,19 @@ def main() -> int:
             for L in range(7):
                 ctx.set_tuning(20 + L, 1)
         if a.tune:
-            from fastesp_amd import autotune
             r = autotune.tune(tab, pts, op=mode, precision=PREC[prec], fix_mixed=force32)
-            tuned[f"{mode}_{'mixed32' if force32 else prec}"] = dict(ppl_L=r["best"]["ppl_L"],
-                                           target_wg=r["best"]["target_wg"],
-                                           fp32_classes=r["best"]["mixed_L"],
-                                           trials=len(r["all"]))
-            log(f"autotune {mode} {prec}: best ppl_L={r['best']['ppl_L']} "
-                f"wg={r['best']['target_wg']} {r['best']['ms']:.2f} ms "
-                f"(first {r['all'][0]['ms']:.2f} ms, {len(r['all'])} trials)")
+            best = r["best"]
+            shape = dict(ppl_L=best["ppl_L"], target_wg=best["target_wg"],
+                         fp32_classes=best["mixed_L"] or ([1] * 7 if force32 else ctx_fp32),
+                         trials=len(r["all"]))
+            log(f"autotune {mode} {prec}: best ppl_L={best['ppl_L']} wg={best['target_wg']} "
+                f"{best['ms']:.2f} ms (first {r['all'][0]['ms']:.2f} ms, {len(r['all'])} trials)")
+        else:
+            shape = current_shape()
+            if force32:
+                shape["fp32_classes"] = [1] * 7
+        autotune.apply(ctx, shape["ppl_L"], shape["target_wg"],
+                       shape["fp32_classes"] if prec == "mixed" else None)
         step = make_step(mode, prec)
         for _ in range(a.warmup):
             step()
@@ -527,9 +815,13 @@ def main() -> int:
         phases = ctx.profile_stop()
         ms = e0.elapsed_time(e1) / a.steps
         res = dict(ms_per_step=round(ms, 4), ms_per_1e4_pts=round(ms / (n / 1e4), 4),
+                   tuning=shape,
                    phases_ms_per_step={PHASE_NAMES.get(k, str(k)): round(v[0] / a.steps, 4)
                                        for k, v in sorted(phases.items())})
+        if prec == "mixed":
+            res["fp32_classes_used"] = [L for L in range(7) if shape["fp32_classes"][L]]
         res["_phases"] = phases
+        res["_shape"] = shape
         if g is not None:
             d = V.cpu().numpy()[g["idx"]] - g["V"]
             res["max_abs_dV_au"] = float(np.abs(d).max())
@@ -538,9 +830,9 @@ def main() -> int:
             f"{res['ms_per_1e4_pts']:.3f} ms/1e4 pts, "
             f"max |dV| {res.get('max_abs_dV_au', float('nan')):.2e}")
         if force32:
-            res["fp32_classes"] = [1] * 7
             for L in range(7):  # back to the library default
                 ctx.set_tuning(20 + L, int(L >= 3))
+        tuned[f"{mode}_{'mixed32' if force32 else prec}"] = shape
         return res
 
     legs = {}
@@ -559,38 +851,40 @@ def main() -> int:
         log(f"copy probe: {a.copy_probe} MiB read + {a.copy_probe} MiB written")
     head = legs[key]
     ms_step = head["ms_per_step"]
+    grad = a.mode == "fwdbwd"
 
-    # --- roofline: the dominant launch of the headline leg (VALU) ------------
+    # --- roofline of the headline step: performed work of its launches -------
+    shape = head["_shape"]
+    autotune.apply(ctx, shape["ppl_L"], shape["target_wg"],
+                   shape["fp32_classes"] if a.precision == "mixed" else None)
+    counts = work_counts(lib, tab, pts.data_ptr(), n, grad, sh)
     ppl = info["pairs_per_L"]
-    fp32_used = [0, 0, 0, 1, 1, 1, 1]  # library default under mixed precision
-    if a.fp32_classes:
-        fp32_used = [int(x) for x in a.fp32_classes.split(",")]
-    if key in tuned:
-        ppl_used = tuned[key]["ppl_L"]
-        fp32_used = tuned[key]["fp32_classes"] or fp32_used
-    elif a.ppl_classes:
-        ppl_used = [int(x) for x in a.ppl_classes.split(",")]
-    else:
-        ppl_used = [a.ppl or 2] * 7
     traffic_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_hbm_traffic.md"
     counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
-    roof = dominant_kernel_roofline(head["_phases"], ppl, n, a.mode, a.precision, ppl_used,
-                                    fp32_used, flops, traffic_file, counters_file)
-    # whole step, for reference: all launches of one step against the vector peak
-    # (FP32 here is non-packed: flops.PEAK_FP32_VALU equals the FP64 peak)
-    f_fwd = flops.forward_flops(ppl, n, info["n_atoms"])
-    f_bwd = flops.backward_flops(ppl, n, info["n_atoms"]) if a.mode == "fwdbwd" else 0.0
-    fl = f_fwd + f_bwd
-    if a.precision == "mixed":
-        f32 = flops.fp32_share_fwd(ppl) * fl
-        peak_step = fl / ((fl - f32) / flops.PEAK_FP64_VALU + f32 / flops.PEAK_FP32_VALU)
-    else:
-        peak_step = flops.PEAK_FP64_VALU
-    roof_step = dict(achieved=round(fl / (ms_step * 1e-3) / 1e12, 3),
-                     peak=round(peak_step / 1e12, 2), unit="TFLOP/s",
-                     frac=round(fl / (ms_step * 1e-3) / peak_step, 4), flops_per_step=fl)
+    roof = roofline(head["_phases"], counts, grad, shape["ppl_L"], shape["fp32_classes"],
+                    a.precision, ppl, n, traffic_file, counters_file)
+    fl_launch = launch_flops(counts, grad)
+    f_nuc = n * info["n_atoms"] * (11 + (17 if grad else 0))
+    f_step = sum(fl_launch.values()) + f_nuc
+    f_model = flops.forward_flops(ppl, n, info["n_atoms"]) + (
+        flops.backward_flops(ppl, n, info["n_atoms"]) if grad else 0.0)
+    roof_step = dict(achieved=round(f_step / (ms_step * 1e-3) / 1e12, 3),
+                     peak=round(flops.PEAK_FP64_VALU / 1e12, 2), unit="TFLOP/s",
+                     frac=round(f_step / (ms_step * 1e-3) / flops.PEAK_FP64_VALU, 4),
+                     flops_per_step=f_step, model_flops_per_step=f_model,
+                     model_frac=round(f_model / (ms_step * 1e-3) / flops.PEAK_FP64_VALU, 4),
+                     far_share_eval=round(sum(counts["eval_far"]) /
+                                          max(1, sum(counts["eval_far"]) + sum(counts["eval_near"])), 4),
+                     far_share_dg=round(sum(counts["dg_far"]) /
+                                        max(1, sum(counts["dg_far"]) + sum(counts["dg_near"])), 4),
+                     per_launch={PHASE_NAMES[k]: dict(
+                         ms=round(head["_phases"][k][0] / head["_phases"][k][1], 4),
+                         tflops=round(v / (head["_phases"][k][0] / head["_phases"][k][1] * 1e-3) / 1e12, 2))
+                         for k, v in sorted(fl_launch.items()) if k in head["_phases"]},
+                     work_counts=counts)
     for leg in legs.values():
         leg.pop("_phases", None)
+        leg.pop("_shape", None)
 
     # sustained FMA rates of this device (independent chains): the FP32 the
     # mixed contraction can use is non-packed, so its per-instruction rate is
@@ -599,6 +893,44 @@ def main() -> int:
                  "fp32_nonpacked_tflops": round(probe_valu_rate(0, 1), 2)}
     log(f"measured VALU FMA rate: {valu_rate}")
 
+    # the public API from host memory beside the device-resident forward, same
+    # tuning (the FP64 forward leg's), same timer, alternating in one loop
+    host = None
+    if "fwd_fp64" in legs:
+        sf = legs["fwd_fp64"]["tuning"]
+        autotune.apply(ctx, sf["ppl_L"], sf["target_wg"])
+        reps = max(3, min(a.steps, 10))
+        td, th = [], []
+        for r in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), n, _lib.FE_PREC_FP64,
+                                          _lib.FE_PART_ESP, _lib.FE_PTR_DEVICE, sh, V.data_ptr()),
+                       "fe_esp_forward")
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            tab.esp(w.points, precision=_lib.FE_PREC_FP64)
+            t2 = time.perf_counter()
+            if r:
+                td.append(t1 - t0)
+                th.append(t2 - t1)
+        host = dict(api="PairTable.esp(points): host NumPy points in, host NumPy V out, FP64",
+                    calls=reps, ms_per_call=round(float(np.median(th)) * 1e3, 4),
+                    ms_per_1e4_pts=round(float(np.median(th)) * 1e3 / (n / 1e4), 4),
+                    device_resident_ms_per_call=round(float(np.median(td)) * 1e3, 4),
+                    timer="perf_counter around each call (device call followed by a "
+                          "synchronisation), medians, same launch shape")
+        log(f"host-boundary forward: {host['ms_per_call']} ms per call, device-resident "
+            f"{host['device_resident_ms_per_call']} ms")
+    V_gpu = None
+    if not a.no_cpu:
+        _lib.check(lib.fe_esp_forward(tab._h, pts.data_ptr(), n, _lib.FE_PREC_FP64,
+                                      _lib.FE_PART_ESP, _lib.FE_PTR_DEVICE, sh, V.data_ptr()),
+                   "fe_esp_forward")
+        V_gpu = V.cpu().numpy()
+    autotune.apply(ctx, shape["ppl_L"], shape["target_wg"])
+
+    ag = autograd_leg(w, torch, ctx, stream, a.autograd_steps) if a.autograd_steps else None
     batched = batch_leg(a, batch_ws, torch, ctx, stream) if batch_ws else None
     cfg3 = None
     if a.config3_steps:
@@ -608,29 +940,24 @@ def main() -> int:
     if a.density_steps:
         log("electron density leg")
         dens = density_leg(a, w, torch, ctx, stream, with_cpu=not a.no_cpu)
+    bpf = None
+    if a.build_legs:
+        autotune.apply(ctx, legs.get("fwd_fp64", head)["tuning"]["ppl_L"],
+                       legs.get("fwd_fp64", head)["tuning"]["target_wg"])
+        bpf = build_plus_fwd_legs([int(x) for x in a.build_legs.split(",")], torch, ctx, stream)
 
-    cpu = None
+    cpu = cpus = None
     if not a.no_cpu:
-        log("CPU baseline (C++/OpenMP port)")
-        cpu = cpu_baseline(w, a.cpu_seconds)
-
-    # the public API from host memory: NumPy points in, NumPy V out, so the
-    # transfers over the host link are inside the timing (reported beside
-    # `value`, never as it)
-    host = None
-    if "fwd_fp64" in legs:
-        tab.esp(w.points, precision=_lib.FE_PREC_FP64)
-        reps = max(3, min(a.steps, 10))
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            tab.esp(w.points, precision=_lib.FE_PREC_FP64)
-        dt = (time.perf_counter() - t0) / reps * 1e3
-        host = dict(api="PairTable.esp(points): host NumPy points in, host NumPy V out, FP64",
-                    calls=reps, ms_per_call=round(dt, 4), ms_per_1e4_pts=round(dt / (n / 1e4), 4),
-                    device_resident_fwd_ms=legs["fwd_fp64"]["ms_per_step"])
-        log(f"host-boundary forward: {dt:.3f} ms per call")
+        log("CPU baselines (C++/OpenMP port on every core, NumPy restatement on one)")
+        cpus = cpu_baselines(w, a.cpu_seconds, a.numpy_points, a.numpy_adjoint_points, V_gpu)
+        if a.mode == "fwdbwd":
+            cpu = dict(cpus["numpy_fwdbwd"])
+        else:
+            cpu = dict(cpus["numpy_fwd"])
+        cpu["cpu_model"] = cpus["cpu_model"]
 
     value = head["ms_per_1e4_pts"]
+    mixed = legs.get(f"{a.mode}_mixed")
     out = {
         "metric": "ESP ms/1e4 pts (1634 fns) FP64 vs mixed, fwd+bwd; max err vs FP64 oracle",
         "value": value,
@@ -647,12 +974,20 @@ def main() -> int:
         "config": {"workload": f"esp_{a.mode}_{a.precision}", "molecule": "config1 54 atoms",
                    "n_functions": int(w.basis.n), "n_points": n,
                    "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU"},
+        "max_abs_dV_vs_fp64_oracle_au": head.get("max_abs_dV_au"),
+        "mixed": None if mixed is None else dict(
+            ms_per_1e4_pts=mixed["ms_per_1e4_pts"], max_abs_dV_au=mixed.get("max_abs_dV_au"),
+            fp32_classes_used=mixed.get("fp32_classes_used"),
+            speedup_vs_fp64=round(head["ms_per_step"] / mixed["ms_per_step"], 3),
+            note="the tuner keeps FP32 only where it times faster; non-packed FP32 issues at "
+                 "the FP64 rate on this chip (valu_rate_measured), so mixed equals FP64 "
+                 "within noise"),
         "legs": legs,
         "tuning": tuned,
-        "speedup_mixed_vs_fp64_fwdbwd": (
-            round(legs["fwdbwd_fp64"]["ms_per_step"] / legs["fwdbwd_mixed"]["ms_per_step"], 3)
-            if "fwdbwd_fp64" in legs and "fwdbwd_mixed" in legs else None),
-        "table_build_ms": round(t_build * 1e3, 2),
+        "table_build_ms": round(t_build * 1e3, 3),
+        "table_build": build,
+        "build_plus_fwd": bpf,
+        "autograd": ag,
         "host_boundary_fwd": host,
         "roofline": roof,
         "roofline_step": roof_step,
@@ -661,16 +996,8 @@ def main() -> int:
         "config3": cfg3,
         "density": dens,
         "cpu_baseline": cpu,
+        "cpu_baselines": cpus,
     }
-    if "fwd_fp64" in legs and n == 30000:
-        # the reference's own published figure is table build + FP64 forward at
-        # 30000 points (2.4564 s on an RTX 5090 + 16 cores, benchmarks/README.md:52);
-        # a different metric from `value`, so vs_baseline stays null
-        ours = (t_build * 1e3 + legs["fwd_fp64"]["ms_per_step"]) / (n / 1e4)
-        out["reference_published_build_plus_fwd"] = dict(
-            unit="ms/1e4 pts", reference=819.0, this=round(ours, 3),
-            ratio=round(819.0 / ours, 1),
-            reference_setup="RTX 5090 + 16 CPU cores, C++/OpenMP build + CUDA FP64 forward")
     print(json.dumps(out), flush=True)
     return 0
 

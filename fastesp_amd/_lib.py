@@ -80,6 +80,7 @@ SIGNATURES = {
     "fe_batch_read": [_vp, C.c_int, C.c_int, _vp, _vp],
     "fe_batch_info": [_vp, C.POINTER(C.c_int64), C.c_int],
     "fe_profile": [_vp, C.c_int, _dp, C.POINTER(C.c_int64), C.c_int],
+    "fe_eval_work": [_vp, _vp, C.c_int64, C.c_int, C.c_int, _vp, C.POINTER(C.c_int64)],
     "fe_density_create": [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _vp,
                           C.c_int, _vp, C.POINTER(_vp)],
     "fe_density_eval": [_vp, _vp, C.c_int64, C.c_int, _vp, _vp],

@@ -350,3 +350,160 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
   }
   return FE_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Work counts (fe_eval_work): how many (point, pair) evaluations of each
+// class take the far-field path and how many the tabulated one, in the
+// evaluation launches (value + field, or value only) and in the dL/dG
+// launches.  The far test of both is per wavefront, so the counting kernels
+// repeat the launches' decompositions exactly: the same sorted points, points
+// per lane, pair chunks and pair blocks; only the arithmetic is left out.
+// Each wavefront-level decision is weighted by the wavefront's valid points
+// (evaluation) or valid pairs (dL/dG).
+// ---------------------------------------------------------------------------
+template <int M, int PPL>
+__global__ __launch_bounds__(FE_WG) void k_count_eval(const double* __restrict__ rec, const int S,
+                                                      const int* __restrict__ n_pairs_p,
+                                                      const double* __restrict__ pts,
+                                                      const int n_pts,
+                                                      unsigned long long* __restrict__ out) {
+  const int n_pairs = *n_pairs_p;
+  const int ppc = (n_pairs + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int p0 = blockIdx.y * ppc, p1 = min(n_pairs, p0 + ppc);
+  const int base = blockIdx.x * PPL * FE_WG;
+  double cx[PPL], cy[PPL], cz[PPL];
+  int nv = 0;
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int i = base + q * FE_WG + (int)threadIdx.x;
+    nv += i < n_pts;
+    const int ic = min(i, n_pts - 1);
+    cx[q] = pts[3 * (size_t)ic];
+    cy[q] = pts[3 * (size_t)ic + 1];
+    cz[q] = pts[3 * (size_t)ic + 2];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nv += __shfl_xor(nv, o, 64);
+  unsigned long long far = 0, near = 0;
+  for (int ip = p0; ip < p1; ++ip) {
+    const double* r = rec + (size_t)ip * S;
+    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3];
+    int nr = 0;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      const double x = cx[q] - Px, y = cy[q] - Py, z = cz[q] - Pz;
+      nr |= p * fma(x, x, fma(y, y, z * z)) < fe_tfar(M);
+    }
+    if (__any(nr)) ++near;
+    else ++far;
+  }
+  if ((threadIdx.x & 63) == 0 && nv > 0 && p1 > p0) {
+    atomicAdd(out, far * (unsigned long long)nv);
+    atomicAdd(out + 1, near * (unsigned long long)nv);
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_count_dg(const double* __restrict__ rec, const int S,
+                                                  const int* __restrict__ n_pairs_p,
+                                                  const int stride, const double* __restrict__ pts,
+                                                  const int n_pts, const int ppc,
+                                                  unsigned long long* __restrict__ out) {
+  const int n_pairs = min(*n_pairs_p, stride);
+  if ((int)(blockIdx.x * 256) >= n_pairs) return;  // block-uniform, as in k_dG
+  const int pair = blockIdx.x * 256 + threadIdx.x;
+  const int ip = min(pair, n_pairs - 1);
+  const double* r = rec + (size_t)ip * S;
+  const double Px = r[0], Py = r[1], Pz = r[2], p = r[3];
+  const int nvp = __popcll(__ballot(pair < n_pairs));
+  const int q0 = blockIdx.y * ppc, q1 = min(n_pts, q0 + ppc);
+  unsigned long long far = 0, near = 0;
+  for (int q = q0; q < q1; ++q) {
+    const double x = pts[3 * (size_t)q] - Px, y = pts[3 * (size_t)q + 1] - Py,
+                 z = pts[3 * (size_t)q + 2] - Pz;
+    if (__any(p * fma(x, x, fma(y, y, z * z)) < fe_tfar(L))) ++near;
+    else ++far;
+  }
+  if ((threadIdx.x & 63) == 0 && nvp > 0) {
+    atomicAdd(out, far * (unsigned long long)nvp);
+    atomicAdd(out + 1, near * (unsigned long long)nvp);
+  }
+}
+
+template <int L, int PPL>
+static void count_eval_LP(const fe_table* t, const FwdPlan& pl, const double* pts, int n, bool grad,
+                          unsigned long long* out, hipStream_t st) {
+  const dim3 grid(pl.n_pb[L], pl.chunks[L]);
+  const int S = fe_host::STRIDE[L];
+  const int* np = t->d_stats + FE_ST_KEPT + L;
+  if (grad)
+    hipLaunchKernelGGL((k_count_eval<L + 1, PPL>), grid, dim3(FE_WG), 0, st, t->d_rec[L], S, np,
+                       pts, n, out + 2 * L);
+  else
+    hipLaunchKernelGGL((k_count_eval<L, PPL>), grid, dim3(FE_WG), 0, st, t->d_rec[L], S, np, pts,
+                       n, out + 2 * L);
+}
+
+template <int L>
+static void count_dg_L(const fe_table* t, const DGPlan& d, const double* pts, int n,
+                       unsigned long long* out, hipStream_t st) {
+  const int64_t nL = fe_plan_n(t, L);
+  if (!nL) return;
+  hipLaunchKernelGGL(k_count_dg<L>, dim3((unsigned)((nL + 255) / 256), d.chunks[L]), dim3(256), 0,
+                     st, t->d_rec[L], fe_host::STRIDE[L], t->d_stats + FE_ST_KEPT + L, (int)nL,
+                     pts, n, d.ppc[L], out + 2 * (FE_LMAX + 1) + 2 * L);
+}
+
+extern "C" int fe_eval_work(fe_table* t, const double* pts, int64_t n, int grad, int ptr_kind,
+                            void* stream, int64_t* counts) {
+  if (!t || !counts || (n > 0 && !pts))
+    return fe_set_error(FE_ERR_ARG, "fe_eval_work: null argument");
+  if (n < 0 || n > (int64_t)INT32_MAX / 8)
+    return fe_set_error(FE_ERR_ARG, "fe_eval_work: bad point count");
+  if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
+    return fe_set_error(FE_ERR_ARG, "fe_eval_work: bad ptr_kind");
+  for (int i = 0; i < 4 * (FE_LMAX + 1); ++i) counts[i] = 0;
+  if (n == 0) return FE_OK;
+  fe_ctx* c = t->ctx;
+  FE_CTX_LOCK(c);
+  FE_HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  const double* d_pts = nullptr;
+  int rc = fe_stage_points(c, pts, n, ptr_kind, st, &d_pts);
+  if (rc) return rc;
+  // the evaluation sees the Morton-sorted points, dL/dG the caller's order
+  const double* s_pts = d_pts;
+  const int32_t* perm = nullptr;
+  if (c->sort_points && n >= 2 * FE_WG) {
+    rc = fe_sort_points(c, t, d_pts, (int)n, st, &s_pts, &perm);
+    if (rc) return rc;
+  }
+  rc = fe_ctx_reserve(c, FE_WS_AUX3, 4 * (FE_LMAX + 1) * sizeof(unsigned long long));
+  if (rc) return rc;
+  unsigned long long* d_out = (unsigned long long*)c->ws[FE_WS_AUX3];
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, (double*)d_out,
+                     (int64_t)(4 * (FE_LMAX + 1)), 0.0);
+  const FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
+  FE_FOR_EACH_CLASS(pl, count_eval_LP, t, pl, s_pts, (int)n, grad != 0, d_out, st);
+  const DGPlan d = plan_dG(t, n, c->target_wg);
+  count_dg_L<0>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<1>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<2>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<3>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<4>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<5>(t, d, d_pts, (int)n, d_out, st);
+  count_dg_L<6>(t, d, d_pts, (int)n, d_out, st);
+  FE_HIPCHK(hipGetLastError());
+  unsigned long long h[4 * (FE_LMAX + 1)];
+  FE_HIPCHK(hipMemcpyAsync(h, d_out, sizeof h, hipMemcpyDeviceToHost, st));
+  FE_HIPCHK(hipStreamSynchronize(st));
+  // counts: [far eval L][near eval L] x 7, then [far dG L][near dG L] x 7 ->
+  // out[L], out[7 + L], out[14 + L], out[21 + L]
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    counts[L] = (int64_t)h[2 * L];
+    counts[FE_LMAX + 1 + L] = (int64_t)h[2 * L + 1];
+    counts[2 * (FE_LMAX + 1) + L] = (int64_t)h[2 * (FE_LMAX + 1) + 2 * L];
+    counts[3 * (FE_LMAX + 1) + L] = (int64_t)h[2 * (FE_LMAX + 1) + 2 * L + 1];
+  }
+  return FE_OK;
+}

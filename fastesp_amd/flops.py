@@ -12,8 +12,23 @@ contraction) are FP32, the rest FP64.
 
 Bytes per pair record of the device table: 8 * STRIDE(L) (header, g and the
 integer multiples c g the field contraction reads).
+
+Performed work (``performed_flops``): the model above prices every (point,
+pair) as a tabulated-Boys evaluation with a term-by-term contraction.  The
+kernels take the exact far-field form whenever a whole wavefront is far
+(no table, no exponential) and contract with straight-line code that shares
+subexpressions between terms.  The performed-work count prices each
+evaluation by the path it took, counted by ``fe_eval_work`` with the same
+launch decompositions, at the operations the algorithm of that path needs:
+an FMA is 2 FLOPs, any other arithmetic operation 1; the contraction's
+operations are counted in the generated code (``csrc/fe_lclass.inc``).  It is
+the figure the roofline divides by time.
 """
 from __future__ import annotations
+
+import re
+from functools import lru_cache
+from pathlib import Path
 
 NTERM = [1, 4, 11, 24, 46, 80, 130]
 NMONO = [1, 4, 10, 20, 35, 56, 84]
@@ -81,6 +96,82 @@ def fp32_share_fwd(pairs_per_L) -> float:
 
 def table_bytes(pairs_per_L) -> int:
     return sum(8 * STRIDE[L] * n for L, n in enumerate(pairs_per_L))
+
+
+# ---------------------------------------------------------------------------
+# performed work
+# ---------------------------------------------------------------------------
+_INC = Path(__file__).resolve().parent / "csrc" / "fe_lclass.inc"
+GRAD_ACC_LMAX = 2   # classes whose value + field contraction folds the weight in (fe_internal.h)
+DG_SPLIT = [True, True, True, True, True, False, True]  # dL/dG with per-order scaling (fe_adjoint.h)
+
+
+def op_flops(code: str) -> int:
+    """FLOPs of straight-line code: 2 per fma(...), 1 per other arithmetic operator."""
+    n_fma = len(re.findall(r"\bfmaf?\(", code))
+    n_bin = len(re.findall(r"\s[-+*/]=?\s", code))
+    return 2 * n_fma + n_bin
+
+
+@lru_cache(maxsize=None)
+def generated_flops(name: str, L: int) -> int:
+    """FLOPs of one call of the generated device function ``name<L>``."""
+    txt = _INC.read_text()
+    m = re.search(r"\b" + re.escape(name) + r"<" + str(L) + r">\([^)]*\)\s*\{(.*?)\n\}", txt, re.S)
+    if not m:
+        raise KeyError(f"{name}<{L}> not in {_INC.name}")
+    return op_flops(m.group(1))
+
+
+R_T = 8  # R = C - P (3), r^2 = fma(x, x, fma(y, y, z z)) (5)
+RSQRT = 9  # hardware estimate + third-order Newton step (fe_rsqrt_pos)
+
+
+def boys_near(M: int) -> int:
+    """Tabulated Boys: T = p r^2, grid index and offset, 9-term Taylor of F_M,
+    and for M > 0 exp(-T) from the row's value and a degree-9 polynomial,
+    2T, and one fma + one multiply per downward step."""
+    b = 1 + 2 + 2 + 16
+    return b if M == 0 else b + 19 + 1 + 3 * M
+
+
+def boys_far(M: int) -> int:
+    """Far field: 1/|R| (rsqrt), x = 1/(2p) |R|^-2 (2), two multiplies per
+    upward step, and the weight K F_0 = K sqrt(pi/p)/2 |R|^-1 (1)."""
+    return RSQRT + 2 + 2 * M + 1
+
+
+def f_eval_path(L: int, grad: bool, far: bool) -> int:
+    """Performed FLOPs of one (point, pair) evaluation of class L on one path."""
+    M = L + 1 if grad else L
+    b = boys_far(M) if far else boys_near(M)
+    if grad:
+        if L <= GRAD_ACC_LMAX:
+            c = generated_flops("fe_contract_grad_acc", L)
+        else:
+            c = generated_flops("fe_contract_grad", L) + 8  # w I and w G accumulated (4 fma)
+    else:
+        c = generated_flops("fe_contract", L) + 2  # w s accumulated
+    return R_T + b + c
+
+
+def f_dg_path(L: int, far: bool) -> int:
+    """Performed FLOPs of one (pair, point) step of the dL/dG launch of class L."""
+    acc = generated_flops("fe_dg_accum", L)
+    if DG_SPLIT[L]:
+        w = (RSQRT + 2 + L) if far else (boys_near(L) + 2 * (L + 1))
+    else:
+        w = (RSQRT + 2 + 2 * L + 1 + (L + 1)) if far else (boys_near(L) + 1 + (L + 1))
+    return R_T + w + acc
+
+
+def performed_flops(kind: str, L: int, n_far: int, n_near: int) -> float:
+    """Performed FLOPs of a class launch from its far/tabulated evaluation
+    counts (``fe_eval_work``): kind 'fwd', 'fwd_grad' or 'dg'."""
+    if kind == "dg":
+        return float(n_far) * f_dg_path(L, True) + float(n_near) * f_dg_path(L, False)
+    grad = kind == "fwd_grad"
+    return float(n_far) * f_eval_path(L, grad, True) + float(n_near) * f_eval_path(L, grad, False)
 
 
 assert [f_fwd(L) for L in range(7)] == [34, 49, 79, 131, 215, 341, 522]

@@ -242,6 +242,18 @@ int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
  * recorded into a hipGraph are not timed. */
 int fe_profile(fe_ctx* ctx, int op, double* ms, int64_t* count, int n_phase);
 
+/* Work counts of the launches fe_esp_forward(_grad) and fe_esp_backward would
+ * make for these points with the context's current tuning (no counterpart in
+ * the reference; it prices the performed work of the roofline): counts[L] and
+ * counts[7 + L] are the (point, pair) evaluations of class L that the
+ * evaluation launch (value + field if grad != 0, else value only) takes on the
+ * far-field and on the tabulated path, counts[14 + L] and counts[21 + L] the
+ * same for the dL/dG launch.  Both launches decide the path per wavefront,
+ * and the counts follow the launches' decompositions exactly.  28 values;
+ * synchronises the stream. */
+int fe_eval_work(fe_table* t, const double* pts, int64_t n, int grad, int ptr_kind, void* stream,
+                 int64_t* counts);
+
 /* ---- batched conformers ------------------------------------------------
  * No counterpart in the reference (it evaluates one table per esp_gpu
  * process, fastesp/cli.py:143-163).  A batch references n_conf tables of one

@@ -67,6 +67,20 @@ class CpuTable:
         load().cp_export(self.h, p6.ctypes.data, g.ctypes.data)
         return p6, g
 
+    def oracle_table(self, wfn, points):
+        """The screened table as an oracle table (pair fields, terms in the
+        class key order) for ``oracle.evaluate``."""
+        from types import SimpleNamespace
+        from .ints import pair_type
+        p6, g = self.export()
+        L = p6[:, 5].astype(np.int32)
+        nt = np.array([pair_type(int(l))[0].shape[0] for l in range(7)])[L]
+        off = np.concatenate([[0], np.cumsum(nt)[:-1]]).astype(np.int64)
+        return SimpleNamespace(pp_exp=p6[:, 0], pp_cx=p6[:, 1], pp_cy=p6[:, 2], pp_cz=p6[:, 3],
+                               pp_K=p6[:, 4], pp_L=L, pp_off=off, tm_g=g, n_terms=len(g),
+                               n_pairs=len(L), pts=np.asarray(points, float),
+                               xyz=np.asarray(wfn.coords, float), Z=wfn.Z)
+
     def __del__(self):
         try:
             if self.h:

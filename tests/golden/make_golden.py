@@ -154,17 +154,8 @@ def make_cfg2_sub():
 
 
 def _port_table(ct, w):
-    """The C++ port's screened table as an oracle table (pair fields, terms in
-    the class key order) for ``evaluate.field`` / ``evaluate.esp``."""
-    from types import SimpleNamespace
-    from oracle.ints import pair_type
-    p6, g = ct.export()
-    L = p6[:, 5].astype(np.int32)
-    nt = np.array([pair_type(int(l))[0].shape[0] for l in range(7)])[L]
-    off = np.concatenate([[0], np.cumsum(nt)[:-1]]).astype(np.int64)
-    return SimpleNamespace(pp_exp=p6[:, 0], pp_cx=p6[:, 1], pp_cy=p6[:, 2], pp_cz=p6[:, 3],
-                           pp_K=p6[:, 4], pp_L=L, pp_off=off, tm_g=g, pts=w.points,
-                           xyz=np.asarray(w.wfn.coords, float), Z=w.wfn.Z)
+    """The C++ port's screened table as an oracle table for ``evaluate``."""
+    return ct.oracle_table(w.wfn, w.points)
 
 
 def _field_chunk(args):
