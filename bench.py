@@ -449,9 +449,9 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
     infos = [t.info() for t in tabs]
     b = Batch(tabs, [w.points for w in ws])
     n_tot = int(b.offsets[-1])
-    fl = sum(flops.forward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
-             + flops.backward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
-             for i, w in zip(infos, ws))
+    fl_model = sum(flops.forward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
+                   + flops.backward_flops(i["pairs_per_L"], len(w.points), i["n_atoms"])
+                   for i, w in zip(infos, ws))
     b.forward("fp64")
     V64 = b.values()
     rng = np.random.default_rng(4004)
@@ -467,7 +467,7 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
                functions_range=[int(min(w.basis.n for w in ws)), int(max(w.basis.n for w in ws))],
                points_range=[int(min(len(w.points) for w in ws)),
                              int(max(len(w.points) for w in ws))],
-               table_build_s=round(t_build, 2), flops_per_step=fl, legs={})
+               table_build_s=round(t_build, 2), model_flops_per_step=fl_model, legs={})
     tuned = None
     for prec in a.batch_precisions.split(","):
         if tuned is None:
@@ -477,6 +477,22 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
             log(f"batch autotune ({prec}): ppl_L={tuned['ppl_L']} fp32={tuned['mixed_L']} items/WG="
                 f"{tuned['items_per_wg']} dG items/WG={tuned['dg_items_per_wg']} "
                 f"({tuned['trials']} trials, {tuned['seconds']} s)")
+            # performed work: every conformer's far / tabulated evaluation
+            # counts with the single-table launch decomposition at the
+            # batch's points per lane (the batch's items tile the same
+            # Morton-sorted points and pair order)
+            lib = ctx.lib
+            autotune.apply(ctx, tuned["ppl_L"], 8192)
+            fl = 0.0
+            for t, w, i in zip(tabs, ws, infos):
+                d = torch.from_numpy(np.ascontiguousarray(w.points)).to("cuda")
+                c = work_counts(lib, t, d.data_ptr(), len(w.points), True, sh)
+                fl += sum(launch_flops(c, True).values()) + len(w.points) * i["n_atoms"] * 28
+            torch.cuda.synchronize()
+            out["flops_per_step"] = fl
+            out["flop_model"] = ("performed work (fe_eval_work counts x flops.f_eval_path / "
+                                 "f_dg_path, nuclear 28 per (point, atom)); "
+                                 "model_flops_per_step is the per-unit model")
         b.capture(prec, field=True, dldp=True)
         for _ in range(a.batch_warmup):
             b.launch(sh)
@@ -498,7 +514,9 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
         ph = ctx.profile_stop()
         leg = dict(ms_per_step=round(ms, 3), points_per_s=round(n_tot / (ms * 1e-3), 1),
                    ms_per_1e4_pts=round(ms / (n_tot / 1e4), 5),
-                   achieved_tflops=round(fl / (ms * 1e-3) / 1e12, 2),
+                   achieved_tflops=round(out["flops_per_step"] / (ms * 1e-3) / 1e12, 2),
+                   frac_fp64_peak=round(out["flops_per_step"] / (ms * 1e-3) / flops.PEAK_FP64_VALU, 4),
+                   model_tflops=round(fl_model / (ms * 1e-3) / 1e12, 2),
                    max_abs_dV_vs_fp64_au=float(errs.max()),
                    median_conformer_max_dV_au=float(np.median(errs)),
                    eager_step_phases_ms={PHASE_NAMES.get(k, str(k)): round(v[0], 3)

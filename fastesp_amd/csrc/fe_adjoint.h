@@ -44,6 +44,9 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   // kernel runs under a register cap (two waves per SIMD) keeps the weights
   // w K F_m with K F_0 folded in (no final scale).
   constexpr bool SPLIT = fe_dg_minw(L) == 1;
+  // T = p r^2 >= T_far(L)  <=>  r^2 >= T_far(L) / p: one compare per point
+  // (the product p r^2 is formed only on the tabulated path)
+  const double r2far = fe_tfar(L) * (hp + hp);
   double D[L + 1];
   if constexpr (SPLIT) {
     double c = KF;
@@ -56,29 +59,38 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   double acc[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) acc[k] = 0.0;
+  // far-field weights of one point (lanes with T >= T_far)
+  auto far_weights = [&](const double r2, const double w, double* wm) {
+    const double ri = fe_rsqrt_pos(r2);
+    if constexpr (SPLIT) {
+      const double u = ri * ri;
+      wm[0] = w * ri;
+#pragma unroll
+      for (int m = 1; m <= L; ++m) wm[m] = wm[m - 1] * u;
+    } else {
+      double F[L + 1];
+      fe_boys_far_scaled<L>(hp * (ri * ri), F);
+      const double wf = w * (KF * ri);
+#pragma unroll
+      for (int m = 0; m <= L; ++m) wm[m] = wf * F[m];
+    }
+  };
   auto point = [&](const double x, const double y, const double z, const double w) {
     const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
     const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
     double wm[L + 1];
-    if (!__any(p * r2 < fe_tfar(L))) {
-      const double ri = fe_rsqrt_pos(r2);
-      if constexpr (SPLIT) {
-        const double u = ri * ri;
-        wm[0] = w * ri;
-#pragma unroll
-        for (int m = 1; m <= L; ++m) wm[m] = wm[m - 1] * u;
-      } else {
-        double F[L + 1];
-        fe_boys_far_scaled<L>(hp * (ri * ri), F);
-        const double wf = w * (KF * ri);
-#pragma unroll
-        for (int m = 0; m <= L; ++m) wm[m] = wf * F[m];
-      }
-    } else {
+    const bool near = r2 < r2far;
+    if (!__any(near)) {
+      far_weights(r2, w, wm);
+    } else if (near) {
+      // a tabulated wavefront: its near lanes take the table, the others
+      // the same far-field weights as an all-far wavefront
       double F[L + 1];
-      fe_boys<L>(p * r2, F, s_tab);
+      fe_boys_tab<L>(p * r2, F, s_tab);
 #pragma unroll
       for (int m = 0; m <= L; ++m) wm[m] = SPLIT ? (w * F[m]) * D[m] : (w * K) * F[m];
+    } else {
+      far_weights(r2, w, wm);
     }
     fe_dg_accum<L>(Rx, Ry, Rz, wm, acc);
   };

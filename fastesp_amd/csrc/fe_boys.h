@@ -82,28 +82,35 @@ __device__ __forceinline__ void fe_boys_far_scaled(const double x, double* F) {
 // F[0..M] at T; `tab` holds at least the first fe_boys_rows(M) rows of the
 // table of top order M (normally staged in LDS).  The asymptotic form is used
 // from fe_tfar(M) on, where it is exact to double precision.
+// The tabulated form alone, for T < fe_tfar(M) (rounding may put T a few ulp
+// above it: the rows staged reach one row past the threshold).
+template <int M>
+__device__ __forceinline__ void fe_boys_tab(const double T, double* F, const double* tab) {
+  const int r = (int)fma(T, FE_BOYS_INVDT, 0.5);
+  const double dt = fma(-FE_BOYS_DT, (double)r, T);
+  const double* c = tab + r * FE_BOYS_COLS;
+  double f = c[8];
+  f = fma(f, dt, c[7]);
+  f = fma(f, dt, c[6]);
+  f = fma(f, dt, c[5]);
+  f = fma(f, dt, c[4]);
+  f = fma(f, dt, c[3]);
+  f = fma(f, dt, c[2]);
+  f = fma(f, dt, c[1]);
+  f = fma(f, dt, c[0]);
+  F[M] = f;
+  if constexpr (M > 0) {
+    const double e = c[9] * fe_exp_small(-dt);
+    const double t2 = T + T;
+#pragma unroll
+    for (int m = M; m > 0; --m) F[m - 1] = fma(t2, F[m], e) * (1.0 / (2 * m - 1));
+  }
+}
+
 template <int M>
 __device__ __forceinline__ void fe_boys(const double T, double* F, const double* tab) {
   if (T < fe_tfar(M)) {
-    const int r = (int)fma(T, FE_BOYS_INVDT, 0.5);
-    const double dt = fma(-FE_BOYS_DT, (double)r, T);
-    const double* c = tab + r * FE_BOYS_COLS;
-    double f = c[8];
-    f = fma(f, dt, c[7]);
-    f = fma(f, dt, c[6]);
-    f = fma(f, dt, c[5]);
-    f = fma(f, dt, c[4]);
-    f = fma(f, dt, c[3]);
-    f = fma(f, dt, c[2]);
-    f = fma(f, dt, c[1]);
-    f = fma(f, dt, c[0]);
-    F[M] = f;
-    if constexpr (M > 0) {
-      const double e = c[9] * fe_exp_small(-dt);
-      const double t2 = T + T;
-#pragma unroll
-      for (int m = M; m > 0; --m) F[m - 1] = fma(t2, F[m], e) * (1.0 / (2 * m - 1));
-    }
+    fe_boys_tab<M>(T, F, tab);
   } else {
     const double r = fe_rsqrt_pos(T);
     F[0] = 0.88622692545275801365 * r;  // sqrt(pi)/2

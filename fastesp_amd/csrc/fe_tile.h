@@ -35,6 +35,8 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
   for (int ip = p0; ip < p1; ++ip) {
     const fe_gdptr r = rec + (size_t)ip * S;
     const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
+    // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p (one product per pair)
+    const double r2far = fe_tfar(M) * (r[6] + r[6]);
     double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
     int near = 0, big = 0;
 #pragma unroll
@@ -43,7 +45,7 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
       Ry[q] = cy[q] - Py;
       Rz[q] = cz[q] - Pz;
       r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
-      near |= p * r2[q] < fe_tfar(M);
+      near |= r2[q] < r2far;
       if constexpr (MIXED) big |= (float)r[5] * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
     }
     const bool far = !__any(near);
