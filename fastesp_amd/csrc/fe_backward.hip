@@ -82,7 +82,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
     if (rc) return rc;
     fe_prof_end(c, ps, st);
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
+  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg, c->split_eval);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
   rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * plane * sizeof(double));
   if (rc) return rc;
@@ -316,13 +316,25 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
       hipLaunchKernelGGL(k_fill, dim3((unsigned)((t->n_terms + 255) / 256)), dim3(256), 0, st,
                          flat, (int64_t)t->n_terms, 0.0);
   } else {
-    launch_dG_L<0>(t, d, pw, (int)n, part, dG, flat, st);
+    // The classes are independent (own partial rows, own reduction), so they
+    // run on two streams with about the same work each: the tail of one
+    // class's launch fills with the other stream's workgroups.
+    hipStream_t s2 = st;
+    if (c->split_dg) {
+      rc = fe_fork(c, st, &s2);
+      if (rc) return rc;
+    }
     launch_dG_L<1>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dG_L<2>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dG_L<2>(t, d, pw, (int)n, part, dG, flat, s2);
     launch_dG_L<3>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dG_L<4>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dG_L<0>(t, d, pw, (int)n, part, dG, flat, s2);
     launch_dG_L<5>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dG_L<6>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dG_L<4>(t, d, pw, (int)n, part, dG, flat, s2);
+    launch_dG_L<6>(t, d, pw, (int)n, part, dG, flat, s2);
+    if (c->split_dg) {
+      rc = fe_join(c, st);
+      if (rc) return rc;
+    }
   }
   FE_HIPCHK(hipGetLastError());
   double* d_dP = dLdP;

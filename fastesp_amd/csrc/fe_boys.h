@@ -2,11 +2,14 @@
 //
 // Scheme (standard; Helgaker, Jorgensen & Olsen, "Molecular Electronic-Structure
 // Theory", sec. 9.8.2; Gill, Johnson & Pople, IJQC 40, 745 (1991)):
-//  * T < fe_tfar(M): nearest grid point T_r = r * dT (dT = 1/8), Taylor series
-//    of the top order F_M(T) = sum_{k<9} (-1)^k F_{M+k}(T_r) (T-T_r)^k / k!
-//    (|T-T_r| <= 1/16, truncation < 4e-17 relative), exp(-T) = exp(-T_r) *
-//    exp(-(T-T_r)) with a degree-9 polynomial, then the stable downward
-//    recursion F_{m-1} = (2T F_m + exp(-T)) / (2m-1).
+//  * T < fe_tfar(M): nearest grid point T_r = r * dT (dT = 1/8), a degree-7
+//    polynomial in dt = T - T_r (|dt| <= 1/16) for the top order F_M, exp(-T)
+//    = exp(-T_r) * a degree-7 polynomial of -dt, then the stable downward
+//    recursion F_{m-1} = (2T F_m + exp(-T)) / (2m-1).  Both polynomials are
+//    the degree-9 Taylor polynomials economized to degree 7 over |dt| <= 1/16
+//    (Chebyshev economization: the dt^9 and dt^8 terms are rewritten with
+//    T_9 and T_8, whose remainders h^9/256 and h^8/128 are dropped), so each
+//    is within 5e-17 relative of its function on the interval.
 //  * T >= fe_tfar(M) (35.5 .. 59 for top order M = 0 .. 8): asymptotic
 //    F_0 = sqrt(pi/T)/2 and the upward recursion F_m = F_{m-1} (2m-1)/(2T);
 //    the neglected incomplete-gamma tail is below 2^-54 relative for m <= M.
@@ -17,27 +20,29 @@
 // the tabulated expansion avoids.
 //
 // The grid values F_j(T_r) are computed on the host in 80-bit long double from
-// the all-positive series F_j(T) = exp(-T) sum_i (2T)^i / prod_{k<=i} (2j+2k+1).
+// the all-positive series F_j(T) = exp(-T) sum_i (2T)^i / prod_{k<=i} (2j+2k+1),
+// and the economization is done there in long double.
 #pragma once
 
 #define FE_BOYS_NR 481        // grid rows, T_r = r/8, r = 0..480 (T < 60)
-#define FE_BOYS_COLS 10       // 9 Taylor coefficients + exp(-T_r)
-#define FE_BOYS_NK 9
+#define FE_BOYS_COLS 9        // 8 polynomial coefficients + exp(-T_r)
+#define FE_BOYS_NK 8          // polynomial coefficients per row (degree 7)
+#define FE_BOYS_NTAYLOR 10    // Taylor terms the economization starts from
 #define FE_BOYS_MMAX 8        // highest top order with a table
 #define FE_BOYS_TSW 60.0
 #define FE_BOYS_DT 0.125
 #define FE_BOYS_INVDT 8.0
 
-// exp(x) for |x| <= 1/16, degree-9 Taylor polynomial (error < 3e-19 relative)
+// exp(x) for |x| <= 1/16: the degree-9 Taylor polynomial economized to degree
+// 7 (see above); max relative error 4.8e-17 on the interval (exact rational
+// coefficients rounded to double).
 __device__ __forceinline__ double fe_exp_small(const double x) {
-  double e = 2.7557319223985893e-06;         // 1/9!
-  e = fma(e, x, 2.4801587301587302e-05);     // 1/8!
-  e = fma(e, x, 1.9841269841269841e-04);     // 1/7!
-  e = fma(e, x, 1.3888888888888889e-03);     // 1/6!
-  e = fma(e, x, 8.3333333333333333e-03);     // 1/5!
-  e = fma(e, x, 4.1666666666666667e-02);     // 1/4!
-  e = fma(e, x, 1.6666666666666667e-01);     // 1/3!
-  e = fma(e, x, 0.5);
+  double e = 1.9843691871279762e-04;
+  e = fma(e, x, 1.3890826512896826e-03);
+  e = fma(e, x, 8.3333332623754235e-03);
+  e = fma(e, x, 4.1666666193613928e-02);
+  e = fma(e, x, 1.6666666666674365e-01);
+  e = fma(e, x, 5.0000000000036959e-01);
   e = fma(e, x, 1.0);
   return fma(e, x, 1.0);
 }
@@ -89,8 +94,7 @@ __device__ __forceinline__ void fe_boys_tab(const double T, double* F, const dou
   const int r = (int)fma(T, FE_BOYS_INVDT, 0.5);
   const double dt = fma(-FE_BOYS_DT, (double)r, T);
   const double* c = tab + r * FE_BOYS_COLS;
-  double f = c[8];
-  f = fma(f, dt, c[7]);
+  double f = c[7];
   f = fma(f, dt, c[6]);
   f = fma(f, dt, c[5]);
   f = fma(f, dt, c[4]);
@@ -100,7 +104,7 @@ __device__ __forceinline__ void fe_boys_tab(const double T, double* F, const dou
   f = fma(f, dt, c[0]);
   F[M] = f;
   if constexpr (M > 0) {
-    const double e = c[9] * fe_exp_small(-dt);
+    const double e = c[8] * fe_exp_small(-dt);
     const double t2 = T + T;
 #pragma unroll
     for (int m = M; m > 0; --m) F[m - 1] = fma(t2, F[m], e) * (1.0 / (2 * m - 1));

@@ -75,20 +75,45 @@ static void boys_series(long double T, int jmax, long double* F) {
   }
 }
 
+// Degree-(n-1) polynomial c[0..n-1] in dt economized to degree 7 over
+// |dt| <= h: the dt^9 and dt^8 terms are rewritten through the Chebyshev
+// polynomials (x = dt/h)
+//   x^9 = T_9/256 + 9/4 x^7 - 27/16 x^5 + 15/32 x^3 - 9/256 x,
+//   x^8 = T_8/128 + 2 x^6 - 5/4 x^4 + 1/4 x^2 - 1/128,
+// and the T_9, T_8 parts (at most h^9/256 |c_9|, h^8/128 |c_8|) dropped.
+static void economize_to_7(long double* c, const long double h) {
+  const long double h2 = h * h, h4 = h2 * h2, h6 = h4 * h2, h8 = h4 * h4;
+  const long double a9 = c[9];
+  c[7] += a9 * 2.25L * h2;
+  c[5] -= a9 * 1.6875L * h4;
+  c[3] += a9 * 0.46875L * h6;
+  c[1] -= a9 * 0.03515625L * h8;
+  const long double a8 = c[8];
+  c[6] += a8 * 2.0L * h2;
+  c[4] -= a8 * 1.25L * h4;
+  c[2] += a8 * 0.25L * h6;
+  c[0] -= a8 * h8 / 128.0L;
+  c[8] = c[9] = 0.0L;
+}
+
 static std::vector<double> make_boys_tables() {
-  const int jmax = FE_BOYS_MMAX + FE_BOYS_NK - 1;
+  const int jmax = FE_BOYS_MMAX + FE_BOYS_NTAYLOR - 1;
   std::vector<double> tab((size_t)(FE_BOYS_MMAX + 1) * FE_BOYS_NR * FE_BOYS_COLS);
   std::vector<long double> F(jmax + 1);
-  long double fact[FE_BOYS_NK];
+  long double fact[FE_BOYS_NTAYLOR];
   fact[0] = 1.0L;
-  for (int k = 1; k < FE_BOYS_NK; ++k) fact[k] = fact[k - 1] * k;
+  for (int k = 1; k < FE_BOYS_NTAYLOR; ++k) fact[k] = fact[k - 1] * k;
   for (int r = 0; r < FE_BOYS_NR; ++r) {
     const long double T = (long double)r / 8.0L;
     boys_series(T, jmax, F.data());
     for (int M = 0; M <= FE_BOYS_MMAX; ++M) {
+      // Taylor coefficients of F_M(T_r + dt): d^k F_M / dT^k = (-1)^k F_{M+k}
+      long double c[FE_BOYS_NTAYLOR];
+      for (int k = 0; k < FE_BOYS_NTAYLOR; ++k)
+        c[k] = ((k & 1) ? -1.0L : 1.0L) * F[M + k] / fact[k];
+      economize_to_7(c, 0.5L * (long double)FE_BOYS_DT);
       double* row = &tab[((size_t)M * FE_BOYS_NR + r) * FE_BOYS_COLS];
-      for (int k = 0; k < FE_BOYS_NK; ++k)
-        row[k] = (double)(((k & 1) ? -1.0L : 1.0L) * F[M + k] / fact[k]);
+      for (int k = 0; k < FE_BOYS_NK; ++k) row[k] = (double)c[k];
       row[FE_BOYS_NK] = (double)expl(-T);
     }
   }
@@ -282,6 +307,12 @@ extern "C" int fe_init(int device, fe_ctx** out) {
 extern "C" int fe_finalize(fe_ctx* c) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_finalize: null context");
   FE_HIPCHK(hipSetDevice(c->device));
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);  // its launches read the workspace slots
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   for (int s = 0; s < FE_WS_SLOTS; ++s)
     if (c->ws[s]) (void)hipFree(c->ws[s]);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -314,6 +345,24 @@ extern "C" int fe_ctx_trim(fe_ctx* c) {
   return FE_OK;
 }
 
+int fe_fork(fe_ctx* c, hipStream_t st, hipStream_t* side) {
+  if (!c->side) {
+    FE_HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    FE_HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    FE_HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  FE_HIPCHK(hipEventRecord(c->ev_fork, st));
+  FE_HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  *side = c->side;
+  return FE_OK;
+}
+
+int fe_join(fe_ctx* c, hipStream_t st) {
+  FE_HIPCHK(hipEventRecord(c->ev_join, c->side));
+  FE_HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
+  return FE_OK;
+}
+
 extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_set_tuning: null context");
   FE_CTX_LOCK(c);
@@ -338,6 +387,12 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       return FE_OK;
     case 5:
       c->pair_order = value != 0;
+      return FE_OK;
+    case 6:
+      c->split_dg = value != 0;
+      return FE_OK;
+    case 7:
+      c->split_eval = value != 0;
       return FE_OK;
     case 20: case 21: case 22: case 23: case 24: case 25: case 26:  // FP32 contraction of class L
       c->mixed_L[key - 20] = value != 0;
@@ -751,10 +806,10 @@ __global__ __launch_bounds__(256) void k_boys_eval(const double* __restrict__ T,
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, bool split) {
   FwdPlan pl{};
   pl.ld = (int)((n + 1023) / 1024 * 1024);
-  int rows = 0;
+  pl.split = split;
   for (int L = 0; L <= FE_LMAX; ++L) {
     const int64_t nL = fe_plan_n(t, L);
     pl.ppl[L] = (ppl[L] == 4 || ppl[L] == 2) ? ppl[L] : 1;
@@ -771,8 +826,22 @@ FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg) {
     c = (nL + ppc - 1) / ppc;
     pl.ppc[L] = (int)ppc;
     pl.chunks[L] = (int)c;
-    pl.nadd[L] = rows;
-    rows = std::max(rows, (int)c);
+  }
+  // rows: chunk c of a class writes row c of its set if no class launched
+  // before it on the same set had a chunk c, else adds into it
+  int rows = 0;
+  const int n_sets = split ? 2 : 1;
+  for (int g = 0; g < n_sets; ++g) {
+    const int* order = split ? FE_ORDER_SPLIT[g] : FE_ORDER_ONE;
+    int set_rows = 0;
+    for (int k = 0; order[k] >= 0; ++k) {
+      const int L = order[k];
+      if (L > FE_LMAX) continue;
+      pl.row0[L] = rows;
+      pl.nadd[L] = set_rows;
+      set_rows = std::max(set_rows, pl.chunks[L]);
+    }
+    rows += set_rows;
   }
   pl.rows = rows;
   return pl;
@@ -824,7 +893,7 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
     if (rc) return rc;
     fe_prof_end(c, ps, st);
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
+  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg, c->split_eval);
   double* partial = nullptr;
   if ((parts & FE_PART_ELECTRON) && pl.rows > 0) {
     rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)pl.rows * pl.ld * sizeof(double));

@@ -87,6 +87,21 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
   fe_prof_end(t->ctx, ph, st);
 }
 
+static void launch_eval_class(int L, const fe_table* t, const FwdPlan& pl, const double* pts,
+                              int n, double* partial, size_t plane, bool grad, bool mixed,
+                              float tau, hipStream_t st) {
+  double* rows = partial + (size_t)pl.row0[L] * pl.ld;
+  switch (L) {
+    case 0: FE_DISPATCH_PPL(pl, 0, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    case 1: FE_DISPATCH_PPL(pl, 1, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    case 2: FE_DISPATCH_PPL(pl, 2, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    case 3: FE_DISPATCH_PPL(pl, 3, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    case 4: FE_DISPATCH_PPL(pl, 4, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    case 5: FE_DISPATCH_PPL(pl, 5, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+    default: FE_DISPATCH_PPL(pl, 6, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
+  }
+}
+
 int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, double* partial,
                    size_t plane, bool grad, bool mixed, hipStream_t st) {
   if (mixed) {
@@ -94,11 +109,32 @@ int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, dou
     if (rc) return rc;
   }
   const float tau = (float)t->ctx->mixed_tau;
-  FE_FOR_EACH_CLASS(pl, launch_eval_LP, t, pl, pts, n, partial, plane, grad, mixed, tau, st);
+  if (!pl.split) {
+    for (int k = 0; FE_ORDER_ONE[k] >= 0; ++k)
+      if (FE_ORDER_ONE[k] <= FE_LMAX)
+        launch_eval_class(FE_ORDER_ONE[k], t, pl, pts, n, partial, plane, grad, mixed, tau, st);
+  } else {
+    // the two row sets are independent: the odd classes on the caller's
+    // stream, the even ones on the context's second stream
+    hipStream_t s2 = st;
+    int rc = fe_fork(t->ctx, st, &s2);
+    if (rc) return rc;
+    const hipStream_t sg[2] = {st, s2};
+    // interleaved, so both streams have work queued from the start
+    for (int k = 0; k <= FE_LMAX; ++k)
+      for (int g = 0; g < 2; ++g) {
+        int len = 0;
+        while (FE_ORDER_SPLIT[g][len] >= 0) ++len;
+        if (k < len && FE_ORDER_SPLIT[g][k] <= FE_LMAX)
+          launch_eval_class(FE_ORDER_SPLIT[g][k], t, pl, pts, n, partial, plane, grad, mixed, tau,
+                            sg[g]);
+      }
+    rc = fe_join(t->ctx, st);
+    if (rc) return rc;
+  }
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }
-
 template <int L, int PPL, bool G, bool X>
 static int batch_LPGX(const FeBatchEvalArgs* a, hipStream_t st) {
   if (!a) {

@@ -149,7 +149,20 @@ struct fe_ctx {
   // the last basis fe_table_build set up (host layout + device copy), reused
   // when the next build passes the same basis arrays
   std::shared_ptr<void> basis_cache;
+  // A second stream for launches that are independent of each other (the
+  // dL/dG classes): fe_fork makes it wait for the caller's stream, fe_join
+  // makes the caller's stream wait for it.  Default priority, non-blocking;
+  // created at the first fork.  Under graph capture the fork and join become
+  // graph edges.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool split_dg = true;    // run the dL/dG classes on two streams (tuning key 6)
+  bool split_eval = false;  // run the evaluation classes on two streams (tuning key 7)
 };
+
+// fork / join of the context's second stream with the caller's stream st
+int fe_fork(fe_ctx* c, hipStream_t st, hipStream_t* side);
+int fe_join(fe_ctx* c, hipStream_t st);
 
 #define FE_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> fe_ctx_lock_((c)->mu)
 
@@ -237,14 +250,22 @@ int fe_refresh_g32(fe_table* t, hipStream_t st);
 // chunks[L]) and adds into rows [0, nadd[L]), which earlier classes wrote, so
 // each row holds its classes' sums in a fixed order and the reduction reads
 // max(chunks) rows rather than one set per class.
+// Launch order of the class launches: one stream (0..6), or two streams
+// (split), the odd classes on the caller's stream and the even ones on the
+// context's second stream, each set with rows of its own.  -1 ends a list.
+static constexpr int FE_ORDER_ONE[FE_LMAX + 2] = {0, 1, 2, 3, 4, 5, 6, -1};
+static constexpr int FE_ORDER_SPLIT[2][FE_LMAX + 2] = {{1, 3, 5, -1}, {2, 0, 4, 6, -1}};
+
 struct FwdPlan {
   int ld = 0;     // leading dimension of a partial row (>= n, multiple of 1024)
-  int rows = 0;   // partial rows (the largest chunk count of a class)
+  int rows = 0;   // partial rows (all row sets)
+  bool split = false;           // two streams, two row sets
   int n_pb[FE_LMAX + 1] = {};   // point blocks of the class launch
   int ppl[FE_LMAX + 1] = {};    // points per lane of the class launch (1, 2, 4)
   int chunks[FE_LMAX + 1] = {};
   int ppc[FE_LMAX + 1] = {};
-  int nadd[FE_LMAX + 1] = {};   // rows already holding sums when class L runs
+  int nadd[FE_LMAX + 1] = {};   // rows of its set already holding sums when class L runs
+  int row0[FE_LMAX + 1] = {};   // first row of class L's row set
 };
 
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes);
@@ -255,7 +276,7 @@ int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* 
 // adjoint, whose far-field test is per wavefront.  rec4 holds (Px, Py, Pz, p)
 // at a stride of `stride` doubles; returns a stable permutation.
 std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n);
-FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg);
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, bool split = false);
 
 // dispatch a templated class launcher on the plan's points-per-lane of class L
 #define FE_DISPATCH_PPL(pl, L, FN, ...)                  \
