@@ -94,6 +94,9 @@ def parse():
                     help="point counts of the build + FP64 forward legs ('' = skip)")
     ap.add_argument("--autograd-steps", type=int, default=10,
                     help="timed steps of the autograd / graph-captured leg (0 = skip it)")
+    ap.add_argument("--dg-streams", type=int, choices=[1, 2], default=2,
+                    help="streams of the dL/dG class launches (2 = library default; 1 runs "
+                         "them one at a time, so per-kernel counters and times are their own)")
     return ap.parse_args()
 
 
@@ -691,6 +694,7 @@ def main() -> int:
     log(f"generating workload (config 1, {a.points} points)")
     w = synth.config1(a.points)
     ctx = _lib.Context.get(0)
+    ctx.set_tuning(6, int(a.dg_streams == 2))
     if a.ppl:
         ctx.set_tuning(0, a.ppl)
     if a.ppl_classes:
@@ -891,8 +895,8 @@ def main() -> int:
     ppl = info["pairs_per_L"]
     traffic_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_hbm_traffic.md"
     counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
-    # the dL/dG classes run on two streams (library default), two at a time
-    overlapped = frozenset(16 + L for L in range(7)) if grad else frozenset()
+    # with two streams the dL/dG classes run two at a time
+    overlapped = frozenset(16 + L for L in range(7)) if grad and a.dg_streams == 2 else frozenset()
     roof = roofline(head["_phases"], counts, grad, shape["ppl_L"], shape["fp32_classes"],
                     a.precision, ppl, n, traffic_file, counters_file, overlapped)
     fl_launch = launch_flops(counts, grad)
@@ -1008,7 +1012,8 @@ def main() -> int:
         "data": "synthetic (seeded config-1 molecule, density and vdW-surface points)",
         "config": {"workload": f"esp_{a.mode}_{a.precision}", "molecule": "config1 54 atoms",
                    "n_functions": int(w.basis.n), "n_points": n,
-                   "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU"},
+                   "n_pairs_per_L": ppl, "tol": 1e-12, "parallelism": "single GPU",
+                   "dg_streams": a.dg_streams},
         "max_abs_dV_vs_fp64_oracle_au": head.get("max_abs_dV_au"),
         "mixed": None if mixed is None else dict(
             ms_per_1e4_pts=mixed["ms_per_1e4_pts"], max_abs_dV_au=mixed.get("max_abs_dV_au"),

@@ -243,7 +243,10 @@ static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, in
 // ---------------------------------------------------------------------------
 // dL/dG -> dL/dP, one workgroup (64 lanes) per shell pair
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_dg_to_dp(const DPArgs a) { fe_dg_to_dp_sp(a, blockIdx.x); }
+// the shell pairs sp_list[0 .. gridDim.x) (one class: their dL/dG is ready)
+__global__ __launch_bounds__(64) void k_dg_to_dp(const DPArgs a, const int* __restrict__ sp_list) {
+  fe_dg_to_dp_sp(a, sp_list[blockIdx.x]);
+}
 
 // ---------------------------------------------------------------------------
 extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const double* gV,
@@ -308,6 +311,24 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
       flat = (double*)c->ws[FE_WS_AUX3];
     }
   }
+  // dL/dP: every shell pair writes its own block of dL/dP (and its mirror), so
+  // the shell pairs of class L run as soon as class L's dL/dG is reduced, on
+  // the same stream
+  double* d_dP = dLdP;
+  if (dLdP && ptr_kind == FE_PTR_HOST) {
+    rc = fe_ctx_reserve(c, FE_WS_OUT, (size_t)t->n_func * t->n_func * sizeof(double));
+    if (rc) return rc;
+    d_dP = (double*)c->ws[FE_WS_OUT];
+  }
+  const DPArgs pa = dLdP ? fe_dp_args(t, dG, d.dg_off, d_dP) : DPArgs{};
+  auto launch_dp = [&](int L, hipStream_t s) {
+    const int cnt = t->sp_cls_off[L + 1] - t->sp_cls_off[L];
+    if (!dLdP || cnt <= 0) return;
+    const int pd = fe_prof_begin(c, FE_PH_DP, s);
+    hipLaunchKernelGGL(k_dg_to_dp, dim3((unsigned)cnt), dim3(64), 0, s, pa,
+                       t->b.sp_by_class + t->sp_cls_off[L]);
+    fe_prof_end(c, pd, s);
+  };
   if (n == 0) {
     if (d.dg_total)
       hipLaunchKernelGGL(k_fill, dim3((unsigned)((d.dg_total + 255) / 256)), dim3(256), 0, st, dG,
@@ -315,42 +336,38 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     if (flat && t->n_terms)
       hipLaunchKernelGGL(k_fill, dim3((unsigned)((t->n_terms + 255) / 256)), dim3(256), 0, st,
                          flat, (int64_t)t->n_terms, 0.0);
+    for (int L = 0; L <= FE_LMAX; ++L) launch_dp(L, st);
   } else {
-    // The classes are independent (own partial rows, own reduction), so they
-    // run on two streams with about the same work each: the tail of one
-    // class's launch fills with the other stream's workgroups.
+    // The classes are independent (own partial rows, own reduction, own
+    // shell pairs), so they run on two streams with about the same work
+    // each: the tail of one class's launch fills with the other stream's
+    // workgroups, and the latency-bound dL/dP of a class runs beside the
+    // other stream's dL/dG.
     hipStream_t s2 = st;
     if (c->split_dg) {
       rc = fe_fork(c, st, &s2);
       if (rc) return rc;
     }
     launch_dG_L<1>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dp(1, st);
     launch_dG_L<2>(t, d, pw, (int)n, part, dG, flat, s2);
+    launch_dp(2, s2);
     launch_dG_L<3>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dp(3, st);
     launch_dG_L<0>(t, d, pw, (int)n, part, dG, flat, s2);
+    launch_dp(0, s2);
     launch_dG_L<5>(t, d, pw, (int)n, part, dG, flat, st);
+    launch_dp(5, st);
     launch_dG_L<4>(t, d, pw, (int)n, part, dG, flat, s2);
+    launch_dp(4, s2);
     launch_dG_L<6>(t, d, pw, (int)n, part, dG, flat, s2);
+    launch_dp(6, s2);
     if (c->split_dg) {
       rc = fe_join(c, st);
       if (rc) return rc;
     }
   }
   FE_HIPCHK(hipGetLastError());
-  double* d_dP = dLdP;
-  if (dLdP) {
-    const size_t pb = (size_t)t->n_func * t->n_func * sizeof(double);
-    if (ptr_kind == FE_PTR_HOST) {
-      rc = fe_ctx_reserve(c, FE_WS_OUT, pb);
-      if (rc) return rc;
-      d_dP = (double*)c->ws[FE_WS_OUT];
-    }
-    const DPArgs pa = fe_dp_args(t, dG, d.dg_off, d_dP);
-    const int pd = fe_prof_begin(c, FE_PH_DP, st);
-    if (t->n_sp > 0) hipLaunchKernelGGL(k_dg_to_dp, dim3(t->n_sp), dim3(64), 0, st, pa);
-    fe_prof_end(c, pd, st);
-    FE_HIPCHK(hipGetLastError());
-  }
   if (ptr_kind == FE_PTR_HOST) {
     if (dLdG && t->n_terms)
       FE_HIPCHK(hipMemcpyAsync(dLdG, flat, (size_t)t->n_terms * sizeof(double),

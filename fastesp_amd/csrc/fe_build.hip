@@ -800,7 +800,9 @@ struct BasisSetup {
   void* mem = nullptr;
   size_t bytes = 0;
   size_t o_atom = 0, o_l = 0, o_fn0 = 0, o_nfn = 0, o_p0 = 0, o_np = 0, o_c = 0, o_pexp = 0,
-         o_pcoef = 0, o_lx = 0, o_ly = 0, o_lz = 0, o_spi = 0, o_spj = 0, o_sps = 0, o_spg = 0;
+         o_pcoef = 0, o_lx = 0, o_ly = 0, o_lz = 0, o_spi = 0, o_spj = 0, o_sps = 0, o_spg = 0,
+         o_spc = 0;
+  int sp_cls_off[FE_LMAX + 2] = {};  // shell pairs of class L: sp_by_class[off[L], off[L+1])
   ~BasisSetup() {
     if (mem) {
       (void)hipSetDevice(device);
@@ -897,6 +899,16 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
         return fe_set_error(FE_ERR_UNSUPPORTED, "too many primitive pairs (%lld)", (long long)gtot);
     }
   const int nsp = (int)nsp_all;
+  // the shell pairs grouped by class (reference order within a class): the
+  // dL/dP launch of class L needs only class L's dL/dG
+  std::vector<int> sp_cls(nsp_all);
+  {
+    int cnt[FE_LMAX + 2] = {};
+    for (size_t q = 0; q < nsp_all; ++q) ++cnt[sh[sp_i[q]].l + sh[sp_j[q]].l + 1];
+    for (int L = 0; L <= FE_LMAX; ++L) cnt[L + 1] += cnt[L];
+    for (int L = 0; L <= FE_LMAX + 1; ++L) bs->sp_cls_off[L] = cnt[L];
+    for (size_t q = 0; q < nsp_all; ++q) sp_cls[cnt[sh[sp_i[q]].l + sh[sp_j[q]].l]++] = (int)q;
+  }
   bs->device = c->device;
   bs->n_func = n_func;
   bs->n_prim = n_prim;
@@ -942,6 +954,7 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
   bs->o_spj = push_bytes(hb, sp_j.data(), nsp);
   bs->o_sps = push_bytes(hb, sp_slot.data(), nsp);
   bs->o_spg = push_bytes(hb, sp_gidx.data(), nsp);
+  bs->o_spc = push_bytes(hb, sp_cls.data(), nsp);
   if (hipMalloc(&bs->mem, hb.size()) != hipSuccess) {
     (void)hipGetLastError();
     bs->mem = nullptr;
@@ -1023,6 +1036,8 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   t->b.fn_ly = (const int*)(bm + bs->o_ly); t->b.fn_lz = (const int*)(bm + bs->o_lz);
   t->b.sp_i = (const int*)(bm + bs->o_spi); t->b.sp_j = (const int*)(bm + bs->o_spj);
   t->b.sp_slot = (const int*)(bm + bs->o_sps); t->b.sp_gidx = (const int*)(bm + bs->o_spg);
+  t->b.sp_by_class = (const int*)(bm + bs->o_spc);
+  for (int L = 0; L <= FE_LMAX + 1; ++L) t->sp_cls_off[L] = bs->sp_cls_off[L];
   auto fail = [&](int rc) {
     (void)hipStreamSynchronize(st);
     fe_table_free(t);

@@ -135,7 +135,11 @@ def test_phase_profile_counts_every_class_launch():
     for L in present:
         assert ph[8 + L][1] == 1 and ph[8 + L][0] > 0
         assert ph[16 + L][1] == 1 and ph[24 + L][1] == 1
-    assert ph[1][1] == 1 and ph[2][1] == 1 and ph[3][1] == 1
+    # dL/dP runs once per class that has shell pairs (each class's shell pairs
+    # start as soon as that class's dL/dG is reduced)
+    ls = sorted({s.l for s in w.wfn.shells})
+    sp_classes = {a + b for a in ls for b in ls}
+    assert ph[1][1] == 1 and ph[2][1] == 1 and ph[3][1] == len(sp_classes)
     ctx.profile_start()
     assert ctx.profile_stop() == {}
 
