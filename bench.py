@@ -530,6 +530,39 @@ def batch_leg(a, ws, torch, ctx, stream) -> dict:
         log(f"batch {prec}: {ms:.1f} ms/step = {leg['points_per_s']:.3e} points/s, "
             f"max |dV| vs FP64 {errs.max():.2e}")
     out["tuning"] = {k: v for k, v in tuned.items() if k != "all"} if tuned else None
+    # The same step with every table built again on the device inside the
+    # graph (build -> forward -> loss -> backward): the densities go to the
+    # batch once; the first call gives each table its capacity layout, after
+    # which the plans are cut from capacities.  Timed with and without the
+    # builds at those plans, so their difference is the builds' share.
+    prec = a.batch_precisions.split(",")[0]
+    t1 = time.perf_counter()
+    b.set_density([w.density for w in ws], tol=1e-12)
+    t_first = time.perf_counter() - t1
+    rb = {}
+    for flag in (False, True):
+        b.capture(prec, field=True, dldp=True, rebuild=flag)
+        for _ in range(a.batch_warmup):
+            b.launch(sh)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.batch_steps):
+            b.launch(sh)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rb[flag] = e0.elapsed_time(e1) / a.batch_steps
+    errs = np.array([np.abs(v - r).max() if len(v) else 0.0 for v, r in zip(b.values(), V64)])
+    out["build_in_graph"] = dict(
+        precision=prec, step_ms=round(rb[True], 3), step_without_builds_ms=round(rb[False], 3),
+        builds_ms=round(rb[True] - rb[False], 3),
+        builds_host_api_s=round(t_build, 3), first_set_density_s=round(t_first, 3),
+        max_abs_dV_vs_fp64_au=float(errs.max()),
+        graph="one hipGraph: 256 x fe_table_rebuild -> forward + field -> loss -> dL/dG -> dL/dP",
+        note="the same densities as the tables' own builds, so the kept sets are the same")
+    log(f"batch step with the 256 builds in the graph: {rb[True]:.1f} ms "
+        f"(builds {rb[True] - rb[False]:.1f} ms; {t_build:.2f} s through the host API)")
     b.close()
     for t in tabs:
         t.close()

@@ -30,6 +30,7 @@ FE_BOYS_MMAX = 8
 FE_NOALLOC_MAX_POINTS = 65536
 FE_BATCH_FIELD = 1
 FE_BATCH_DLDP = 2
+FE_BATCH_REBUILD = 4
 FE_BOUT_V, FE_BOUT_FIELD, FE_BOUT_DLDP, FE_BOUT_LOSS, FE_BOUT_GV = range(5)
 FE_BT_ITEMS_PER_WG = 1
 FE_BT_DG_ITEMS_PER_WG = 2
@@ -71,6 +72,7 @@ SIGNATURES = {
     "fe_batch_set_points": [_vp, _vp, C.c_int, _vp],
     "fe_batch_set_loss": [_vp, _vp, _vp, C.c_int, _vp],
     "fe_batch_set_tuning": [_vp, C.c_int, C.c_int64],
+    "fe_batch_set_density": [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp],
     "fe_batch_forward": [_vp, C.c_int, C.c_int, _vp],
     "fe_batch_loss": [_vp, _vp],
     "fe_batch_backward": [_vp, _vp, C.c_int, _vp],

@@ -11,9 +11,11 @@ Per class L (l_A + l_B) one persistent launch covers every conformer, pulling
 (conformer, pair range, point block) work items from a queue; the step --
 forward (+ field), loss gradient gV = 2 w (V - T), dL/dG and dL/dP -- is
 recorded once as a hipGraph and replayed.  New coordinates (same counts) and
-new loss data are taken by the recorded graph without re-recording.  The
-reference has no batched form: it runs one ``esp_gpu`` process per table
-(fastesp/cli.py:143-163).
+new loss data are taken by the recorded graph without re-recording.  With
+``set_density`` and ``rebuild=True`` the step first builds every table again
+on the device from the batch's densities and current points, so one graph
+runs build -> forward -> loss -> backward.  The reference has no batched
+form: it runs one ``esp_gpu`` process per table (fastesp/cli.py:143-163).
 """
 from __future__ import annotations
 
@@ -33,8 +35,9 @@ def _prec(p) -> int:
     raise ValueError(f"unknown precision {p!r}")
 
 
-def _flags(field: bool, dldp: bool) -> int:
-    return (_lib.FE_BATCH_FIELD if field else 0) | (_lib.FE_BATCH_DLDP if dldp else 0)
+def _flags(field: bool, dldp: bool, rebuild: bool = False) -> int:
+    return ((_lib.FE_BATCH_FIELD if field else 0) | (_lib.FE_BATCH_DLDP if dldp else 0)
+            | (_lib.FE_BATCH_REBUILD if rebuild else 0))
 
 
 class Batch:
@@ -100,6 +103,26 @@ class Batch:
             t.ctypes.data if t is not None else None, _lib.FE_PTR_HOST, stream),
             "fe_batch_set_loss")
 
+    def set_density(self, densities: list, tol: float = 1e-12, rule: int = 0,
+                    stream=None) -> None:
+        """Densities of every conformer (n_func x n_func each, in conformer
+        order) and the screen for steps with ``rebuild=True``.  The first call
+        rebuilds every table once (its capacity layout); afterwards the tables
+        are rebuilt by the steps that ask for it."""
+        nf = self.n_func
+        if len(densities) != len(nf):
+            raise ValueError("set_density: one density per conformer expected")
+        parts = []
+        for P, n in zip(densities, nf):
+            P = np.asarray(P, np.float64)
+            if P.shape != (n, n):
+                raise ValueError(f"set_density: density of shape {P.shape}, expected {(n, n)}")
+            parts.append(P.ravel())
+        flat = np.ascontiguousarray(np.concatenate(parts))
+        _lib.check(self._ctx.lib.fe_batch_set_density(self._h, flat.ctypes.data, float(tol),
+                                                      int(rule), _lib.FE_PTR_HOST, stream),
+                   "fe_batch_set_density")
+
     def set_tuning(self, key: int, value: int) -> None:
         _lib.check(self._ctx.lib.fe_batch_set_tuning(self._h, key, int(value)),
                    "fe_batch_set_tuning")
@@ -122,12 +145,16 @@ class Batch:
             self._h, g.ctypes.data if g is not None else None, _lib.FE_PTR_HOST, stream),
             "fe_batch_backward")
 
-    def step(self, precision="fp64", field: bool = True, dldp: bool = True, stream=None) -> None:
-        _lib.check(self._ctx.lib.fe_batch_step(self._h, _prec(precision), _flags(field, dldp),
-                                               stream), "fe_batch_step")
+    def step(self, precision="fp64", field: bool = True, dldp: bool = True, stream=None,
+             rebuild: bool = False) -> None:
+        _lib.check(self._ctx.lib.fe_batch_step(self._h, _prec(precision),
+                                               _flags(field, dldp, rebuild), stream),
+                   "fe_batch_step")
 
-    def capture(self, precision="fp64", field: bool = True, dldp: bool = True) -> None:
-        _lib.check(self._ctx.lib.fe_batch_capture(self._h, _prec(precision), _flags(field, dldp)),
+    def capture(self, precision="fp64", field: bool = True, dldp: bool = True,
+                rebuild: bool = False) -> None:
+        _lib.check(self._ctx.lib.fe_batch_capture(self._h, _prec(precision),
+                                                  _flags(field, dldp, rebuild)),
                    "fe_batch_capture")
 
     def launch(self, stream=None) -> None:

@@ -46,7 +46,8 @@ struct FeDgItem {
 
 struct DgSeg {  // one (conformer, class) of the dG reduction, non-empty only
   int64_t part, dg, e0;
-  int n_pairs, n_chunks;
+  int n_pairs, n_chunks;  // n_pairs: the planned count (the slot's row stride)
+  int conf, pad;
 };
 
 struct ConfDev {
@@ -86,7 +87,20 @@ struct fe_batch {
   int *perm = nullptr, *pt_conf = nullptr, *counters = nullptr;
   int64_t* d_pt_off = nullptr;
   const double** recs[FE_LMAX + 1] = {};
+  const int** kept[FE_LMAX + 1] = {};  // per class: each conformer's device kept count
   const float** g32s[FE_LMAX + 1] = {};
+  // Tables rebuilt inside the step (fe_batch_set_density): the densities of
+  // every conformer and a build workspace of the batch's own, so a recorded
+  // step depends on no context workspace.  Plans are then cut from the
+  // tables' capacities and the kernels read the kept counts on the device.
+  bool rb = false;
+  double rb_tol = 0.0;
+  int rb_rule = 0;
+  void* rbmem = nullptr;
+  double* dens = nullptr;
+  char* bws = nullptr;
+  size_t bws_bytes = 0;
+  std::vector<int64_t> dens_off;  // n_conf + 1
   // forward plan
   bool fplan = false, f_grad = false, f_mixed = false;
   void* fmem = nullptr;
@@ -232,6 +246,7 @@ template <int L>
 __global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG_batch(const FeDgItem* __restrict__ items,
                                                   const int n_items, int* __restrict__ counter,
                                                   const double* const* __restrict__ recs,
+                                                  const int* const* __restrict__ kept,
                                                   const double* __restrict__ pw,
                                                   const double* __restrict__ boys_tab,
                                                   double* __restrict__ part) {
@@ -246,7 +261,11 @@ __global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG_batch(const FeDgItem*
     const int it = __builtin_amdgcn_readfirstlane(s_it);
     if (it >= n_items) break;
     const FeDgItem d = items[it];
-    fe_dg_block<L>(recs[d.conf], d.n_pairs, d.pb * 256 + (int)threadIdx.x, pw, d.q0, d.q1, s_tab,
+    // pairs past the conformer's kept count (plans cut from capacities) are
+    // skipped; the test is block-uniform
+    const int np = min(*kept[d.conf], d.n_pairs);
+    if (d.pb * 256 >= np) continue;
+    fe_dg_block<L>(recs[d.conf], np, d.pb * 256 + (int)threadIdx.x, pw, d.q0, d.q1, s_tab,
                    part + d.out, d.n_pairs);
   }
 }
@@ -255,6 +274,7 @@ __global__ __launch_bounds__(256, fe_dg_minw(L)) void k_dG_batch(const FeDgItem*
 template <int L>
 __global__ __launch_bounds__(256) void k_dG_reduce_batch(const DgSeg* __restrict__ segs,
                                                          const int n_segs, const int64_t n_elem,
+                                                         const int* const* __restrict__ kept,
                                                          const double* __restrict__ part,
                                                          double* __restrict__ dG) {
   constexpr int NT = FeL<L>::NT;
@@ -269,6 +289,7 @@ __global__ __launch_bounds__(256) void k_dG_reduce_batch(const DgSeg* __restrict
   const DgSeg sg = segs[lo];
   const int64_t el = e - sg.e0;
   const int k = (int)(el / sg.n_pairs), pair = (int)(el - (int64_t)k * sg.n_pairs);
+  if (pair >= *kept[sg.conf]) return;  // no record there (dL/dP reads kept pairs only)
   double s = 0.0;
   for (int c = 0; c < sg.n_chunks; ++c)
     s += part[sg.part + ((int64_t)c * NT + k) * sg.n_pairs + pair];
@@ -351,6 +372,30 @@ static int sort_points(fe_batch* b, hipStream_t st) {
   return FE_OK;
 }
 
+// per class, each conformer's record pointer and kept-count pointer (the
+// first rebuild of a table moves both)
+static int upload_table_ptrs(fe_batch* b) {
+  std::vector<const void*> rp(b->n_conf), kp(b->n_conf);
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    for (int i = 0; i < b->n_conf; ++i) {
+      rp[i] = b->tabs[i]->d_rec[L];
+      kp[i] = b->tabs[i]->d_stats + FE_ST_KEPT + L;
+    }
+    if (hipMemcpy(b->recs[L], rp.data(), b->n_conf * sizeof(void*), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(b->kept[L], kp.data(), b->n_conf * sizeof(void*), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return fe_set_error(FE_ERR_HIP, "batch: upload of table pointers failed");
+  }
+  return FE_OK;
+}
+
+// pairs of class L a plan is cut for: the kept count, or the capacity when
+// the batch rebuilds its tables
+static int64_t plan_pairs(const fe_batch* b, const fe_table* t, int L) {
+  return b->rb ? t->cap_L[L] : t->n_L[L];
+}
+
 extern "C" int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables,
                                const double* pts, const int64_t* n_pts, int ptr_kind,
                                void* stream, fe_batch** out) {
@@ -401,10 +446,11 @@ extern "C" int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables,
   const size_t o_loss = cv.take<double>(n_conf), o_pw = cv.take<double>(4 * N);
   const size_t o_perm = cv.take<int>(N), o_pc = cv.take<int>(N), o_cnt = cv.take<int>(CNT_N);
   const size_t o_off = cv.take<int64_t>(n_conf + 1);
-  size_t o_rec[FE_LMAX + 1], o_g32[FE_LMAX + 1];
+  size_t o_rec[FE_LMAX + 1], o_g32[FE_LMAX + 1], o_kept[FE_LMAX + 1];
   for (int L = 0; L <= FE_LMAX; ++L) {
     o_rec[L] = cv.take<const double*>(n_conf);
     o_g32[L] = cv.take<const float*>(n_conf);
+    o_kept[L] = cv.take<const int*>(n_conf);
   }
   if (int rc = batch_alloc(&b->mem, cv.off, "points and outputs")) return fail(rc);
   char* m = (char*)b->mem;
@@ -421,15 +467,12 @@ extern "C" int fe_batch_create(fe_ctx* ctx, int n_conf, fe_table* const* tables,
   b->pt_conf = (int*)(m + o_pc);
   b->counters = (int*)(m + o_cnt);
   b->d_pt_off = (int64_t*)(m + o_off);
-  std::vector<const double*> rp(n_conf);
   for (int L = 0; L <= FE_LMAX; ++L) {
     b->recs[L] = (const double**)(m + o_rec[L]);
     b->g32s[L] = (const float**)(m + o_g32[L]);
-    for (int i = 0; i < n_conf; ++i) rp[i] = tables[i]->d_rec[L];
-    if (hipMemcpy(b->recs[L], rp.data(), n_conf * sizeof(void*), hipMemcpyHostToDevice) !=
-        hipSuccess)
-      return fail(fe_set_error(FE_ERR_HIP, "fe_batch_create: upload of record pointers failed"));
+    b->kept[L] = (const int**)(m + o_kept[L]);
   }
+  if (int rc = upload_table_ptrs(b)) return fail(rc);
   if (hipMemcpy(b->d_pt_off, b->pt_off.data(), (n_conf + 1) * sizeof(int64_t),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail(fe_set_error(FE_ERR_HIP, "fe_batch_create: upload of point offsets failed"));
@@ -458,6 +501,7 @@ extern "C" int fe_batch_free(fe_batch* b) {
   (void)hipSetDevice(b->ctx->device);
   drop_fplan(b);
   drop_bplan(b);
+  if (b->rbmem) (void)hipFree(b->rbmem);
   if (b->mem) (void)hipFree(b->mem);
   if (b->cs) (void)hipStreamDestroy(b->cs);
   delete b;
@@ -488,7 +532,7 @@ static int plan_fwd(fe_batch* b, bool grad, bool mixed, hipStream_t st) {
     const int blk = FE_WG * b->ppl[L];
     for (int i = 0; i < nc; ++i) {
       const int64_t n = b->pt_off[i + 1] - b->pt_off[i];
-      total[L] += (double)((n + blk - 1) / blk) * blk * (double)b->tabs[i]->n_L[L];
+      total[L] += (double)((n + blk - 1) / blk) * blk * (double)plan_pairs(b, b->tabs[i], L);
     }
   }
   std::vector<ConfDev> conf(nc);
@@ -500,7 +544,7 @@ static int plan_fwd(fe_batch* b, bool grad, bool mixed, hipStream_t st) {
     const int ld = (n + 63) / 64 * 64;
     int rows = 0;
     for (int L = 0; L <= FE_LMAX; ++L) {
-      const int64_t nL = t->n_L[L];
+      const int64_t nL = plan_pairs(b, t, L);
       if (nL == 0 || n == 0) continue;
       const int blk = FE_WG * b->ppl[L];
       const double cap = std::max(1.0, total[L] / ((double)resident[L] * b->k_eval));
@@ -538,7 +582,7 @@ static int plan_fwd(fe_batch* b, bool grad, bool mixed, hipStream_t st) {
     for (const FeBatchItem& it : items[L]) {
       const int blk = FE_WG * b->ppl[L];
       const int span = std::min(blk, it.end - it.base);
-      if (it.p0 < 0 || it.p1 > b->tabs[it.conf]->n_L[L] || it.p0 >= it.p1 || span <= 0 ||
+      if (it.p0 < 0 || it.p1 > plan_pairs(b, b->tabs[it.conf], L) || it.p0 >= it.p1 || span <= 0 ||
           it.end > b->n_tot || it.out < 0 || (size_t)(it.out + span) > plane)
         return fe_set_error(FE_ERR_STATE, "batch: internal plan error (class %d)", L);
     }
@@ -598,7 +642,7 @@ static int plan_bwd(fe_batch* b) {
   for (int L = 0; L <= FE_LMAX; ++L) {
     if (occ[L] < 1) return fe_set_error(FE_ERR_HIP, "batch: dG kernel %d cannot be resident", L);
     for (int i = 0; i < nc; ++i)
-      total[L] += (double)((b->tabs[i]->n_L[L] + 255) / 256) * 256.0 *
+      total[L] += (double)((plan_pairs(b, b->tabs[i], L) + 255) / 256) * 256.0 *
                   (double)(b->pt_off[i + 1] - b->pt_off[i]);
   }
   std::vector<FeDgItem> items[FE_LMAX + 1];
@@ -615,7 +659,7 @@ static int plan_bwd(fe_batch* b) {
     const int pt0 = (int)b->pt_off[i], n = (int)(b->pt_off[i + 1] - b->pt_off[i]);
     dgbase[i] = dg_total;
     for (int L = 0; L <= FE_LMAX; ++L) {
-      const int64_t nL = t->n_L[L];
+      const int64_t nL = plan_pairs(b, t, L);
       const int NT = fe_host::NTERM[L];
       dg_off[i][L] = dg_total - dgbase[i];
       if (nL == 0) continue;
@@ -635,6 +679,7 @@ static int plan_bwd(fe_batch* b) {
       sg.e0 = e_run[L];
       sg.n_pairs = (int)nL;
       sg.n_chunks = (int)ch;
+      sg.conf = i;
       segs[L].push_back(sg);
       const int nb = (int)((nL + 255) / 256);
       // point-chunk-major: workgroups that run together read the same points
@@ -722,6 +767,7 @@ static void launch_forward(fe_batch* b, hipStream_t st) {
     a.n_items = b->n_items[L];
     a.counter = b->counters + CNT_EVAL + L;
     a.recs = b->recs[L];
+    a.kept = b->kept[L];
     a.g32s = b->g32s[L];
     a.pts = b->pts;
     a.boys = c->d_boys;
@@ -757,15 +803,15 @@ static void launch_dG_L(fe_batch* b, hipStream_t st) {
   if (b->n_dgitems[L] && (b->mask & (1 << L))) {
     const int pg = prof_begin(b, FE_PH_DG0 + L, st);
     hipLaunchKernelGGL(k_dG_batch<L>, dim3(b->dggrid[L]), dim3(256), 0, st, b->dgitems[L],
-                       b->n_dgitems[L], b->counters + CNT_DG + L, b->recs[L], b->pw,
+                       b->n_dgitems[L], b->counters + CNT_DG + L, b->recs[L], b->kept[L], b->pw,
                        b->ctx->d_boys + (size_t)L * FE_BOYS_NR * FE_BOYS_COLS, b->dg_part);
     fe_prof_end(b->ctx, pg, st);
   }
   if (b->n_elem[L]) {
     const int pr = prof_begin(b, FE_PH_DGRED0 + L, st);
     hipLaunchKernelGGL(k_dG_reduce_batch<L>, dim3((unsigned)((b->n_elem[L] + 255) / 256)),
-                       dim3(256), 0, st, b->segs[L], b->n_segs[L], b->n_elem[L], b->dg_part,
-                       b->dG);
+                       dim3(256), 0, st, b->segs[L], b->n_segs[L], b->n_elem[L], b->kept[L],
+                       b->dg_part, b->dG);
     fe_prof_end(b->ctx, pr, st);
   }
 }
@@ -841,16 +887,35 @@ extern "C" int fe_batch_backward(fe_batch* b, const double* gV, int ptr_kind, vo
 
 static int prepare_step(fe_batch* b, int precision, int flags, hipStream_t st) {
   if (int rc = check_prec(precision, "fe_batch_step")) return rc;
+  if ((flags & FE_BATCH_REBUILD) && !b->rb)
+    return fe_set_error(FE_ERR_STATE,
+                        "fe_batch_step: FE_BATCH_REBUILD needs the densities (fe_batch_set_density)");
   if (int rc = plan_fwd(b, flags & FE_BATCH_FIELD, precision == FE_PREC_MIXED, st)) return rc;
   if (flags & FE_BATCH_DLDP)
     if (int rc = plan_bwd(b)) return rc;
   return FE_OK;
 }
 
-static void launch_step(fe_batch* b, int flags, hipStream_t st) {
+// every table built again from the batch's densities and current points (the
+// screen reads the sorted copy; it does not depend on the order)
+static int launch_rebuilds(fe_batch* b, hipStream_t st) {
+  for (int i = 0; i < b->n_conf; ++i) {
+    const int64_t pt0 = b->pt_off[i], n = b->pt_off[i + 1] - pt0;
+    if (int rc = fe_table_rebuild_ws(b->tabs[i], b->dens + b->dens_off[i], (int)n,
+                                     b->pts + 3 * pt0, b->rb_tol, b->rb_rule, st, b->bws,
+                                     b->bws_bytes))
+      return rc;
+  }
+  return FE_OK;
+}
+
+static int launch_step(fe_batch* b, int flags, hipStream_t st) {
+  if (flags & FE_BATCH_REBUILD)
+    if (int rc = launch_rebuilds(b, st)) return rc;
   launch_forward(b, st);
   launch_loss(b, st);
   if (flags & FE_BATCH_DLDP) launch_backward(b, st);
+  return FE_OK;
 }
 
 extern "C" int fe_batch_step(fe_batch* b, int precision, int flags, void* stream) {
@@ -859,7 +924,7 @@ extern "C" int fe_batch_step(fe_batch* b, int precision, int flags, void* stream
   FE_HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = (hipStream_t)stream;
   if (int rc = prepare_step(b, precision, flags, st)) return rc;
-  launch_step(b, flags, st);
+  if (int rc = launch_step(b, flags, st)) return rc;
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }
@@ -873,13 +938,18 @@ extern "C" int fe_batch_capture(fe_batch* b, int precision, int flags) {
   drop_graph(b);
   FE_HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
   b->capturing = true;
-  launch_step(b, flags, b->cs);
+  // no phase events in the graph (the table rebuilds would record them)
+  const bool prof = b->ctx->prof_on;
+  b->ctx->prof_on = false;
+  const int rc_step = launch_step(b, flags, b->cs);
+  b->ctx->prof_on = prof;
   b->capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->cs, &g);
-  if (e != hipSuccess || !g) {
+  if (rc_step != FE_OK || e != hipSuccess || !g) {
     (void)hipGetLastError();
     if (g) (void)hipGraphDestroy(g);
+    if (rc_step != FE_OK) return rc_step;
     return fe_set_error(FE_ERR_HIP, "fe_batch_capture: capture failed: %s", hipGetErrorString(e));
   }
   b->graph = g;
@@ -932,6 +1002,64 @@ extern "C" int fe_batch_set_loss(fe_batch* b, const double* w, const double* tar
   const size_t bytes = (size_t)b->n_tot * sizeof(double);
   if (w && bytes) FE_HIPCHK(hipMemcpyAsync(b->w, w, bytes, k, st));
   if (target && bytes) FE_HIPCHK(hipMemcpyAsync(b->target, target, bytes, k, st));
+  if (ptr_kind == FE_PTR_HOST) FE_HIPCHK(hipStreamSynchronize(st));
+  return FE_OK;
+}
+
+extern "C" int fe_batch_set_density(fe_batch* b, const double* dens, double tol, int rule,
+                                    int ptr_kind, void* stream) {
+  if (!b || !dens) return fe_set_error(FE_ERR_ARG, "fe_batch_set_density: null argument");
+  if (ptr_kind != FE_PTR_HOST && ptr_kind != FE_PTR_DEVICE)
+    return fe_set_error(FE_ERR_ARG, "fe_batch_set_density: bad ptr_kind");
+  if (!(tol >= 0.0) || !std::isfinite(tol))
+    return fe_set_error(FE_ERR_ARG, "fe_batch_set_density: tol must be finite and >= 0");
+  if (rule < 0 || rule > 2) return fe_set_error(FE_ERR_ARG, "fe_batch_set_density: bad rule %d", rule);
+  if (!b->all_basis)
+    return fe_set_error(FE_ERR_STATE,
+                        "fe_batch_set_density: every table must be built from a basis (fe_table_build)");
+  FE_CTX_LOCK(b->ctx);
+  FE_HIPCHK(hipSetDevice(b->ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = b->n_conf;
+  if (!b->rbmem) {
+    b->dens_off.assign(nc + 1, 0);
+    size_t ws = 0;
+    for (int i = 0; i < nc; ++i) {
+      const fe_table* t = b->tabs[i];
+      b->dens_off[i + 1] = b->dens_off[i] + (int64_t)t->n_func * t->n_func;
+      ws = std::max(ws, fe_build_ws_bytes(t, (int)(b->pt_off[i + 1] - b->pt_off[i])));
+    }
+    const size_t db = up256((size_t)b->dens_off[nc] * sizeof(double));
+    if (int rc = batch_alloc(&b->rbmem, db + ws, "densities and build workspace")) return rc;
+    b->dens = (double*)b->rbmem;
+    b->bws = (char*)b->rbmem + db;
+    b->bws_bytes = ws;
+  }
+  if (b->dens_off[nc])
+    FE_HIPCHK(hipMemcpyAsync(b->dens, dens, (size_t)b->dens_off[nc] * sizeof(double),
+                             ptr_kind == FE_PTR_HOST ? hipMemcpyHostToDevice
+                                                     : hipMemcpyDeviceToDevice,
+                             st));
+  if (!b->rb) {
+    // first time: every table's first rebuild (its capacity layout; this
+    // allocates and synchronises), then the plans are cut from capacities
+    for (int i = 0; i < nc; ++i) {
+      fe_table* t = b->tabs[i];
+      const int64_t pt0 = b->pt_off[i], n = b->pt_off[i + 1] - pt0;
+      if (int rc = fe_table_rebuild(t, b->dens + b->dens_off[i], (int)n, b->pts + 3 * pt0, tol,
+                                    rule, FE_PTR_DEVICE, st))
+        return rc;
+    }
+    FE_HIPCHK(hipStreamSynchronize(st));
+    if (int rc = upload_table_ptrs(b)) return rc;
+    drop_fplan(b);
+    drop_bplan(b);
+    b->rb = true;
+  } else if (tol != b->rb_tol || rule != b->rb_rule) {
+    drop_graph(b);  // the recorded rebuilds carry tol and rule
+  }
+  b->rb_tol = tol;
+  b->rb_rule = rule;
   if (ptr_kind == FE_PTR_HOST) FE_HIPCHK(hipStreamSynchronize(st));
   return FE_OK;
 }

@@ -589,8 +589,68 @@ int check_build_args(int n_points, const double* pts, const double* dens, double
 // it.  fe_table_rebuild (`sized` true): the table already holds a record for
 // every K-valid candidate, and nothing is synchronised or allocated beyond the
 // context's build workspace (which only grows for a larger point count).
+// Byte layout of a build's workspace (candidates, keys, flags, thresholds,
+// boxes, sort and scan scratch); host inputs also stage the density and points.
+struct BuildLayout {
+  size_t o_dens = 0, o_pts = 0, o_cand[FE_LMAX + 1] = {}, o_gidx = 0, o_key = 0, o_val = 0,
+         o_gflag = 0, o_skey = 0, o_sval = 0, o_thr = 0, o_srule = 0, o_stats = 0, o_spart = 0,
+         o_box = 0, o_psort = 0, sort_pts = 0, o_sort = 0, total = 0;
+};
+
+static BuildLayout build_layout(const fe_table* t, int n_points, bool host_in) {
+  BuildLayout y;
+  int64_t n_all = 0;
+  for (int L = 0; L <= FE_LMAX; ++L) n_all += t->n_cand_L[L];
+  const int n_box = (n_points + FE_BOX - 1) / FE_BOX;
+  size_t wo = 0;
+  y.o_dens = wo;
+  if (host_in) wo += al256((size_t)t->n_func * t->n_func * sizeof(double));
+  y.o_pts = wo;
+  if (host_in) wo += al256((size_t)n_points * 3 * sizeof(double));
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    y.o_cand[L] = wo;
+    wo += al256((size_t)t->n_cand_L[L] * fe_host::STRIDE[L] * sizeof(double));
+  }
+  const size_t na = (size_t)std::max<int64_t>(n_all, 1);
+  y.o_gidx = wo;
+  wo += al256(na * sizeof(int));
+  y.o_key = wo;
+  wo += al256(na * sizeof(uint32_t));
+  y.o_val = wo;
+  wo += al256(na * sizeof(int32_t));
+  y.o_gflag = wo;
+  wo += al256(na * sizeof(unsigned long long));
+  y.o_skey = wo;
+  wo += al256(na * sizeof(uint32_t));
+  y.o_sval = wo;
+  wo += al256(na * sizeof(int32_t));
+  y.o_thr = wo;
+  wo += al256(na * sizeof(double));
+  y.o_srule = wo;
+  wo += al256((size_t)std::max(t->n_sp, 1) * sizeof(double));
+  y.o_stats = wo;  // FE_ST_N counters, the open count, the sum rule
+  wo += al256(FE_ST_SUMRULE_BYTE + sizeof(double));
+  y.o_spart = wo;
+  wo += al256(FE_SRULE_BLOCKS * sizeof(double));
+  y.o_box = wo;
+  wo += al256((size_t)std::max(n_box, 1) * 6 * sizeof(double));
+  // the point sort (its sorted copy is read by the screen), then the
+  // candidate sorts and the scan
+  y.o_psort = wo;
+  const size_t b_key = al256((size_t)n_points * 4), b_pts = al256((size_t)n_points * 24);
+  y.sort_pts = 2 * b_key + b_pts + fe_radix_sort_scratch(n_points);
+  wo += al256(y.sort_pts);
+  y.o_sort = wo;
+  wo += al256(std::max(fe_radix_sort_scratch(n_all), fe_scan_scratch(n_all)));
+  y.total = wo;
+  return y;
+}
+
+// The build proper.  `ws` / `ws_bytes`: a caller-owned workspace of at least
+// fe_build_ws_bytes bytes (device inputs only), else the context's.
 int build_on_device(fe_table* t, const double* dens, int n_points, const double* pts, double tol,
-                    int rule, int ptr_kind, hipStream_t st, bool sized) {
+                    int rule, int ptr_kind, hipStream_t st, bool sized, char* ws = nullptr,
+                    size_t ws_bytes = 0) {
   fe_ctx* c = t->ctx;
   const int nsp = t->n_sp;
   ClassInfo ci{};
@@ -603,49 +663,21 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   const int64_t n_all = ci.cand_off[FE_LMAX + 1];
   const bool host_in = ptr_kind == FE_PTR_HOST;
   const int n_box = (n_points + FE_BOX - 1) / FE_BOX;
-  // ---- workspace layout
-  size_t wo = 0;
-  const size_t o_dens = wo;
-  if (host_in) wo += al256((size_t)t->n_func * t->n_func * sizeof(double));
-  const size_t o_pts = wo;
-  if (host_in) wo += al256((size_t)n_points * 3 * sizeof(double));
-  size_t o_cand[FE_LMAX + 1];
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    o_cand[L] = wo;
-    wo += al256((size_t)t->n_cand_L[L] * fe_host::STRIDE[L] * sizeof(double));
+  const BuildLayout y = build_layout(t, n_points, host_in);
+  const size_t o_dens = y.o_dens, o_pts = y.o_pts, o_gidx = y.o_gidx, o_key = y.o_key,
+               o_val = y.o_val, o_gflag = y.o_gflag, o_skey = y.o_skey, o_sval = y.o_sval,
+               o_thr = y.o_thr, o_srule = y.o_srule, o_stats = y.o_stats, o_spart = y.o_spart,
+               o_box = y.o_box, o_psort = y.o_psort, sort_pts = y.sort_pts, o_sort = y.o_sort;
+  const size_t* o_cand = y.o_cand;
+  char* wb = ws;
+  if (wb) {
+    if (host_in || ws_bytes < y.total)
+      return fe_set_error(FE_ERR_STATE, "build: caller workspace of %zu bytes, %zu needed",
+                          ws_bytes, y.total);
+  } else {
+    if (int rc = fe_ctx_reserve(c, FE_WS_BUILD, y.total)) return rc;
+    wb = (char*)c->ws[FE_WS_BUILD];
   }
-  const size_t o_gidx = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(int));
-  const size_t o_key = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(uint32_t));
-  const size_t o_val = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(int32_t));
-  const size_t o_gflag = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(unsigned long long));
-  const size_t o_skey = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(uint32_t));
-  const size_t o_sval = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(int32_t));
-  const size_t o_thr = wo;
-  wo += al256((size_t)std::max<int64_t>(n_all, 1) * sizeof(double));
-  const size_t o_srule = wo;
-  wo += al256((size_t)std::max(nsp, 1) * sizeof(double));
-  const size_t o_stats = wo;  // FE_ST_N counters, the open count, the sum rule
-  wo += al256(FE_ST_SUMRULE_BYTE + sizeof(double));
-  const size_t o_spart = wo;
-  wo += al256(FE_SRULE_BLOCKS * sizeof(double));
-  const size_t o_box = wo;
-  wo += al256((size_t)std::max(n_box, 1) * 6 * sizeof(double));
-  // the point sort (its sorted copy is read by the screen), then the
-  // candidate sorts and the scan
-  const size_t o_psort = wo;
-  const size_t b_key = al256((size_t)n_points * 4), b_pts = al256((size_t)n_points * 24);
-  const size_t sort_pts = 2 * b_key + b_pts + fe_radix_sort_scratch(n_points);
-  wo += al256(sort_pts);
-  const size_t o_sort = wo;
-  wo += al256(std::max(fe_radix_sort_scratch(n_all), fe_scan_scratch(n_all)));
-  if (int rc = fe_ctx_reserve(c, FE_WS_BUILD, wo)) return rc;
-  char* wb = (char*)c->ws[FE_WS_BUILD];
   int* ws_stats = (int*)(wb + o_stats);
 
   // ---- density and points
@@ -782,6 +814,10 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   return FE_OK;
 }
 }  // namespace
+
+size_t fe_build_ws_bytes(const fe_table* t, int n_points) {
+  return build_layout(t, n_points, false).total;
+}
 
 namespace {
 // A basis set up for the device: the shells, the shell pairs (si <= sj in
@@ -1048,6 +1084,25 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   if (int rc = build_on_device(t, dens, n_points, pts, tol, rule, ptr_kind, st, false))
     return fail(rc);
   *out = t;
+  return FE_OK;
+}
+
+// The rebuild of a table already in its capacity layout (a first
+// fe_table_rebuild has run), on device inputs, with a caller-owned workspace:
+// nothing here touches the context's workspace, so a graph that records it
+// stays valid whatever the context does later.
+int fe_table_rebuild_ws(fe_table* t, const double* dens, int n_points, const double* pts,
+                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes) {
+  if (!t || !t->has_basis || !t->capacity_layout)
+    return fe_set_error(FE_ERR_STATE, "rebuild: the table has no capacity layout yet");
+  if (int rc = check_build_args(n_points, pts, dens, tol, rule, FE_PTR_DEVICE, "rebuild"))
+    return rc;
+  if (int rc = build_on_device(t, dens, n_points, pts, tol, rule, FE_PTR_DEVICE, st, true, ws,
+                               ws_bytes))
+    return rc;
+  if (int rc = fe_refresh_g32(t, st)) return rc;
+  t->counts_stale = true;
+  t->count_stream = st;
   return FE_OK;
 }
 

@@ -40,6 +40,7 @@ template <int L, int PPL, bool GRAD, bool MIXED>
 __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_batch_eval(const FeBatchItem* __restrict__ items,
                                                       const int n_items, int* __restrict__ counter,
                                                       const double* const* __restrict__ recs,
+                                                      const int* const* __restrict__ kept,
                                                       const float* const* __restrict__ g32s,
                                                       const double* __restrict__ pts,
                                                       const double* __restrict__ boys_tab,
@@ -56,8 +57,13 @@ __global__ __launch_bounds__(FE_WG, fe_eval_minw(L, PPL)) void k_batch_eval(cons
     const int it = __builtin_amdgcn_readfirstlane(s_it);
     if (it >= n_items) break;
     const FeBatchItem b = items[it];
+    // items are cut from the tables' capacities when the batch rebuilds its
+    // tables; the pairs past a table's kept count are not evaluated (an item
+    // with none left still writes its zero row slot)
+    const int p1 = min(b.p1, *kept[b.conf]);
+    const int p0 = min(b.p0, p1);
     fe_eval_block<L, PPL, GRAD, MIXED>((fe_gdptr)recs[b.conf],
-                                       (fe_gfptr)(MIXED ? g32s[b.conf] : nullptr), b.p0, b.p1,
+                                       (fe_gfptr)(MIXED ? g32s[b.conf] : nullptr), p0, p1,
                                        pts, b.base, b.end, s_tab, tau, partial + b.out, plane,
                                        false);
   }
@@ -149,8 +155,8 @@ static int batch_LPGX(const FeBatchEvalArgs* a, hipStream_t st) {
   const int M = G ? L + 1 : L;
   const double* tab = a->boys + (size_t)M * FE_BOYS_NR * FE_BOYS_COLS;
   hipLaunchKernelGGL((k_batch_eval<L, PPL, G, X>), dim3(a->grid), dim3(FE_WG), 0, st, a->items,
-                     a->n_items, a->counter, a->recs, a->g32s, a->pts, tab, a->partial, a->plane,
-                     a->tau);
+                     a->n_items, a->counter, a->recs, a->kept, a->g32s, a->pts, tab, a->partial,
+                     a->plane, a->tau);
   return 0;
 }
 

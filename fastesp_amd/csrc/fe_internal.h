@@ -245,6 +245,12 @@ int fe_table_layout(fe_table* t, const int64_t* cap, int64_t n_slots);
 int fe_table_sync_counts(fe_table* t);
 // the FP32 coefficients again after the records changed (no-op without them)
 int fe_refresh_g32(fe_table* t, hipStream_t st);
+// workspace bytes of a build of table t on n_points device points, and the
+// rebuild on device inputs into a caller-owned workspace (the table must be in
+// its capacity layout: a first fe_table_rebuild has run)
+size_t fe_build_ws_bytes(const fe_table* t, int n_points);
+int fe_table_rebuild_ws(fe_table* t, const double* dens, int n_points, const double* pts,
+                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes);
 
 
 // Partial sums: pair chunk c of every class adds into row c.  The classes run
@@ -323,6 +329,7 @@ struct FeBatchEvalArgs {
   int n_items;
   int* counter;                 // work-queue head, zero before the launch
   const double* const* recs;    // per conformer: class-L records
+  const int* const* kept;       // per conformer: its class-L kept count (device)
   const float* const* g32s;     // per conformer: FP32 coefficients (mixed)
   const double* pts;            // sorted points of all conformers
   const double* boys;           // Boys grids of the context (all orders)

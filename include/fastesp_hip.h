@@ -270,6 +270,9 @@ typedef struct fe_batch fe_batch;
 
 #define FE_BATCH_FIELD 1   /* forward also computes dV/dC                 */
 #define FE_BATCH_DLDP 2    /* step runs loss + backward to dL/dP          */
+#define FE_BATCH_REBUILD 4 /* step first builds every table again from the
+                              densities of fe_batch_set_density and the
+                              batch's current points                      */
 
 #define FE_BOUT_V 0        /* V, n_tot                                    */
 #define FE_BOUT_FIELD 1    /* dV/dC, n_tot x 3                            */
@@ -291,6 +294,19 @@ int fe_batch_set_points(fe_batch* b, const double* pts, int ptr_kind, void* stre
 /* loss weights w and targets T (n_tot each; NULL keeps; default w = 1, T = 0) */
 int fe_batch_set_loss(fe_batch* b, const double* w, const double* target, int ptr_kind,
                       void* stream);
+/* The densities of every conformer (the n_func_c x n_func_c row-major blocks,
+ * concatenated in conformer order) and the screen (tol, rule as in
+ * fe_table_build) for steps with FE_BATCH_REBUILD, which build every table
+ * again on the device before the forward, so a recorded step runs build ->
+ * forward -> loss -> backward.  No counterpart in the reference (each of its
+ * runs builds one table, build_pairs.cpp:162-425).  The first call rebuilds
+ * every table once (fe_table_rebuild: their capacity layout, which allocates
+ * and synchronises) and sizes a build workspace owned by the batch; later
+ * calls only copy the densities.  Every table must come from fe_table_build;
+ * from the first call on the tables are the batch's to rebuild (host queries
+ * of a table read its latest counts). */
+int fe_batch_set_density(fe_batch* b, const double* dens, double tol, int rule, int ptr_kind,
+                         void* stream);
 int fe_batch_set_tuning(fe_batch* b, int key, int64_t value);
 /* V (and dV/dC with FE_BATCH_FIELD) of every conformer */
 int fe_batch_forward(fe_batch* b, int precision, int flags, void* stream);
