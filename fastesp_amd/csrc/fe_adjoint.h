@@ -19,6 +19,8 @@ struct DPArgs {
   const double* sh_c; const double* pexp; const double* pcoef;
   const int* fn_lx; const int* fn_ly; const int* fn_lz;
   const int* sp_i; const int* sp_j; const int* sp_slot;
+  const int* sp_by_class;  // shell pairs grouped by class: [sp_cls_off[L], sp_cls_off[L+1])
+  int sp_cls_off[FE_LMAX + 2];
   const int* slot2kept[FE_LMAX + 1];
   const double* dG[FE_LMAX + 1];
   int n_func;
@@ -223,6 +225,8 @@ inline DPArgs fe_dp_args(const fe_table* t, double* dG, const size_t* dg_off, do
   pa.sh_nprim = t->b.nprim; pa.sh_c = t->b.c; pa.pexp = t->b.pexp; pa.pcoef = t->b.pcoef;
   pa.fn_lx = t->b.fn_lx; pa.fn_ly = t->b.fn_ly; pa.fn_lz = t->b.fn_lz;
   pa.sp_i = t->b.sp_i; pa.sp_j = t->b.sp_j; pa.sp_slot = t->b.sp_slot;
+  pa.sp_by_class = t->b.sp_by_class;
+  for (int L = 0; L <= FE_LMAX + 1; ++L) pa.sp_cls_off[L] = t->sp_cls_off[L];
   for (int L = 0; L <= FE_LMAX; ++L) {
     pa.slot2kept[L] = t->d_slot2kept[L];
     pa.dG[L] = dG + dg_off[L];

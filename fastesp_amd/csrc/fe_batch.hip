@@ -120,8 +120,8 @@ struct fe_batch {
   int64_t n_elem[FE_LMAX + 1] = {};
   double *dg_part = nullptr, *dG = nullptr, *dP = nullptr;
   DPArgs* dpargs = nullptr;
-  int64_t* sp_off = nullptr;
-  int64_t n_sp_tot = 0;
+  int64_t* sp_off = nullptr;  // per class L: (n_conf + 1) offsets of its shell pairs
+  int64_t n_sp[FE_LMAX + 1] = {}, n_sp_tot = 0;
   std::vector<int64_t> dp_off;  // n_conf + 1
   // tuning
   int ppl[FE_LMAX + 1] = {2, 2, 2, 2, 2, 1, 1};
@@ -296,9 +296,11 @@ __global__ __launch_bounds__(256) void k_dG_reduce_batch(const DgSeg* __restrict
   dG[sg.dg + (int64_t)pair * NT + k] = s;
 }
 
+// the class-L shell pairs of every conformer (their dL/dG is reduced)
 __global__ __launch_bounds__(64) void k_dg_to_dp_batch(const DPArgs* __restrict__ args,
                                                        const int64_t* __restrict__ sp_off,
-                                                       const int n_conf, const int64_t n_sp) {
+                                                       const int n_conf, const int64_t n_sp,
+                                                       const int L) {
   for (int64_t s = blockIdx.x; s < n_sp; s += gridDim.x) {
     int lo = 0, hi = n_conf - 1;  // last conformer with sp_off <= s
     while (lo < hi) {
@@ -306,7 +308,8 @@ __global__ __launch_bounds__(64) void k_dg_to_dp_batch(const DPArgs* __restrict_
       if (sp_off[mid] <= s) lo = mid;
       else hi = mid - 1;
     }
-    fe_dg_to_dp_sp(args[lo], (int)(s - sp_off[lo]));
+    const DPArgs& a = args[lo];
+    fe_dg_to_dp_sp(a, a.sp_by_class[a.sp_cls_off[L] + (int)(s - sp_off[lo])]);
     __syncthreads();
   }
 }
@@ -650,7 +653,7 @@ static int plan_bwd(fe_batch* b) {
   int64_t e_run[FE_LMAX + 1] = {};
   size_t part_total = 0, dg_total = 0;
   std::vector<DPArgs> args(nc);
-  std::vector<int64_t> sp_off(nc + 1, 0);
+  std::vector<int64_t> sp_off((size_t)(FE_LMAX + 1) * (nc + 1), 0);
   std::vector<size_t> dgbase(nc);
   std::vector<std::array<size_t, FE_LMAX + 1>> dg_off(nc);
   b->dp_off.assign(nc + 1, 0);
@@ -698,7 +701,9 @@ static int plan_bwd(fe_batch* b) {
       dg_total += (size_t)nL * NT;
       e_run[L] += nL * NT;
     }
-    sp_off[i + 1] = sp_off[i] + t->n_sp;
+    for (int L = 0; L <= FE_LMAX; ++L)
+      sp_off[(size_t)L * (nc + 1) + i + 1] =
+          sp_off[(size_t)L * (nc + 1) + i] + (t->sp_cls_off[L + 1] - t->sp_cls_off[L]);
     b->dp_off[i + 1] = b->dp_off[i] + (int64_t)t->n_func * t->n_func;
   }
   for (int L = 0; L <= FE_LMAX; ++L) {
@@ -722,7 +727,7 @@ static int plan_bwd(fe_batch* b) {
   }
   const size_t o_part = cv.take<double>(part_total), o_dG = cv.take<double>(dg_total);
   const size_t o_dP = cv.take<double>((size_t)b->dp_off[nc]);
-  const size_t o_args = cv.take<DPArgs>(nc), o_spo = cv.take<int64_t>(nc + 1);
+  const size_t o_args = cv.take<DPArgs>(nc), o_spo = cv.take<int64_t>(sp_off.size());
   FE_HIPCHK(hipSetDevice(c->device));
   if (int rc = batch_alloc(&b->bmem, cv.off, "backward buffers")) return rc;
   char* m = (char*)b->bmem;
@@ -748,9 +753,13 @@ static int plan_bwd(fe_batch* b) {
   for (int i = 0; i < nc; ++i)
     args[i] = fe_dp_args(b->tabs[i], b->dG + dgbase[i], dg_off[i].data(), b->dP + b->dp_off[i]);
   FE_HIPCHK(hipMemcpy(b->dpargs, args.data(), nc * sizeof(DPArgs), hipMemcpyHostToDevice));
-  FE_HIPCHK(hipMemcpy(b->sp_off, sp_off.data(), (nc + 1) * sizeof(int64_t),
+  FE_HIPCHK(hipMemcpy(b->sp_off, sp_off.data(), sp_off.size() * sizeof(int64_t),
                       hipMemcpyHostToDevice));
-  b->n_sp_tot = sp_off[nc];
+  b->n_sp_tot = 0;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    b->n_sp[L] = sp_off[(size_t)L * (nc + 1) + nc];
+    b->n_sp_tot += b->n_sp[L];
+  }
   b->bplan = true;
   return FE_OK;
 }
@@ -816,7 +825,16 @@ static void launch_dG_L(fe_batch* b, hipStream_t st) {
   }
 }
 
-static void launch_backward(fe_batch* b, hipStream_t st) {
+static void launch_dp_L(fe_batch* b, int L, hipStream_t st) {
+  if (!b->n_sp[L]) return;
+  const int64_t grid = std::min<int64_t>(b->n_sp[L], (int64_t)b->ctx->n_cu * 32);
+  const int pd = prof_begin(b, FE_PH_DP, st);
+  hipLaunchKernelGGL(k_dg_to_dp_batch, dim3((unsigned)grid), dim3(64), 0, st, b->dpargs,
+                     b->sp_off + (size_t)L * (b->n_conf + 1), b->n_conf, b->n_sp[L], L);
+  fe_prof_end(b->ctx, pd, st);
+}
+
+static int launch_backward(fe_batch* b, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, b->counters + CNT_DG, FE_LMAX + 1);
   if (b->n_tot) {
     const int pp = prof_begin(b, FE_PH_PACK, st);
@@ -824,20 +842,28 @@ static void launch_backward(fe_batch* b, hipStream_t st) {
                        b->pts, b->gV, b->perm, (int)b->n_tot, b->pw);
     fe_prof_end(b->ctx, pp, st);
   }
-  launch_dG_L<0>(b, st);
+  // the classes on two streams (as in the single-table backward), each class's
+  // dL/dP right after its dL/dG
+  hipStream_t s2 = st;
+  if (b->ctx->split_dg)
+    if (int rc = fe_fork(b->ctx, st, &s2)) return rc;
   launch_dG_L<1>(b, st);
-  launch_dG_L<2>(b, st);
+  launch_dp_L(b, 1, st);
+  launch_dG_L<2>(b, s2);
+  launch_dp_L(b, 2, s2);
   launch_dG_L<3>(b, st);
-  launch_dG_L<4>(b, st);
+  launch_dp_L(b, 3, st);
+  launch_dG_L<0>(b, s2);
+  launch_dp_L(b, 0, s2);
   launch_dG_L<5>(b, st);
-  launch_dG_L<6>(b, st);
-  if (b->n_sp_tot) {
-    const int64_t grid = std::min<int64_t>(b->n_sp_tot, (int64_t)b->ctx->n_cu * 32);
-    const int pd = prof_begin(b, FE_PH_DP, st);
-    hipLaunchKernelGGL(k_dg_to_dp_batch, dim3((unsigned)grid), dim3(64), 0, st, b->dpargs,
-                       b->sp_off, b->n_conf, b->n_sp_tot);
-    fe_prof_end(b->ctx, pd, st);
-  }
+  launch_dp_L(b, 5, st);
+  launch_dG_L<4>(b, s2);
+  launch_dp_L(b, 4, s2);
+  launch_dG_L<6>(b, s2);
+  launch_dp_L(b, 6, s2);
+  if (b->ctx->split_dg)
+    if (int rc = fe_join(b->ctx, st)) return rc;
+  return FE_OK;
 }
 
 static int check_prec(int precision, const char* fn) {
@@ -880,7 +906,7 @@ extern "C" int fe_batch_backward(fe_batch* b, const double* gV, int ptr_kind, vo
                              ptr_kind == FE_PTR_HOST ? hipMemcpyHostToDevice
                                                      : hipMemcpyDeviceToDevice,
                              st));
-  launch_backward(b, st);
+  if (int rc = launch_backward(b, st)) return rc;
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
 }
@@ -914,7 +940,8 @@ static int launch_step(fe_batch* b, int flags, hipStream_t st) {
     if (int rc = launch_rebuilds(b, st)) return rc;
   launch_forward(b, st);
   launch_loss(b, st);
-  if (flags & FE_BATCH_DLDP) launch_backward(b, st);
+  if (flags & FE_BATCH_DLDP)
+    if (int rc = launch_backward(b, st)) return rc;
   return FE_OK;
 }
 
