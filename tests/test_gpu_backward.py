@@ -182,3 +182,32 @@ def test_trim_releases_workspace_and_next_call_works():
     _lib.Context.get().trim()
     V1, F1 = esp_with_grad(t, pts)
     assert np.array_equal(V0, V1) and np.array_equal(F0, F1)
+
+
+def test_stream_modes_agree():
+    """The dL/dG classes on two streams (default) or one at a time give the
+    same bits; the evaluation classes on two streams (tuning key 7, own
+    partial rows) change only the summation order."""
+    from fastesp_amd import _lib
+    ctx = _lib.Context.get(0)
+    w = synth.config0(1500)
+    t = build_table_hip(w.wfn, w.basis, w.density, w.points, tol=1e-12)
+    gV = np.linspace(-1.0, 1.0, len(w.points))
+    out = {}
+    try:
+        for dg2, ev2 in ((1, 0), (0, 0), (1, 1)):
+            say(f"dL/dG streams {1 + dg2}, evaluation streams {1 + ev2}")
+            ctx.set_tuning(6, dg2)
+            ctx.set_tuning(7, ev2)
+            V, F = esp_with_grad(t, w.points)
+            dG, dP = esp_backward(t, w.points, gV, want_dG=True, n_func=w.basis.n)
+            out[(dg2, ev2)] = (V, F, dG, dP)
+    finally:
+        ctx.set_tuning(6, 1)
+        ctx.set_tuning(7, 0)
+    for a, b in zip(out[(1, 0)], out[(0, 0)]):
+        assert np.array_equal(a, b)
+    V0, F0, dG0, dP0 = out[(1, 0)]
+    V2, F2, dG2, dP2 = out[(1, 1)]
+    assert np.abs(V2 - V0).max() < 1e-12 and np.abs(F2 - F0).max() < 1e-12
+    assert np.array_equal(dG2, dG0) and np.array_equal(dP2, dP0)
