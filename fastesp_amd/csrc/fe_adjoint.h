@@ -1,8 +1,9 @@
 // Adjoint device code shared by the single-table and the batched backward.
 //
-// fe_dg_block<L>: one lane per pair of class L; the points (x, y, z, gV) of
-//   [q0, q1) are wave-uniform scalar loads, so there is no cross-lane
-//   reduction.  Writes out[k * stride + pair] = -sum_i gV_i K R_i^a F_m(T_i).
+// fe_dg_sum<L> / fe_dg_block<L>: one lane per pair of class L; the points
+//   (x, y, z, gV) of [q0, q1) are wave-uniform scalar loads, so there is no
+//   cross-lane reduction.  fe_dg_block writes
+//   out[k * stride + pair] = -sum_i gV_i K R_i^a F_m(T_i).
 // fe_dg_to_dp_sp: dL/dG -> dL/dP for one shell pair (64 lanes):
 //   d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t], d blk_{mu nu} = c_i c_j
 //   sum_t d etab_t Ex Ey Ez; both triangles of P receive d blk because the
@@ -28,11 +29,13 @@ struct DPArgs {
   LTabDev lt;
 };
 
+// acc[k] = sum_{i in [q0, q1)} gV_i K R_i^a F_m(T_i) for the lane's pair (the
+// pair index is clamped to the last pair; an empty range gives zeros)
 template <int L>
-__device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, const int n_pairs,
-                                            const int pair, const double* __restrict__ pw,
-                                            const int q0, const int q1, const double* s_tab,
-                                            double* __restrict__ out, const int stride) {
+__device__ __forceinline__ void fe_dg_sum(const double* __restrict__ rec, const int n_pairs,
+                                          const int pair, const double* __restrict__ pw,
+                                          const int q0, const int q1, const double* s_tab,
+                                          double* acc) {
   constexpr int S = FeL<L>::STRIDE, NT = FeL<L>::NT;
   const int ip = min(pair, n_pairs - 1);
   const double* r = rec + (size_t)ip * S;
@@ -47,7 +50,8 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   // w K F_m with K F_0 folded in (no final scale).
   constexpr bool SPLIT = fe_dg_minw(L) == 1;
   // T = p r^2 >= T_far(L)  <=>  r^2 >= T_far(L) / p: one compare per point
-  // (the product p r^2 is formed only on the tabulated path)
+  // (the product p r^2 is formed only on the tabulated path).  The class
+  // under a register cap tests p r^2 itself: one live value fewer.
   const double r2far = fe_tfar(L) * (hp + hp);
   double D[L + 1];
   if constexpr (SPLIT) {
@@ -58,7 +62,6 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
       c *= (2 * m + 1) * hp;
     }
   }
-  double acc[NT];
 #pragma unroll
   for (int k = 0; k < NT; ++k) acc[k] = 0.0;
   // far-field weights of one point (lanes with T >= T_far)
@@ -81,18 +84,27 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
     const double Rx = x - Px, Ry = y - Py, Rz = z - Pz;
     const double r2 = fma(Rx, Rx, fma(Ry, Ry, Rz * Rz));
     double wm[L + 1];
-    const bool near = r2 < r2far;
+    const bool near = SPLIT ? r2 < r2far : p * r2 < fe_tfar(L);
     if (!__any(near)) {
       far_weights(r2, w, wm);
-    } else if (near) {
+    } else if constexpr (SPLIT) {
       // a tabulated wavefront: its near lanes take the table, the others
       // the same far-field weights as an all-far wavefront
-      double F[L + 1];
-      fe_boys_tab<L>(p * r2, F, s_tab);
+      if (near) {
+        double F[L + 1];
+        fe_boys_tab<L>(p * r2, F, s_tab);
 #pragma unroll
-      for (int m = 0; m <= L; ++m) wm[m] = SPLIT ? (w * F[m]) * D[m] : (w * K) * F[m];
+        for (int m = 0; m <= L; ++m) wm[m] = (w * F[m]) * D[m];
+      } else {
+        far_weights(r2, w, wm);
+      }
     } else {
-      far_weights(r2, w, wm);
+      // the class under a register cap: one Boys evaluation for all lanes
+      // (table or asymptotic form per lane) keeps fewer values live
+      double F[L + 1];
+      fe_boys<L>(p * r2, F, s_tab);
+#pragma unroll
+      for (int m = 0; m <= L; ++m) wm[m] = (w * K) * F[m];
     }
     fe_dg_accum<L>(Rx, Ry, Rz, wm, acc);
   };
@@ -115,6 +127,17 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
     for (int m = 1; m <= L; ++m) C[m] = C[m - 1] * ((2 * m - 1) * r[6]);
     fe_dg_scale<L>(acc, C);
   }
+}
+
+// one lane per pair: writes out[k * stride + pair] = -sum over [q0, q1)
+template <int L>
+__device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, const int n_pairs,
+                                            const int pair, const double* __restrict__ pw,
+                                            const int q0, const int q1, const double* s_tab,
+                                            double* __restrict__ out, const int stride) {
+  constexpr int NT = FeL<L>::NT;
+  double acc[NT];
+  fe_dg_sum<L>(rec, n_pairs, pair, pw, q0, q1, s_tab, acc);
   if (pair < n_pairs) {
 #pragma unroll
     for (int k = 0; k < NT; ++k) out[(size_t)k * stride + pair] = -acc[k];

@@ -206,7 +206,7 @@ static DGPlan plan_dG(const fe_table* t, int64_t n, int target_wg) {
     const int NT = fe_host::NTERM[L];
     const int64_t nblk = (nL + 255) / 256;
     int64_t ch = std::max<int64_t>(1, target_wg / nblk);
-    // bound the partial buffer (written and re-read once) and keep >= 128
+    // bound the partial buffer (written and re-read once) and keep >= 64
     // points per chunk so the reduction stays small next to the kernel
     ch = std::min<int64_t>(ch, std::max<int64_t>(1, (16ll << 20) / (nL * NT)));
     ch = std::min<int64_t>(ch, std::max<int64_t>(1, n / 64));
@@ -416,12 +416,13 @@ __global__ __launch_bounds__(FE_WG) void k_count_eval(const double* __restrict__
   unsigned long long far = 0, near = 0;
   for (int ip = p0; ip < p1; ++ip) {
     const double* r = rec + (size_t)ip * S;
-    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3];
+    const double Px = r[0], Py = r[1], Pz = r[2];
+    const double r2far = fe_tfar(M) * (r[6] + r[6]);  // the far test of fe_eval_tile
     int nr = 0;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
       const double x = cx[q] - Px, y = cy[q] - Py, z = cz[q] - Pz;
-      nr |= p * fma(x, x, fma(y, y, z * z)) < fe_tfar(M);
+      nr |= fma(x, x, fma(y, y, z * z)) < r2far;
     }
     if (__any(nr)) ++near;
     else ++far;
@@ -443,14 +444,17 @@ __global__ __launch_bounds__(256) void k_count_dg(const double* __restrict__ rec
   const int pair = blockIdx.x * 256 + threadIdx.x;
   const int ip = min(pair, n_pairs - 1);
   const double* r = rec + (size_t)ip * S;
-  const double Px = r[0], Py = r[1], Pz = r[2], p = r[3];
+  const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], hp = r[6];
+  const double r2far = fe_tfar(L) * (hp + hp);  // the far test of fe_dg_sum
+  constexpr bool SPLIT = fe_dg_minw(L) == 1;
   const int nvp = __popcll(__ballot(pair < n_pairs));
   const int q0 = blockIdx.y * ppc, q1 = min(n_pts, q0 + ppc);
   unsigned long long far = 0, near = 0;
   for (int q = q0; q < q1; ++q) {
     const double x = pts[3 * (size_t)q] - Px, y = pts[3 * (size_t)q + 1] - Py,
                  z = pts[3 * (size_t)q + 2] - Pz;
-    if (__any(p * fma(x, x, fma(y, y, z * z)) < fe_tfar(L))) ++near;
+    const double r2 = fma(x, x, fma(y, y, z * z));
+    if (__any(SPLIT ? r2 < r2far : p * r2 < fe_tfar(L))) ++near;
     else ++far;
   }
   if ((threadIdx.x & 63) == 0 && nvp > 0) {
