@@ -522,8 +522,8 @@ __global__ __launch_bounds__(256) void k_build_stats(const double* __restrict__ 
 // slots 6/7 holding the far-field constants 1/(2p) and K sqrt(pi/p)/2 instead
 // of reff and the flags; column 0's thread also writes the flat term offset,
 // the reference-order index and the slot -> record map.
-__global__ __launch_bounds__(256) void k_compact(const double* __restrict__ cand, const int S,
-                                                 const int64_t slot_base,
+__global__ __launch_bounds__(256) void k_compact(const double* __restrict__ cand, const int CS,
+                                                 const int S, const int64_t slot_base,
                                                  const int32_t* __restrict__ sorted,
                                                  const int* __restrict__ kept_p, const int cap,
                                                  const int* __restrict__ gidx,
@@ -533,12 +533,13 @@ __global__ __launch_bounds__(256) void k_compact(const double* __restrict__ cand
                                                  int* __restrict__ slot2kept) {
   const int n = min(*kept_p, cap);
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)n * S) return;
-  const int r = (int)(e / S), c = (int)(e - (int64_t)r * S);
+  if (e >= (int64_t)n * CS) return;
+  const int r = (int)(e / CS), c = (int)(e - (int64_t)r * CS);
   const int ge = sorted[r];
   const int slot = (int)(ge - slot_base);
-  const double* s = cand + (size_t)slot * S;
-  rec[e] = c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
+  const double* s = cand + (size_t)slot * CS;
+  rec[(size_t)r * S + c] =
+      c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
   if (c == 0) {
     const unsigned long long g = gscan[gidx[ge]];
     flatoff[r] = (int)(g >> 32);
@@ -547,12 +548,13 @@ __global__ __launch_bounds__(256) void k_compact(const double* __restrict__ cand
   }
 }
 
-// the scaled coefficients c g_k of each kept record of class L (one lane per record)
-__global__ __launch_bounds__(256) void k_fill_dcoef(double* __restrict__ rec, const int S,
-                                                    const int* __restrict__ kept_p, const int cap,
-                                                    const int L) {
+// the derived parts of each kept record of class L (one lane per record): the
+// scaled coefficients c g_k and the far-field block
+__global__ __launch_bounds__(256) void k_fill_record(double* __restrict__ rec, const int S,
+                                                     const int* __restrict__ kept_p, const int cap,
+                                                     const int L) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < min(*kept_p, cap)) fe_fill_dcoef_any(L, rec + (size_t)i * S + 8);
+  if (i < min(*kept_p, cap)) fe_fill_record_any(L, rec + (size_t)i * S);
 }
 
 // ---------------------------------------------------------------------------
@@ -609,7 +611,7 @@ static BuildLayout build_layout(const fe_table* t, int n_points, bool host_in) {
   if (host_in) wo += al256((size_t)n_points * 3 * sizeof(double));
   for (int L = 0; L <= FE_LMAX; ++L) {
     y.o_cand[L] = wo;
-    wo += al256((size_t)t->n_cand_L[L] * fe_host::STRIDE[L] * sizeof(double));
+    wo += al256((size_t)t->n_cand_L[L] * fe_host::CSTRIDE[L] * sizeof(double));
   }
   const size_t na = (size_t)std::max<int64_t>(n_all, 1);
   y.o_gidx = wo;
@@ -655,7 +657,7 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   const int nsp = t->n_sp;
   ClassInfo ci{};
   for (int L = 0; L <= FE_LMAX; ++L) {
-    ci.stride[L] = fe_host::STRIDE[L];
+    ci.stride[L] = fe_host::CSTRIDE[L];
     ci.nterm[L] = fe_host::NTERM[L];
     ci.nmono[L] = fe_host::NMONO[L];
     ci.cand_off[L + 1] = ci.cand_off[L] + t->n_cand_L[L];
@@ -799,16 +801,15 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((t->n_cand_L[L] + 255) / 256)), dim3(256), 0,
                        st, t->d_slot2kept[L], t->n_cand_L[L], -1);
     if (!t->cap_L[L]) continue;
-    const int S = fe_host::STRIDE[L];
+    const int S = fe_host::STRIDE[L], CS = fe_host::CSTRIDE[L];
     const int* kept = t->d_stats + FE_ST_KEPT + L;
-    const int64_t ne = t->cap_L[L] * S;
+    const int64_t ne = t->cap_L[L] * CS;
     hipLaunchKernelGGL(k_compact, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st,
-                       ba.cand[L], S, ci.cand_off[L], val + ci.cand_off[L], kept,
+                       ba.cand[L], CS, S, ci.cand_off[L], val + ci.cand_off[L], kept,
                        (int)t->cap_L[L], ba.gidx, gflag, t->d_rec[L], t->d_flatoff[L],
                        t->d_orig[L], t->d_slot2kept[L]);
-    if (fe_host::NDCOEF[L])
-      hipLaunchKernelGGL(k_fill_dcoef, dim3((unsigned)((t->cap_L[L] + 255) / 256)), dim3(256), 0,
-                         st, t->d_rec[L], S, kept, (int)t->cap_L[L], L);
+    hipLaunchKernelGGL(k_fill_record, dim3((unsigned)((t->cap_L[L] + 255) / 256)), dim3(256), 0,
+                       st, t->d_rec[L], S, kept, (int)t->cap_L[L], L);
   }
   FE_HIPCHK(hipGetLastError());
   return FE_OK;

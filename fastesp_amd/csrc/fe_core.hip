@@ -8,6 +8,9 @@
 //     [8 .. 8+NT)  g_k in canonical key order
 //     [8+NT .. 8+NT+ND)  the integer multiples c g_k the field contraction uses
 //     (fe_fill_dcoef<L>), zero padded to a multiple of 8
+//     [KOFF .. KOFF+NK)  the far-field block (fe_fill_kelvin<L>): the pair's
+//     potential where every Boys order is asymptotic, as |u| H(u) with
+//     u = R/|R|^2 and H a polynomial of degree L, zero padded to a multiple of 8
 //
 // A wavefront evaluates 64 points (x points-per-lane) against the same pair,
 // so the whole record is wave-uniform and is fetched with scalar loads into
@@ -616,7 +619,7 @@ extern "C" int fe_table_from_pairs(fe_ctx* c, int n_pairs, int n_terms, const do
     r[6] = 0.5 / pp_exp[i];                        // 1/(2p)
     r[7] = pp_K[i] * (0.88622692545275801365 / sqrt(pp_exp[i]));  // K sqrt(pi/p)/2
     for (int j = 0; j < nt; ++j) r[8 + j] = tm_g[off + j];
-    fe_fill_dcoef_any(L, r + 8);
+    fe_fill_record_any(L, r);
     orig[L].push_back(i);
   }
   if (c->pair_order)

@@ -23,7 +23,7 @@
 // VGPRs.  1 leaves the allocation to the compiler.  Overridable at build time (-DFE_EVAL_MINW_L=...,
 // -DFE_DG_MINW_L=..., comma lists of 7 values).
 #ifndef FE_EVAL_MINW_L
-#define FE_EVAL_MINW_L 1, 1, 1, 1, 1, 1, 1
+#define FE_EVAL_MINW_L 1, 1, 1, 1, 3, 1, 1
 #endif
 #ifndef FE_DG_MINW_L
 #define FE_DG_MINW_L 1, 1, 1, 1, 1, 2, 1
@@ -32,6 +32,12 @@
 // the contraction (fe_contract_grad_acc) instead of weighting its four results.
 #ifndef FE_GRAD_ACC_LMAX
 #define FE_GRAD_ACC_LMAX 2
+#endif
+// Lowest class whose far-field wavefronts (value, and value + field) use the
+// record's far-field block (|u| H(u), u = R/|R|^2) instead of the asymptotic
+// Boys vector and the keyed contraction; FE_LMAX + 1 turns it off.
+#ifndef FE_KELVIN_LMIN
+#define FE_KELVIN_LMIN 0
 #endif
 __host__ __device__ constexpr int fe_eval_minw(int L, int ppl) {
   constexpr int t[7] = {FE_EVAL_MINW_L};

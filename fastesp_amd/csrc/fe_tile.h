@@ -23,6 +23,129 @@ __device__ __forceinline__ void fe_kahan_add(double& s, double& c, const double 
   s = t;
 }
 
+// R = C - P and |R|^2 of the wavefront's PPL points per lane against record r;
+// returns whether every lane is past the Boys switch point of the top order
+// (wave-uniform, read from a scalar so branches on it are scalar branches),
+// and under MIXED whether some lane needs the FP64 contraction.
+template <int L, int PPL, bool GRAD, bool MIXED>
+__device__ __forceinline__ bool fe_pair_geometry(const fe_gdptr r, const double* cx,
+                                                 const double* cy, const double* cz,
+                                                 const float tau, double* Rx, double* Ry,
+                                                 double* Rz, double* r2, bool& fp64) {
+  constexpr int M = GRAD ? L + 1 : L;
+  const double Px = r[0], Py = r[1], Pz = r[2];
+  // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p (one product per pair)
+  const double r2far = fe_tfar(M) * (r[6] + r[6]);
+  int near = 0, big = 0;
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    Rx[q] = cx[q] - Px;
+    Ry[q] = cy[q] - Py;
+    Rz[q] = cz[q] - Pz;
+    r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
+    near |= r2[q] < r2far;
+    if constexpr (MIXED) big |= (float)r[5] * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
+  }
+  fp64 = true;
+  if constexpr (MIXED) fp64 = __any(big);
+  return __builtin_amdgcn_readfirstlane(!__any(near));
+}
+
+// The points of one record by its Boys vector (tabulated, or the per-lane
+// asymptotic form past the switch point; with far = true every lane takes the
+// latter) and the keyed contraction.
+template <int L, int PPL, bool GRAD, bool MIXED>
+__device__ __forceinline__ void fe_keyed_points(const fe_gdptr r, const fe_gfptr g,
+                                                const bool far, const bool fp64,
+                                                const double* Rx, const double* Ry,
+                                                const double* Rz, const double* r2,
+                                                const double* s_tab, double* av, double* cv,
+                                                double* ax, double* ay, double* az) {
+  constexpr int M = GRAD ? L + 1 : L;
+  const double p = r[3], K = r[4], hp = r[6], KF = r[7];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    double F[M + 1];
+    double w = K;
+    if (far) {
+      // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
+      const double ri = fe_rsqrt_pos(r2[q]);
+      fe_boys_far_scaled<M>(hp * (ri * ri), F);
+      w = KF * ri;
+    } else {
+      fe_boys<M>(p * r2[q], F, s_tab);
+    }
+    if constexpr (GRAD && L <= FE_GRAD_ACC_LMAX && !MIXED) {
+      // the weight folded into F before the contraction: fewer operations
+      // than weighting its four results for the light classes
+      fe_contract_grad_acc<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, w, av[q], ax[q], ay[q], az[q]);
+    } else if constexpr (GRAD) {
+      double I, Gx, Gy, Gz;
+      if (fp64)
+        fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
+      else
+        fe_contract_grad_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g, p, Rx[q], Ry[q],
+                                Rz[q], I, Gx, Gy, Gz);
+      if constexpr (MIXED)
+        fe_kahan_add(av[q], cv[q], w * I);
+      else
+        av[q] = fma(w, I, av[q]);
+      ax[q] = fma(w, Gx, ax[q]);
+      ay[q] = fma(w, Gy, ay[q]);
+      az[q] = fma(w, Gz, az[q]);
+    } else {
+      double s;
+      if (fp64)
+        s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
+      else
+        s = fe_contract_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g);
+      if constexpr (MIXED)
+        fe_kahan_add(av[q], cv[q], w * s);
+      else
+        av[q] = fma(w, s, av[q]);
+    }
+  }
+}
+
+// The points of one record whose every Boys order is asymptotic on every lane:
+// the pair's integral is |u| H(u), u = R/|R|^2, from the record's far-field
+// block, and (GRAD) its gradient |u|^3 (grad H - (H + 2 u.grad H) R).  FP64 in
+// every mode.
+template <int L, int PPL, bool GRAD>
+__device__ __forceinline__ void fe_far_points(const fe_gdptr r, const double* Rx,
+                                              const double* Ry, const double* Rz,
+                                              const double* r2, double* av, double* ax,
+                                              double* ay, double* az) {
+  const fe_gdptr kel = r + FeL<L>::KOFF;
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const double ri = fe_rsqrt_pos(r2[q]);
+    if constexpr (!GRAD && L == 0) {
+      av[q] = fma(ri, kel[0], av[q]);
+    } else if constexpr (!GRAD) {
+      const double ri2 = ri * ri;
+      av[q] = fma(ri, fe_far_value<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel), av[q]);
+    } else if constexpr (L == 0) {
+      const double ri3 = ri * (ri * ri);
+      const double H = kel[0];
+      av[q] = fma(ri, H, av[q]);
+      const double c = -H * ri3;
+      ax[q] = fma(c, Rx[q], ax[q]);
+      ay[q] = fma(c, Ry[q], ay[q]);
+      az[q] = fma(c, Rz[q], az[q]);
+    } else {
+      const double ri2 = ri * ri, ri3 = ri * ri2;
+      double H, Hx, Hy, Hz, D;
+      fe_far_grad<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel, H, Hx, Hy, Hz, D);
+      const double t = fma(2.0, D, H);
+      av[q] = fma(ri, H, av[q]);
+      ax[q] = fma(ri3, fma(-t, Rx[q], Hx), ax[q]);
+      ay[q] = fma(ri3, fma(-t, Ry[q], Hy), ay[q]);
+      az[q] = fma(ri3, fma(-t, Rz[q], Hz), az[q]);
+    }
+  }
+}
+
 template <int L, int PPL, bool GRAD, bool MIXED>
 __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
                                              fe_gfptr __restrict__ g32, const int p0,
@@ -31,69 +154,44 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
                                              const float tau, double* av, double* cv, double* ax,
                                              double* ay, double* az) {
   constexpr int S = FeL<L>::STRIDE, S32 = FE_S32(FeL<L>::NT + FeL<L>::ND);
-  constexpr int M = GRAD ? L + 1 : L;
-  for (int ip = p0; ip < p1; ++ip) {
-    const fe_gdptr r = rec + (size_t)ip * S;
-    const double Px = r[0], Py = r[1], Pz = r[2], p = r[3], K = r[4];
-    // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p (one product per pair)
-    const double r2far = fe_tfar(M) * (r[6] + r[6]);
-    double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
-    int near = 0, big = 0;
+  double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
+  bool fp64;
+  if constexpr (L >= FE_KELVIN_LMIN) {
+    // Each path forms its contributions from zero and one accumulation after
+    // the join adds them (with each path adding into the running sums itself,
+    // the compiler kept them in other registers in the two paths and copied
+    // them at every join).
+    for (int ip = p0; ip < p1; ++ip) {
+      const fe_gdptr r = rec + (size_t)ip * S;
+      double dv[PPL], dc[PPL], dx[PPL], dy[PPL], dz[PPL];
 #pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      Rx[q] = cx[q] - Px;
-      Ry[q] = cy[q] - Py;
-      Rz[q] = cz[q] - Pz;
-      r2[q] = fma(Rx[q], Rx[q], fma(Ry[q], Ry[q], Rz[q] * Rz[q]));
-      near |= r2[q] < r2far;
-      if constexpr (MIXED) big |= (float)r[5] * rsqrtf(fmaxf((float)r2[q], 0.0025f)) >= tau;
+      for (int q = 0; q < PPL; ++q) dv[q] = dc[q] = dx[q] = dy[q] = dz[q] = 0.0;
+      if (fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64))
+        fe_far_points<L, PPL, GRAD>(r, Rx, Ry, Rz, r2, dv, dx, dy, dz);
+      else
+        fe_keyed_points<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, false, fp64, Rx, Ry,
+                                             Rz, r2, s_tab, dv, dc, dx, dy, dz);
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        if constexpr (MIXED) {
+          fe_kahan_add(av[q], cv[q], dv[q]);
+          cv[q] += dc[q];
+        } else {
+          av[q] += dv[q];
+        }
+        if constexpr (GRAD) {
+          ax[q] += dx[q];
+          ay[q] += dy[q];
+          az[q] += dz[q];
+        }
+      }
     }
-    const bool far = !__any(near);
-    bool fp64 = true;
-    if constexpr (MIXED) fp64 = __any(big);
-    const double hp = r[6], KF = r[7];
-    const fe_gfptr g = g32 + (size_t)ip * S32;
-#pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      double F[M + 1];
-      double w = K;
-      if (far) {
-        // F_m = F_0 F'_m with K F_0 = KF / |R| folded into the weight
-        const double ri = fe_rsqrt_pos(r2[q]);
-        fe_boys_far_scaled<M>(hp * (ri * ri), F);
-        w = KF * ri;
-      } else {
-        fe_boys<M>(p * r2[q], F, s_tab);
-      }
-      if constexpr (GRAD && L <= FE_GRAD_ACC_LMAX && !MIXED) {
-        // the weight folded into F before the contraction: fewer operations
-        // than weighting its four results for the light classes
-        fe_contract_grad_acc<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, w, av[q], ax[q], ay[q], az[q]);
-      } else if constexpr (GRAD) {
-        double I, Gx, Gy, Gz;
-        if (fp64)
-          fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, I, Gx, Gy, Gz);
-        else
-          fe_contract_grad_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g, p, Rx[q],
-                                  Ry[q], Rz[q], I, Gx, Gy, Gz);
-        if constexpr (MIXED)
-          fe_kahan_add(av[q], cv[q], w * I);
-        else
-          av[q] = fma(w, I, av[q]);
-        ax[q] = fma(w, Gx, ax[q]);
-        ay[q] = fma(w, Gy, ay[q]);
-        az[q] = fma(w, Gz, az[q]);
-      } else {
-        double s;
-        if (fp64)
-          s = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
-        else
-          s = fe_contract_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g);
-        if constexpr (MIXED)
-          fe_kahan_add(av[q], cv[q], w * s);
-        else
-          av[q] = fma(w, s, av[q]);
-      }
+  } else {
+    for (int ip = p0; ip < p1; ++ip) {
+      const fe_gdptr r = rec + (size_t)ip * S;
+      const bool far = fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64);
+      fe_keyed_points<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, far, fp64, Rx, Ry, Rz, r2,
+                                           s_tab, av, cv, ax, ay, az);
     }
   }
 }

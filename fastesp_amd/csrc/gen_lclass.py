@@ -95,6 +95,194 @@ def stride_of(nt):
     return 8 + ((nt + 7) // 8) * 8
 
 
+def kelvin_of(L):
+    """The far-field form of class L as a polynomial in u = R / |R|^2.
+
+    Where T = p |R|^2 is past the Boys switch point, K F_m(T) equals
+    S_m |R|^-(2m+1) with S_m = K sqrt(pi/p)/2 (2m-1)!! (2p)^-m, so a pair's
+    electron integral is sum_k g_k S_m R^a |R|^-(2m+1).  With R^a = u^a |R|^(2|a|)
+    and |u| = 1/|R| every term is |u| S_m g_k u^a (u.u)^(m-|a|); m >= |a| for
+    every key, so I = |u| H(u) with H a polynomial of degree <= L (the Kelvin
+    inversion of the pair's multipole potential).  Returns the monomials b of H
+    with their contributions (key k, multinomial factor, order m)."""
+    from math import factorial
+    keys, _ = keys_of(L)
+    h = {}
+    for k, (ax, ay, az, m) in enumerate(keys):
+        j = m - (ax + ay + az)
+        assert j >= 0, (L, ax, ay, az, m)
+        for i1 in range(j + 1):
+            for i2 in range(j - i1 + 1):
+                i3 = j - i1 - i2
+                c = factorial(j) // (factorial(i1) * factorial(i2) * factorial(i3))
+                b = (ax + 2 * i1, ay + 2 * i2, az + 2 * i3)
+                h.setdefault(b, []).append((k, c, m))
+    return h
+
+
+def kelvin_monos(L):
+    """Monomials of degree <= L - 1 in sorted order (each one's parent, the
+    first nonzero axis lowered, comes earlier) with (parent, axis)."""
+    order = sorted((a, b, c) for a in range(L) for b in range(L) for c in range(L)
+                   if a + b + c <= L - 1)
+    idx = {m: i for i, m in enumerate(order)}
+    parent, axis = [], []
+    for m in order:
+        par, axv = 0, 0
+        for ax in range(3):
+            if m[ax] > 0:
+                w = list(m)
+                w[ax] -= 1
+                par, axv = idx[tuple(w)], ax
+                break
+        parent.append(par)
+        axis.append(axv)
+    return order, parent, axis
+
+
+def kelvin_top(L):
+    """The degree-L monomials of H grouped by the axis of their parent (the
+    first nonzero axis lowered, a monomial of degree L - 1): [(a, axis,
+    parent index in kelvin_monos(L))], x group first."""
+    kord = kelvin_monos(L)[0]
+    idx = {m: i for i, m in enumerate(kord)}
+    out = []
+    for a in sorted((x, y, L - x - y) for x in range(L + 1) for y in range(L + 1 - x)):
+        ax = next(d for d in range(3) if a[d] > 0)
+        par = list(a)
+        par[ax] -= 1
+        out.append((a, ax, idx[tuple(par)]))
+    return sorted(out, key=lambda t: t[1])
+
+
+def nkel_of(L):
+    """Length of the record's far-field block: h_0 for L = 0; else, over the
+    monomials of degree <= L - 1, the coefficients of Q = H - (u . grad H)/L,
+    of the three components of grad H and of H itself, then H's degree-L
+    coefficients (kelvin_top order)."""
+    if L == 0:
+        return 1
+    n1 = len(kelvin_monos(L)[0])
+    return 5 * n1 + (L + 1) * (L + 2) // 2
+
+
+def record_stride(L):
+    """Doubles per device record of class L: header, g_k and c g_k (the
+    candidate record the table build writes), then the far-field block."""
+    keys, _ = keys_of(L)
+    return stride_of(len(keys) + len(dcoef_of(L))) + ((nkel_of(L) + 7) // 8) * 8
+
+
+def gen_kelvin(L, keys, koff):
+    """fe_fill_kelvin<L>: the record's far-field block from its header and g_k
+    (once per pair, at table build or upload); fe_far_grad<L>: H(u), grad H
+    and D = u . grad H at one point (the far-field value + field).
+
+    Field: with n = R/|R| and du/dR = (1 - 2 n n^T)/|R|^2, the gradient of
+    |u| H(u) with respect to R is |u|^3 (grad H - (H + 2 D) R)."""
+    o = []
+    h = kelvin_of(L)
+    ms = sorted({m for lst in h.values() for (_, _, m) in lst})
+    o.append(f"template <> __host__ __device__ inline void fe_fill_kelvin<{L}>(double* r) {{")
+    o.append("  const double hp = r[6];")
+    o.append("  const double* g = r + 8;")
+    o.append(f"  double* kel = r + {koff};")
+    # s_m = K sqrt(pi/p)/2 (2m-1)!! (2p)^-m = s_{m-1} (2m-1) hp
+    o.append("  const double s0 = r[7];")
+    for m in range(1, max(ms) + 1):
+        o.append(f"  const double s{m} = s{m - 1} * ({2 * m - 1}.0 * hp);")
+    allb = sorted(h)
+    bidx = {b: i for i, b in enumerate(allb)}
+
+    def hname(b):
+        return f"h{bidx[b]}" if b in bidx else "0.0"
+
+    for b in allb:
+        terms = [(f"g[{k}] * s{m}" if c == 1 else f"{c}.0 * g[{k}] * s{m}") for (k, c, m) in h[b]]
+        o.append(f"  const double h{bidx[b]} = {' + '.join(terms)};")
+    if L == 0:
+        o.append("  kel[0] = h0;")
+        o.append("}")
+        return o
+    kord, kpar, kax = kelvin_monos(L)
+    n1 = len(kord)
+    for j, b in enumerate(kord):
+        d = sum(b)
+        q = hname(b) if d == 0 else f"{(L - d) / L!r} * {hname(b)}"
+        o.append(f"  kel[{j}] = {q};")
+    for a in range(3):
+        for j, b in enumerate(kord):
+            up = list(b)
+            up[a] += 1
+            up = tuple(up)
+            c = b[a] + 1
+            o.append(f"  kel[{(a + 1) * n1 + j}] = " + (f"{hname(up)};" if c == 1 else f"{c}.0 * {hname(up)};"))
+    top = kelvin_top(L)
+    for j, b in enumerate(kord):
+        o.append(f"  kel[{4 * n1 + j}] = {hname(b)};")
+    for j, (a, _, _) in enumerate(top):
+        o.append(f"  kel[{5 * n1 + j}] = {hname(a)};")
+    o.append("}")
+
+    o.append(f"template <> __device__ __forceinline__ void fe_far_grad<{L}>("
+             "const double ux, const double uy, const double uz, fe_gdptr __restrict__ kel, "
+             "double& H, double& Hx, double& Hy, double& Hz, double& D) {")
+    U = ("ux", "uy", "uz")
+
+    def mname(j):
+        return U[kax[j]] if sum(kord[j]) == 1 else f"m{j}"
+
+    # each monomial is formed and at once used by the four sums, so it stays
+    # live only while later monomials still need it as a parent
+    chains = (("Q", 0), ("Hx", n1), ("Hy", 2 * n1), ("Hz", 3 * n1))
+    o.append("  double Q = kel[0];")
+    for v, base in chains[1:]:
+        o.append(f"  {v} = kel[{base}];")
+    for j in range(1, n1):
+        if sum(kord[j]) >= 2:
+            o.append(f"  const double m{j} = {mname(kpar[j])} * {U[kax[j]]};")
+        for v, base in chains:
+            o.append(f"  {v} = fma(kel[{base + j}], {mname(j)}, {v});")
+    o.append("  D = fma(uz, Hz, fma(uy, Hy, ux * Hx));")
+    o.append(f"  H = " + ("Q + D;" if L == 1 else f"fma(D, {1.0 / L!r}, Q);"))
+    o.append("}")
+
+    # value only: H = sum over |b| <= L-1 of h_b u^b + ux Px + uy Py + uz Pz,
+    # the degree-L terms through their parents (never forming degree L)
+    o.append(f"template <> __device__ __forceinline__ double fe_far_value<{L}>("
+             "const double ux, const double uy, const double uz, fe_gdptr __restrict__ kel) {")
+    parents = {}
+    for j, (a, ax, par) in enumerate(top):
+        parents.setdefault(par, []).append((j, ax))
+    o.append(f"  double H = kel[{4 * n1}];")
+    init = set()
+
+    def topacc(j, ax, mono):
+        v = "P" + "xyz"[ax]
+        k = f"kel[{5 * n1 + j}]"
+        if v not in init:
+            init.add(v)
+            o.append(f"  double {v} = " + (f"{k};" if mono is None else f"{k} * {mono};"))
+        else:
+            o.append(f"  {v} = " + (f"{v} + {k};" if mono is None else f"fma({k}, {mono}, {v});"))
+
+    for j, ax in parents.get(0, []):
+        topacc(j, ax, None)
+    for j in range(1, n1):
+        if sum(kord[j]) >= 2:
+            o.append(f"  const double m{j} = {mname(kpar[j])} * {U[kax[j]]};")
+        o.append(f"  H = fma(kel[{4 * n1 + j}], {mname(j)}, H);")
+        for jj, ax in parents.get(j, []):
+            topacc(jj, ax, mname(j))
+    for ax in range(3):
+        v = "P" + "xyz"[ax]
+        if v in init:
+            o.append(f"  H = fma({U[ax]}, {v}, H);")
+    o.append("  return H;")
+    o.append("}")
+    return o
+
+
 DCOEF_LMAX = 3
 
 
@@ -211,13 +399,16 @@ def gen_L(L):
     dc = dcoef_of(L)
     dci = {kc: nt + j for j, kc in enumerate(dc)}  # (key, multiplier) -> slot after the g_k
     o = []
-    o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}, scaled coefficients = {len(dc)}")
+    koff, nk = stride_of(nt + len(dc)), nkel_of(L)
+    o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}, scaled coefficients = {len(dc)}, "
+             f"far-field coefficients = {nk}")
     o.append(f"template <> struct FeL<{L}> {{ static constexpr int NT = {nt}, NM = {nm}, "
-             f"ND = {len(dc)}, STRIDE = {stride_of(nt + len(dc))}; }};")
+             f"ND = {len(dc)}, KOFF = {koff}, NK = {nk}, STRIDE = {record_stride(L)}; }};")
     o.append(f"template <> __host__ __device__ inline void fe_fill_dcoef<{L}>(double* g) {{")
     for j, (k, c) in enumerate(dc):
         o.append(f"  g[{nt + j}] = {c}.0 * g[{k}];")
     o.append("}")
+    o.extend(gen_kelvin(L, keys, koff))
 
     def mono(j, pre="m", one="1.0"):
         return one if j == 0 else f"{pre}{j}"
@@ -450,6 +641,9 @@ def main():
            "template <int L> __device__ __forceinline__ void fe_contract_grad_f32(const float, const float, const float, const double*, fe_gfptr __restrict__, const double, const double, const double, const double, double&, double&, double&, double&);",
            "template <int L> __host__ __device__ inline void fe_fill_dcoef(double*);",
            "template <int L> __device__ __forceinline__ void fe_contract_grad_acc(const double, const double, const double, const double*, fe_gdptr __restrict__, const double, const double, double&, double&, double&, double&);",
+           "template <int L> __host__ __device__ inline void fe_fill_kelvin(double*);",
+           "template <int L> __device__ __forceinline__ void fe_far_grad(const double, const double, const double, fe_gdptr __restrict__, double&, double&, double&, double&, double&);",
+           "template <int L> __device__ __forceinline__ double fe_far_value(const double, const double, const double, fe_gdptr __restrict__);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, nds, koff, moff = [], [], [], [0], [0]
@@ -467,11 +661,12 @@ def main():
             allmono.append((*order[j], parent[j], axis[j]))
         koff.append(koff[-1] + len(keys))
         moff.append(moff[-1] + len(order))
-    out.append("// fe_fill_dcoef<L> for a class known at run time")
-    out.append("__host__ __device__ inline void fe_fill_dcoef_any(const int L, double* g) {")
+    out.append("// the derived parts of a record of a class known at run time: the integer")
+    out.append("// multiples c g_k (fe_fill_dcoef<L>) and the far-field block (fe_fill_kelvin<L>)")
+    out.append("__host__ __device__ inline void fe_fill_record_any(const int L, double* r) {")
     out.append("  switch (L) {")
     for L in range(lmax + 1):
-        out.append(f"    case {L}: fe_fill_dcoef<{L}>(g); break;")
+        out.append(f"    case {L}: fe_fill_dcoef<{L}>(r + 8); fe_fill_kelvin<{L}>(r); break;")
     out.append("    default: break;")
     out.append("  }")
     out.append("}")
@@ -479,8 +674,14 @@ def main():
     host.append(f"static const int NTERM[{lmax + 1}] = {{{', '.join(map(str, nts))}}};")
     host.append(f"static const int NMONO[{lmax + 1}] = {{{', '.join(map(str, nms))}}};")
     host.append(f"static const int NDCOEF[{lmax + 1}] = {{{', '.join(map(str, nds))}}};")
+    host.append("// record stride; the candidate records of the table build stop before the")
+    host.append("// far-field block (CSTRIDE = its offset)")
     host.append(f"static const int STRIDE[{lmax + 1}] = "
+                f"{{{', '.join(str(record_stride(L)) for L in range(lmax + 1))}}};")
+    host.append(f"static const int CSTRIDE[{lmax + 1}] = "
                 f"{{{', '.join(str(stride_of(n + d)) for n, d in zip(nts, nds))}}};")
+    host.append(f"static const int NKEL[{lmax + 1}] = "
+                f"{{{', '.join(str(nkel_of(L)) for L in range(lmax + 1))}}};")
     host.append(f"static const int KEY_OFF[{lmax + 2}] = {{{', '.join(map(str, koff))}}};")
     host.append(f"static const int MONO_OFF[{lmax + 2}] = {{{', '.join(map(str, moff))}}};")
     host.append("// ax, ay, az, m, aidx")

@@ -10,8 +10,8 @@ Backward (field + dL/dG), per (point, pair):
 Mixed precision: (nmono - 1) + 3 nterm of the forward (and of the backward
 contraction) are FP32, the rest FP64.
 
-Bytes per pair record of the device table: 8 * STRIDE(L) (header, g and the
-integer multiples c g the field contraction reads).
+Bytes per pair record of the device table: 8 * STRIDE(L) (header, g, the
+integer multiples c g the field contraction reads and the far-field block).
 
 Performed work (``performed_flops``): the model above prices every (point,
 pair) as a tabulated-Boys evaluation with a term-by-term contraction.  The
@@ -32,7 +32,7 @@ from pathlib import Path
 
 NTERM = [1, 4, 11, 24, 46, 80, 130]
 NMONO = [1, 4, 10, 20, 35, 56, 84]
-STRIDE = [16, 16, 24, 48, 56, 88, 144]
+STRIDE = [24, 24, 56, 112, 176, 288, 456]
 
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_PACKED = 157.3e12  # MI355X FP32 vector with packed (2-wide) FMAs, FLOP/s
@@ -103,6 +103,7 @@ def table_bytes(pairs_per_L) -> int:
 # ---------------------------------------------------------------------------
 _INC = Path(__file__).resolve().parent / "csrc" / "fe_lclass.inc"
 GRAD_ACC_LMAX = 2   # classes whose value + field contraction folds the weight in (fe_internal.h)
+KELVIN_LMIN = 0     # classes whose far value + field wavefronts use the far-field block (fe_internal.h)
 DG_SPLIT = [True, True, True, True, True, False, True]  # dL/dG with per-order scaling (fe_adjoint.h)
 
 
@@ -144,6 +145,16 @@ def boys_far(M: int) -> int:
 def f_eval_path(L: int, grad: bool, far: bool) -> int:
     """Performed FLOPs of one (point, pair) evaluation of class L on one path."""
     M = L + 1 if grad else L
+    if not grad and far and L >= KELVIN_LMIN:
+        # |u| H(u): 1/|R| and its square (1), u (3), the product and the
+        # accumulation (2); class 0: H = h_0
+        return R_T + RSQRT + (2 if L == 0 else generated_flops("fe_far_value", L) + 6)
+    if grad and far and L >= KELVIN_LMIN:
+        # |u| H(u) and |u|^3 (grad H - (H + 2D) R): 1/|R| and its square and
+        # cube (2), u (3), t = H + 2D (2), the four accumulations (2 + 3 x 4);
+        # class 0: H = h_0 wave-uniform, -H |R|^-3 (1) and three fma
+        c = 11 if L == 0 else generated_flops("fe_far_grad", L) + 21
+        return R_T + RSQRT + c
     b = boys_far(M) if far else boys_near(M)
     if grad:
         if L <= GRAD_ACC_LMAX:

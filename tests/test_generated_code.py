@@ -113,3 +113,133 @@ def test_record_strides_match_the_flop_model(src):
     from fastesp_amd import flops
     m = re.search(r"static const int STRIDE\[\d+\] = \{([^}]*)\}", src)
     assert [int(x) for x in m.group(1).split(",")] == flops.STRIDE
+
+
+def _gen():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen", INC.with_name("gen_lclass.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    return gen
+
+
+def _fill_record(src, L, g, p, K):
+    """A record of class L as the device holds it: header, g_k, the c g_k
+    (fe_fill_dcoef) and the far-field block (fe_fill_kelvin), the generated
+    bodies run on the CPU over a NumPy record."""
+    stride = int(re.search(rf"struct FeL<{L}> {{[^}}]*STRIDE = (\d+)", src).group(1))
+    r = np.zeros(stride)
+    r[3], r[4], r[6], r[7] = p, K, 0.5 / p, K * np.sqrt(np.pi / p) / 2
+    r[8:8 + len(g)] = g
+    run(body(src, "fe_fill_dcoef", L), dict(g=r[8:]))
+    env = dict(r=r)
+    for s in body(src, "fe_fill_kelvin", L):
+        m = re.fullmatch(r"(?:double\* )?(\w+) = r \+ (\d+)", s.replace("const ", ""))
+        if m:  # pointer into the record: a NumPy view, so writes land in r
+            env[m.group(1)] = r[int(m.group(2)):]
+        else:
+            env = run([s], env)
+    return r
+
+
+def _far_keyed(keys, g, p, K, R):
+    """The keyed far form: sum_k g_k K F_m(T) R^a with K F_m = S_m |R|^-(2m+1),
+    S_m = K sqrt(pi/p)/2 (2m-1)!! (2p)^-m, and its gradient in R."""
+    from math import prod
+    r = np.sqrt(R @ R)
+    I, G = 0.0, np.zeros(3)
+    for gk, (ax, ay, az, m) in zip(g, keys):
+        S = K * np.sqrt(np.pi / p) / 2 * prod(range(2 * m - 1, 0, -2)) * (2 * p) ** -m
+        a = np.array([ax, ay, az])
+        mono = np.prod(R ** a)
+        I += gk * S * mono * r ** -(2 * m + 1)
+        for d in range(3):
+            if a[d]:
+                lo = a.copy()
+                lo[d] -= 1
+                G[d] += gk * S * a[d] * np.prod(R ** lo) * r ** -(2 * m + 1)
+            G[d] -= gk * S * (2 * m + 1) * mono * R[d] * r ** -(2 * m + 3)
+    return I, G
+
+
+@pytest.mark.parametrize("L", range(7))
+def test_far_field_block_equals_the_keyed_far_form(src, L):
+    """fe_fill_kelvin + fe_far_grad (as the evaluation tile combines them)
+    against the keyed asymptotic form, term by term, at points of several
+    distances and directions."""
+    keys = keys_of(src, L)
+    rng = np.random.default_rng(70 + L)
+    g = rng.normal(size=len(keys))
+    p, K = 0.83, 1.7
+    r = _fill_record(src, L, g, p, K)
+    koff = int(re.search(rf"struct FeL<{L}> {{[^}}]*KOFF = (\d+)", src).group(1))
+    kel = r[koff:]
+    for _ in range(5):
+        R = rng.normal(size=3) * rng.uniform(2.0, 9.0)
+        want_I, want_G = _far_keyed(keys, g, p, K, R)
+        ri = 1.0 / np.sqrt(R @ R)
+        ri2, ri3 = ri * ri, ri ** 3
+        if L == 0:
+            H, Hg, D = kel[0], np.zeros(3), 0.0
+        else:
+            e = run(body(src, "fe_far_grad", L),
+                    dict(ux=R[0] * ri2, uy=R[1] * ri2, uz=R[2] * ri2, kel=kel))
+            H, Hg, D = e["H"], np.array([e["Hx"], e["Hy"], e["Hz"]]), e["D"]
+            assert D == pytest.approx(Hg @ (R * ri2), rel=1e-13, abs=1e-300)
+        got_I = ri * H
+        got_G = ri3 * (Hg - (H + 2 * D) * R)
+        if L > 0:  # the value-only form
+            Hv = run(body(src, "fe_far_value", L),
+                     dict(ux=R[0] * ri2, uy=R[1] * ri2, uz=R[2] * ri2, kel=kel))
+            assert abs(ri * Hv - want_I) <= 1e-13 * (np.abs(want_I) + np.abs(want_G).max())
+        scale = np.abs(want_I) + np.abs(want_G).max()
+        assert abs(got_I - want_I) <= 1e-13 * scale
+        np.testing.assert_allclose(got_G, want_G, rtol=0, atol=1e-13 * scale)
+
+
+@pytest.mark.parametrize("L", range(1, 7))
+def test_far_field_polynomial_is_harmonic_by_degree(L):
+    """For coefficients g that come from Hermite expansion coefficients (the
+    table's G = (2 pi/p) (2p)^m M0 etab), the far field is the potential of
+    point multipoles at P, so each homogeneous part H_n of the Kelvin
+    polynomial is harmonic.  A wrong multinomial factor, order factor S_m or a
+    dropped key breaks this (checked on the coefficient map of kelvin_of)."""
+    from math import prod
+    gen = _gen()
+    keys, acc = gen.keys_of(L)
+    h = gen.kelvin_of(L)
+    rng = np.random.default_rng(90 + L)
+    T = L + 1
+    p, K = 0.61, 1.0
+    etab = {(tx, ty, tz): rng.normal() for tx in range(T) for ty in range(T) for tz in range(T)
+            if tx + ty + tz <= L}
+    g = []
+    for (ax, ay, az, m) in keys:
+        s = sum(v * etab[(col // (T * T), (col // T) % T, col % T)] for col, v in acc[(ax, ay, az, m)].items())
+        g.append(2 * np.pi / p * (2 * p) ** m * s)
+    S = [K * np.sqrt(np.pi / p) / 2 * prod(range(2 * m - 1, 0, -2)) * (2 * p) ** -m
+         for m in range(L + 2)]
+    coef = {b: sum(c * g[k] * S[m] for (k, c, m) in lst) for b, lst in h.items()}
+    for n in range(L + 1):
+        lap = {}
+        for b, v in coef.items():
+            if sum(b) != n:
+                continue
+            for d in range(3):
+                if b[d] >= 2:
+                    lo = list(b)
+                    lo[d] -= 2
+                    lap[tuple(lo)] = lap.get(tuple(lo), 0.0) + b[d] * (b[d] - 1) * v
+        big = max(abs(v) for b, v in coef.items() if sum(b) == n)
+        assert all(abs(v) <= 1e-12 * big for v in lap.values()), (n, lap)
+    # and arbitrary g (not from a Hermite expansion) is not harmonic: the test has teeth
+    g2 = rng.normal(size=len(keys))
+    coef2 = {b: sum(c * g2[k] * S[m] for (k, c, m) in lst) for b, lst in h.items()}
+    lap2 = {}
+    for b, v in coef2.items():
+        if sum(b) == 2:
+            for d in range(3):
+                if b[d] >= 2:
+                    lap2[0] = lap2.get(0, 0.0) + b[d] * (b[d] - 1) * v
+    if L >= 2:
+        assert abs(lap2[0]) > 1e-6
