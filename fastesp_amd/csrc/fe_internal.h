@@ -39,9 +39,20 @@
 #ifndef FE_KELVIN_LMIN
 #define FE_KELVIN_LMIN 0
 #endif
+// Highest class whose far and near evaluation paths hand the factors of their
+// accumulation to one shared multiply-add (fewest operations); the classes
+// above it form the products inside each path (fewest live registers).
+#ifndef FE_TAIL_LMAX
+#define FE_TAIL_LMAX 4
+#endif
+// The same for one point per lane.
+#ifndef FE_EVAL_MINW1_L
+#define FE_EVAL_MINW1_L 1, 1, 1, 1, 1, 3, 3
+#endif
 __host__ __device__ constexpr int fe_eval_minw(int L, int ppl) {
   constexpr int t[7] = {FE_EVAL_MINW_L};
-  return ppl == 2 ? t[L] : 1;  // the table is for 2 points per lane
+  constexpr int t1[7] = {FE_EVAL_MINW1_L};
+  return ppl == 2 ? t[L] : ppl == 1 ? t1[L] : 1;
 }
 __host__ __device__ constexpr int fe_dg_minw(int L) {
   constexpr int t[7] = {FE_DG_MINW_L};

@@ -48,7 +48,7 @@ __device__ __forceinline__ bool fe_pair_geometry(const fe_gdptr r, const double*
   }
   fp64 = true;
   if constexpr (MIXED) fp64 = __any(big);
-  return __builtin_amdgcn_readfirstlane(!__any(near));
+  return __builtin_amdgcn_ballot_w64(near != 0) == 0;
 }
 
 // The points of one record by its Boys vector (tabulated, or the per-lane
@@ -107,41 +107,78 @@ __device__ __forceinline__ void fe_keyed_points(const fe_gdptr r, const fe_gfptr
   }
 }
 
-// The points of one record whose every Boys order is asymptotic on every lane:
-// the pair's integral is |u| H(u), u = R/|R|^2, from the record's far-field
-// block, and (GRAD) its gradient |u|^3 (grad H - (H + 2 u.grad H) R).  FP64 in
-// every mode.
+// The points of one record as the factors of their accumulation,
+//   value += A B,   field += C (Dx, Dy, Dz),
+// so that the far and the near paths below end in one shared accumulation
+// (with each path adding into the running sums itself, the compiler kept the
+// sums in other registers in the two paths and copied them at every join).
+//
+// Far (every Boys order asymptotic on every lane): the pair's integral is
+// |u| H(u), u = R/|R|^2, from the record's far-field block, and its gradient
+// |u|^3 (grad H - (H + 2 u.grad H) R).  FP64 in every mode.
 template <int L, int PPL, bool GRAD>
-__device__ __forceinline__ void fe_far_points(const fe_gdptr r, const double* Rx,
-                                              const double* Ry, const double* Rz,
-                                              const double* r2, double* av, double* ax,
-                                              double* ay, double* az) {
+__device__ __forceinline__ void fe_far_terms(const fe_gdptr r, const double* Rx,
+                                             const double* Ry, const double* Rz,
+                                             const double* r2, double* A, double* B, double* C,
+                                             double* Dx, double* Dy, double* Dz) {
   const fe_gdptr kel = r + FeL<L>::KOFF;
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     const double ri = fe_rsqrt_pos(r2[q]);
-    if constexpr (!GRAD && L == 0) {
-      av[q] = fma(ri, kel[0], av[q]);
-    } else if constexpr (!GRAD) {
-      const double ri2 = ri * ri;
-      av[q] = fma(ri, fe_far_value<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel), av[q]);
+    const double ri2 = ri * ri;
+    A[q] = ri;
+    if constexpr (!GRAD) {
+      if constexpr (L == 0)
+        B[q] = kel[0];
+      else
+        B[q] = fe_far_value<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel);
     } else if constexpr (L == 0) {
-      const double ri3 = ri * (ri * ri);
-      const double H = kel[0];
-      av[q] = fma(ri, H, av[q]);
-      const double c = -H * ri3;
-      ax[q] = fma(c, Rx[q], ax[q]);
-      ay[q] = fma(c, Ry[q], ay[q]);
-      az[q] = fma(c, Rz[q], az[q]);
+      B[q] = kel[0];
+      C[q] = -kel[0] * (ri * ri2);
+      Dx[q] = Rx[q];
+      Dy[q] = Ry[q];
+      Dz[q] = Rz[q];
     } else {
-      const double ri2 = ri * ri, ri3 = ri * ri2;
       double H, Hx, Hy, Hz, D;
       fe_far_grad<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel, H, Hx, Hy, Hz, D);
       const double t = fma(2.0, D, H);
-      av[q] = fma(ri, H, av[q]);
-      ax[q] = fma(ri3, fma(-t, Rx[q], Hx), ax[q]);
-      ay[q] = fma(ri3, fma(-t, Ry[q], Hy), ay[q]);
-      az[q] = fma(ri3, fma(-t, Rz[q], Hz), az[q]);
+      B[q] = H;
+      C[q] = ri * ri2;
+      Dx[q] = fma(-t, Rx[q], Hx);
+      Dy[q] = fma(-t, Ry[q], Hy);
+      Dz[q] = fma(-t, Rz[q], Hz);
+    }
+  }
+}
+
+// Near (some lane short of the switch point): the Boys vector per lane
+// (tabulated, or the asymptotic form for the lanes past it) and the keyed
+// contraction, FP64 or (MIXED, fp64 false) FP32; A = C = K.
+template <int L, int PPL, bool GRAD, bool MIXED>
+__device__ __forceinline__ void fe_near_terms(const fe_gdptr r, const fe_gfptr g, const bool fp64,
+                                              const double* Rx, const double* Ry,
+                                              const double* Rz, const double* r2,
+                                              const double* s_tab, double* A, double* B,
+                                              double* C, double* Dx, double* Dy, double* Dz) {
+  constexpr int M = GRAD ? L + 1 : L;
+  const double p = r[3], K = r[4];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    double F[M + 1];
+    fe_boys<M>(p * r2[q], F, s_tab);
+    A[q] = K;
+    if constexpr (GRAD) {
+      C[q] = K;
+      if (!MIXED || fp64)
+        fe_contract_grad<L>(Rx[q], Ry[q], Rz[q], F, r + 8, p, B[q], Dx[q], Dy[q], Dz[q]);
+      else
+        fe_contract_grad_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g, p, Rx[q], Ry[q],
+                                Rz[q], B[q], Dx[q], Dy[q], Dz[q]);
+    } else {
+      if (!MIXED || fp64)
+        B[q] = fe_contract<L>(Rx[q], Ry[q], Rz[q], F, r + 8);
+      else
+        B[q] = fe_contract_f32<L>((float)Rx[q], (float)Ry[q], (float)Rz[q], F, g);
     }
   }
 }
@@ -157,32 +194,64 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
   double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
   bool fp64;
   if constexpr (L >= FE_KELVIN_LMIN) {
-    // Each path forms its contributions from zero and one accumulation after
-    // the join adds them (with each path adding into the running sums itself,
-    // the compiler kept them in other registers in the two paths and copied
-    // them at every join).
     for (int ip = p0; ip < p1; ++ip) {
       const fe_gdptr r = rec + (size_t)ip * S;
-      double dv[PPL], dc[PPL], dx[PPL], dy[PPL], dz[PPL];
+      double A[PPL], B[PPL], C[PPL], Dx[PPL], Dy[PPL], Dz[PPL];
+      const bool far =
+          fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64);
+      if constexpr (L > FE_TAIL_LMAX) {
+        // the heavy classes: each path adds its products into contributions
+        // that start from zero, so only those four are live at the join
+        // (fewer registers than the six factors), at one more operation per sum
+        double dv[PPL], dc[PPL], dx[PPL], dy[PPL], dz[PPL];
 #pragma unroll
-      for (int q = 0; q < PPL; ++q) dv[q] = dc[q] = dx[q] = dy[q] = dz[q] = 0.0;
-      if (fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64))
-        fe_far_points<L, PPL, GRAD>(r, Rx, Ry, Rz, r2, dv, dx, dy, dz);
+        for (int q = 0; q < PPL; ++q) dv[q] = dc[q] = dx[q] = dy[q] = dz[q] = 0.0;
+        if (far) {
+          fe_far_terms<L, PPL, GRAD>(r, Rx, Ry, Rz, r2, A, B, C, Dx, Dy, Dz);
+#pragma unroll
+          for (int q = 0; q < PPL; ++q) {
+            dv[q] = A[q] * B[q];
+            if constexpr (GRAD) {
+              dx[q] = C[q] * Dx[q];
+              dy[q] = C[q] * Dy[q];
+              dz[q] = C[q] * Dz[q];
+            }
+          }
+        } else {
+          fe_keyed_points<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, false, fp64, Rx, Ry,
+                                               Rz, r2, s_tab, dv, dc, dx, dy, dz);
+        }
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) {
+          if constexpr (MIXED) {
+            fe_kahan_add(av[q], cv[q], dv[q]);
+            cv[q] += dc[q];
+          } else {
+            av[q] += dv[q];
+          }
+          if constexpr (GRAD) {
+            ax[q] += dx[q];
+            ay[q] += dy[q];
+            az[q] += dz[q];
+          }
+        }
+        continue;
+      }
+      if (far)
+        fe_far_terms<L, PPL, GRAD>(r, Rx, Ry, Rz, r2, A, B, C, Dx, Dy, Dz);
       else
-        fe_keyed_points<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, false, fp64, Rx, Ry,
-                                             Rz, r2, s_tab, dv, dc, dx, dy, dz);
+        fe_near_terms<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, fp64, Rx, Ry, Rz, r2,
+                                           s_tab, A, B, C, Dx, Dy, Dz);
 #pragma unroll
       for (int q = 0; q < PPL; ++q) {
-        if constexpr (MIXED) {
-          fe_kahan_add(av[q], cv[q], dv[q]);
-          cv[q] += dc[q];
-        } else {
-          av[q] += dv[q];
-        }
+        if constexpr (MIXED)
+          fe_kahan_add(av[q], cv[q], A[q] * B[q]);
+        else
+          av[q] = fma(A[q], B[q], av[q]);
         if constexpr (GRAD) {
-          ax[q] += dx[q];
-          ay[q] += dy[q];
-          az[q] += dz[q];
+          ax[q] = fma(C[q], Dx[q], ax[q]);
+          ay[q] = fma(C[q], Dy[q], ay[q]);
+          az[q] = fma(C[q], Dz[q], az[q]);
         }
       }
     }

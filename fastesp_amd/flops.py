@@ -156,6 +156,10 @@ def f_eval_path(L: int, grad: bool, far: bool) -> int:
         c = 11 if L == 0 else generated_flops("fe_far_grad", L) + 21
         return R_T + RSQRT + c
     b = boys_far(M) if far else boys_near(M)
+    if grad and L >= KELVIN_LMIN:
+        # tabulated wavefronts of the classes with a far-field block: the
+        # keyed contraction and the shared weighted accumulation (4 fma)
+        return R_T + b + generated_flops("fe_contract_grad", L) + 8
     if grad:
         if L <= GRAD_ACC_LMAX:
             c = generated_flops("fe_contract_grad_acc", L)
