@@ -136,13 +136,21 @@ def launch_flops(counts: dict, grad: bool) -> dict:
 
 
 def profile_values(path: Path, kernel: str) -> dict:
-    """key = value lines of a committed rocprofv3 summary, if it covers `kernel`."""
+    """The key = value lines of a committed rocprofv3 summary that follow its
+    `kernel = <kernel>` line (a summary holds one such section per launch it
+    covers), or {} if it does not cover `kernel`."""
     if not path.exists():
         return {}
-    kv = dict((k.strip(), v.strip()) for k, v in
-              (line.split("=", 1) for line in path.read_text().splitlines()
-               if "=" in line and not line.startswith(("#", "`", " ", "|"))))
-    return kv if kv.get("kernel") == kernel else {}
+    sections, cur = {}, None
+    for line in path.read_text().splitlines():
+        if "=" not in line or line.startswith(("#", "`", " ", "|")):
+            continue
+        k, v = (s.strip() for s in line.split("=", 1))
+        if k == "kernel":
+            cur = sections.setdefault(v, {"kernel": v})
+        elif cur is not None:
+            cur[k] = v
+    return sections.get(kernel, {})
 
 
 def roofline(phases, counts, grad, ppl_L, fp32_L, prec, pairs_per_L, n, traffic_file,

@@ -34,8 +34,9 @@ __device__ __forceinline__ bool fe_pair_geometry(const fe_gdptr r, const double*
                                                  double* Rz, double* r2, bool& fp64) {
   constexpr int M = GRAD ? L + 1 : L;
   const double Px = r[0], Py = r[1], Pz = r[2];
-  // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p (one product per pair)
-  const double r2far = fe_tfar(M) * (r[6] + r[6]);
+  // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p, the threshold stored in
+  // the record's far-field block
+  const double r2far = r[FeL<L>::KOFF + (GRAD ? 1 : 0)];
   int near = 0, big = 0;
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
@@ -121,23 +122,25 @@ __device__ __forceinline__ void fe_far_terms(const fe_gdptr r, const double* Rx,
                                              const double* Ry, const double* Rz,
                                              const double* r2, double* A, double* B, double* C,
                                              double* Dx, double* Dy, double* Dz) {
-  const fe_gdptr kel = r + FeL<L>::KOFF;
+  const fe_gdptr kel = r + FeL<L>::KCOEF;
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     const double ri = fe_rsqrt_pos(r2[q]);
     const double ri2 = ri * ri;
     A[q] = ri;
-    if constexpr (!GRAD) {
-      if constexpr (L == 0)
-        B[q] = kel[0];
-      else
-        B[q] = fe_far_value<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel);
-    } else if constexpr (L == 0) {
-      B[q] = kel[0];
-      C[q] = -kel[0] * (ri * ri2);
-      Dx[q] = Rx[q];
-      Dy[q] = Ry[q];
-      Dz[q] = Rz[q];
+    if constexpr (L == 0) {
+      // class 0 (see fe_eval_tile): value A = h_0/|R|, field C R with
+      // C = -h_0/|R|^3
+      A[q] = kel[0] * ri;
+      B[q] = 1.0;
+      if constexpr (GRAD) {
+        C[q] = -kel[0] * (ri * ri2);
+        Dx[q] = Rx[q];
+        Dy[q] = Ry[q];
+        Dz[q] = Rz[q];
+      }
+    } else if constexpr (!GRAD) {
+      B[q] = fe_far_value<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel);
     } else {
       double H, Hx, Hy, Hz, D;
       fe_far_grad<L>(Rx[q] * ri2, Ry[q] * ri2, Rz[q] * ri2, kel, H, Hx, Hy, Hz, D);
@@ -162,6 +165,20 @@ __device__ __forceinline__ void fe_near_terms(const fe_gdptr r, const fe_gfptr g
                                               double* C, double* Dx, double* Dy, double* Dz) {
   constexpr int M = GRAD ? L + 1 : L;
   const double p = r[3], K = r[4];
+  if constexpr (L == 0 && !MIXED) {
+    // class 0: I = g_0 F_0 and grad I = -2p g_0 F_1 R, so value A = K g_0 F_0
+    // and field C R with C = -2p K g_0 F_1
+    const double kg = K * r[8];
+    const double kg2 = -2.0 * p * kg;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      double F[M + 1];
+      fe_boys<M>(p * r2[q], F, s_tab);
+      A[q] = kg * F[0];
+      if constexpr (GRAD) C[q] = kg2 * F[1];
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
     double F[M + 1];
@@ -244,6 +261,16 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
                                            s_tab, A, B, C, Dx, Dy, Dz);
 #pragma unroll
       for (int q = 0; q < PPL; ++q) {
+        if constexpr (L == 0 && !MIXED) {
+          // class 0: the value is A itself and the field C R in both paths
+          av[q] += A[q];
+          if constexpr (GRAD) {
+            ax[q] = fma(C[q], Rx[q], ax[q]);
+            ay[q] = fma(C[q], Ry[q], ay[q]);
+            az[q] = fma(C[q], Rz[q], az[q]);
+          }
+          continue;
+        }
         if constexpr (MIXED)
           fe_kahan_add(av[q], cv[q], A[q] * B[q]);
         else

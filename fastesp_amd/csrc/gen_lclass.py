@@ -156,14 +156,16 @@ def kelvin_top(L):
 
 
 def nkel_of(L):
-    """Length of the record's far-field block: h_0 for L = 0; else, over the
-    monomials of degree <= L - 1, the coefficients of Q = H - (u . grad H)/L,
-    of the three components of grad H and of H itself, then H's degree-L
-    coefficients (kelvin_top order)."""
+    """Length of the record's far-field block: the far-field thresholds on
+    |R|^2 of the value and of the value + field evaluation, T_far(L)/p and
+    T_far(L+1)/p; then h_0 for L = 0; else, over the monomials of degree
+    <= L - 1, the coefficients of Q = H - (u . grad H)/L, of the three
+    components of grad H and of H itself, then H's degree-L coefficients
+    (kelvin_top order)."""
     if L == 0:
-        return 1
+        return 3
     n1 = len(kelvin_monos(L)[0])
-    return 5 * n1 + (L + 1) * (L + 2) // 2
+    return 2 + 5 * n1 + (L + 1) * (L + 2) // 2
 
 
 def record_stride(L):
@@ -186,7 +188,11 @@ def gen_kelvin(L, keys, koff):
     o.append(f"template <> __host__ __device__ inline void fe_fill_kelvin<{L}>(double* r) {{")
     o.append("  const double hp = r[6];")
     o.append("  const double* g = r + 8;")
-    o.append(f"  double* kel = r + {koff};")
+    # the evaluation tile's far test r^2 >= T_far(M)/p, formed as it was
+    # formed per pair before (same operations, same bits)
+    o.append(f"  r[{koff}] = fe_tfar({L}) * (hp + hp);")
+    o.append(f"  r[{koff + 1}] = fe_tfar({L + 1}) * (hp + hp);")
+    o.append(f"  double* kel = r + {koff + 2};")
     # s_m = K sqrt(pi/p)/2 (2m-1)!! (2p)^-m = s_{m-1} (2m-1) hp
     o.append("  const double s0 = r[7];")
     for m in range(1, max(ms) + 1):
@@ -403,7 +409,8 @@ def gen_L(L):
     o.append(f"// ---- class L = {L}: nterm = {nt}, nmono = {nm}, scaled coefficients = {len(dc)}, "
              f"far-field coefficients = {nk}")
     o.append(f"template <> struct FeL<{L}> {{ static constexpr int NT = {nt}, NM = {nm}, "
-             f"ND = {len(dc)}, KOFF = {koff}, NK = {nk}, STRIDE = {record_stride(L)}; }};")
+             f"ND = {len(dc)}, KOFF = {koff}, KCOEF = {koff + 2}, NK = {nk}, "
+             f"STRIDE = {record_stride(L)}; }};")
     o.append(f"template <> __host__ __device__ inline void fe_fill_dcoef<{L}>(double* g) {{")
     for j, (k, c) in enumerate(dc):
         o.append(f"  g[{nt + j}] = {c}.0 * g[{k}];")

@@ -32,7 +32,7 @@ from pathlib import Path
 
 NTERM = [1, 4, 11, 24, 46, 80, 130]
 NMONO = [1, 4, 10, 20, 35, 56, 84]
-STRIDE = [24, 24, 56, 112, 176, 288, 456]
+STRIDE = [24, 32, 56, 112, 176, 288, 456]
 
 PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
 PEAK_FP32_PACKED = 157.3e12  # MI355X FP32 vector with packed (2-wide) FMAs, FLOP/s

@@ -132,7 +132,8 @@ def _fill_record(src, L, g, p, K):
     r[3], r[4], r[6], r[7] = p, K, 0.5 / p, K * np.sqrt(np.pi / p) / 2
     r[8:8 + len(g)] = g
     run(body(src, "fe_fill_dcoef", L), dict(g=r[8:]))
-    env = dict(r=r)
+    tfar = [35.5, 39.5, 43.0, 46.0, 49.0, 51.5, 54.0, 56.5, 59.0]  # fe_boys.h fe_tfar
+    env = dict(r=r, fe_tfar=lambda M: tfar[M])
     for s in body(src, "fe_fill_kelvin", L):
         m = re.fullmatch(r"(?:double\* )?(\w+) = r \+ (\d+)", s.replace("const ", ""))
         if m:  # pointer into the record: a NumPy view, so writes land in r
@@ -173,7 +174,11 @@ def test_far_field_block_equals_the_keyed_far_form(src, L):
     p, K = 0.83, 1.7
     r = _fill_record(src, L, g, p, K)
     koff = int(re.search(rf"struct FeL<{L}> {{[^}}]*KOFF = (\d+)", src).group(1))
-    kel = r[koff:]
+    kcoef = int(re.search(rf"struct FeL<{L}> {{[^}}]*KCOEF = (\d+)", src).group(1))
+    # the far tests of the value and the value + field tiles
+    tfar = [35.5, 39.5, 43, 46, 49, 51.5, 54, 56.5, 59]
+    assert r[koff] == tfar[L] * (r[6] + r[6]) and r[koff + 1] == tfar[L + 1] * (r[6] + r[6])
+    kel = r[kcoef:]
     for _ in range(5):
         R = rng.normal(size=3) * rng.uniform(2.0, 9.0)
         want_I, want_G = _far_keyed(keys, g, p, K, R)
