@@ -45,6 +45,11 @@
 #ifndef FE_TAIL_LMAX
 #define FE_TAIL_LMAX 4
 #endif
+// Classes (bit L) whose evaluation loop reads the next record's header one
+// trip ahead.
+#ifndef FE_HDR_PREFETCH
+#define FE_HDR_PREFETCH 0x6f
+#endif
 // The same for one point per lane.
 #ifndef FE_EVAL_MINW1_L
 #define FE_EVAL_MINW1_L 1, 1, 1, 1, 1, 3, 3

@@ -27,16 +27,23 @@ __device__ __forceinline__ void fe_kahan_add(double& s, double& c, const double 
 // returns whether every lane is past the Boys switch point of the top order
 // (wave-uniform, read from a scalar so branches on it are scalar branches),
 // and under MIXED whether some lane needs the FP64 contraction.
+// The record values the geometry needs: P and the far threshold on |R|^2
+// (T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p, stored in the record's
+// far-field block).
+struct FeHdr {
+  double x, y, z, thr;
+};
+template <int L, bool GRAD>
+__device__ __forceinline__ FeHdr fe_hdr(const fe_gdptr r) {
+  return FeHdr{r[0], r[1], r[2], r[FeL<L>::KOFF + (GRAD ? 1 : 0)]};
+}
+
 template <int L, int PPL, bool GRAD, bool MIXED>
-__device__ __forceinline__ bool fe_pair_geometry(const fe_gdptr r, const double* cx,
-                                                 const double* cy, const double* cz,
-                                                 const float tau, double* Rx, double* Ry,
-                                                 double* Rz, double* r2, bool& fp64) {
-  constexpr int M = GRAD ? L + 1 : L;
-  const double Px = r[0], Py = r[1], Pz = r[2];
-  // T = p r^2 < T_far(M)  <=>  r^2 < T_far(M) / p, the threshold stored in
-  // the record's far-field block
-  const double r2far = r[FeL<L>::KOFF + (GRAD ? 1 : 0)];
+__device__ __forceinline__ bool fe_pair_geometry(const fe_gdptr r, const FeHdr h,
+                                                 const double* cx, const double* cy,
+                                                 const double* cz, const float tau, double* Rx,
+                                                 double* Ry, double* Rz, double* r2, bool& fp64) {
+  const double Px = h.x, Py = h.y, Pz = h.z, r2far = h.thr;
   int near = 0, big = 0;
 #pragma unroll
   for (int q = 0; q < PPL; ++q) {
@@ -211,11 +218,20 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
   double Rx[PPL], Ry[PPL], Rz[PPL], r2[PPL];
   bool fp64;
   if constexpr (L >= FE_KELVIN_LMIN) {
+    // the next record's header is read one trip ahead, so its scalar loads
+    // are in flight while this record is evaluated (classes in
+    // FE_HDR_PREFETCH; elsewhere the extra live SGPRs cost more spills than
+    // the latency it hides)
+    constexpr bool PF = (FE_HDR_PREFETCH >> L) & 1;
+    FeHdr h = fe_hdr<L, GRAD>(rec + (size_t)min(p0, max(p1 - 1, 0)) * S);
     for (int ip = p0; ip < p1; ++ip) {
       const fe_gdptr r = rec + (size_t)ip * S;
+      if constexpr (!PF) h = fe_hdr<L, GRAD>(r);
+      const FeHdr hn = PF ? fe_hdr<L, GRAD>(rec + (size_t)min(ip + 1, p1 - 1) * S) : h;
       double A[PPL], B[PPL], C[PPL], Dx[PPL], Dy[PPL], Dz[PPL];
       const bool far =
-          fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64);
+          fe_pair_geometry<L, PPL, GRAD, MIXED>(r, h, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64);
+      h = hn;
       if constexpr (L > FE_TAIL_LMAX) {
         // the heavy classes: each path adds its products into contributions
         // that start from zero, so only those four are live at the join
@@ -285,7 +301,8 @@ __device__ __forceinline__ void fe_eval_tile(fe_gdptr __restrict__ rec,
   } else {
     for (int ip = p0; ip < p1; ++ip) {
       const fe_gdptr r = rec + (size_t)ip * S;
-      const bool far = fe_pair_geometry<L, PPL, GRAD, MIXED>(r, cx, cy, cz, tau, Rx, Ry, Rz, r2, fp64);
+      const bool far = fe_pair_geometry<L, PPL, GRAD, MIXED>(r, fe_hdr<L, GRAD>(r), cx, cy, cz,
+                                                             tau, Rx, Ry, Rz, r2, fp64);
       fe_keyed_points<L, PPL, GRAD, MIXED>(r, g32 + (size_t)ip * S32, far, fp64, Rx, Ry, Rz, r2,
                                            s_tab, av, cv, ax, ay, az);
     }
