@@ -84,14 +84,30 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
   }
   FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg, c->split_eval);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
-  rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * plane * sizeof(double));
+  const int groups = fe_row_groups((int)n, pl.rows);
+  const size_t gplane = (size_t)groups * pl.ld;
+  rc = fe_ctx_reserve(c, FE_WS_PARTIAL, 4 * (plane + gplane) * sizeof(double));
   if (rc) return rc;
   double* partial = (double*)c->ws[FE_WS_PARTIAL];
   rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, plane, true, precision == FE_PREC_MIXED, st);
   if (rc) return rc;
   const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
+  const double* red = partial;
+  int nrows = pl.rows;
+  size_t rplane = plane;
+  if (groups) {
+    // few points: the rows are first added in groups over a 2-D grid
+    double* staged = partial + 4 * plane;
+    const int rpg = (pl.rows + groups - 1) / groups;
+    hipLaunchKernelGGL(k_rows_stage<4>, dim3((unsigned)((n + 63) / 64), (unsigned)groups),
+                       dim3(256), 0, st, partial, pl.rows, pl.ld, plane, (int)n, rpg, staged,
+                       gplane);
+    red = staged;
+    nrows = groups;
+    rplane = gplane;
+  }
   hipLaunchKernelGGL(k_esp_reduce_grad, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st,
-                     partial, pl.rows, pl.ld, plane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
+                     red, nrows, pl.ld, rplane, d_pts, (int)n, t->d_atoms, t->n_atoms, perm,
                      d_V, d_G);
   fe_prof_end(c, pr, st);
   FE_HIPCHK(hipGetLastError());

@@ -898,15 +898,25 @@ extern "C" int fe_esp_forward(fe_table* t, const double* pts, int64_t n, int pre
   }
   FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg, c->split_eval);
   double* partial = nullptr;
+  const int groups = (parts & FE_PART_ELECTRON) ? fe_row_groups((int)n, pl.rows) : 0;
   if ((parts & FE_PART_ELECTRON) && pl.rows > 0) {
-    rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)pl.rows * pl.ld * sizeof(double));
+    rc = fe_ctx_reserve(c, FE_WS_PARTIAL, (size_t)(pl.rows + groups) * pl.ld * sizeof(double));
     if (rc) return rc;
     partial = (double*)c->ws[FE_WS_PARTIAL];
     rc = fe_launch_eval(t, pl, d_pts, (int)n, partial, 0, false, precision == FE_PREC_MIXED, st);
     if (rc) return rc;
   }
-  const int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
+  int nrows = (parts & FE_PART_ELECTRON) ? pl.rows : 0;
   const int pr = fe_prof_begin(c, FE_PH_REDUCE, st);
+  if (groups) {
+    double* staged = partial + (size_t)pl.rows * pl.ld;
+    const int rpg = (pl.rows + groups - 1) / groups;
+    hipLaunchKernelGGL(k_rows_stage<1>, dim3((unsigned)((n + 63) / 64), (unsigned)groups),
+                       dim3(256), 0, st, partial, pl.rows, pl.ld, (size_t)0, (int)n, rpg, staged,
+                       (size_t)0);
+    partial = staged;
+    nrows = groups;
+  }
   hipLaunchKernelGGL(k_esp_reduce, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, partial,
                      nrows, pl.ld, d_pts, (int)n, t->d_atoms, t->n_atoms, parts, perm, d_V);
   fe_prof_end(c, pr, st);

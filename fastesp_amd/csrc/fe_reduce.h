@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 // sum_r p[r * ld], four chains
 __device__ __forceinline__ double fe_sum_rows(const double* __restrict__ p, const int n_rows,
                                               const size_t ld) {
@@ -90,6 +92,39 @@ __device__ __forceinline__ void fe_sum_rows_split(const double* __restrict__ p, 
     for (int q = 0; q < NP; ++q)
       out[q] = (s_red[q][0][lane] + s_red[q][1][lane]) + (s_red[q][2][lane] + s_red[q][3][lane]);
   }
+}
+
+// First stage of the reduction for few points and many partial rows (a short
+// grid of point blocks would leave most of the chip idle while each block
+// walks thousands of rows): block (b, g) adds rows [g rpg, (g+1) rpg) of the
+// 64 points of block b, split over its waves as above, and writes row g of
+// `out` (NP planes `oplane` apart).  The final stage then adds those rows in
+// order, so the result stays bitwise reproducible for a given launch shape.
+template <int NP>
+__global__ __launch_bounds__(256) void k_rows_stage(const double* __restrict__ partial,
+                                                    const int n_rows, const int ld,
+                                                    const size_t plane, const int n_pts,
+                                                    const int rpg, double* __restrict__ out,
+                                                    const size_t oplane) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int r0 = blockIdx.y * rpg;
+  const int nr = max(0, min(n_rows - r0, rpg));
+  double sum[NP];
+  fe_sum_rows_split<NP>(partial + (size_t)r0 * ld + i, i < n_pts, nr, ld, plane, sum);
+  if (threadIdx.x < 64 && i < n_pts) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) out[q * oplane + (size_t)blockIdx.y * ld + i] = sum[q];
+  }
+}
+
+// Row groups of the first stage for n points and n_rows partial rows (0: one
+// stage suffices): enough blocks to fill the chip, at least 16 rows a group.
+inline int fe_row_groups(const int n, const int n_rows) {
+  const int nb = (n + 63) / 64;
+  if (nb >= 128 || n_rows <= 64) return 0;
+  int g = std::min((n_rows + 15) / 16, std::max(1, 2048 / nb));
+  const int rpg = (n_rows + g - 1) / g;
+  return (n_rows + rpg - 1) / rpg;
 }
 
 // sum_A Z_A / |C - R_A|; atoms as (x, y, z, Z)
