@@ -22,6 +22,9 @@ struct DPArgs {
   const int* sp_i; const int* sp_j; const int* sp_slot;
   const int* sp_by_class;  // shell pairs grouped by class: [sp_cls_off[L], sp_cls_off[L+1])
   int sp_cls_off[FE_LMAX + 2];
+  const uint32_t* tasks;  // dL/dP tasks grouped by class: [task_cls_off[L], task_cls_off[L+1])
+  int task_cls_off[FE_LMAX + 2];
+  const double* det[FE_LMAX + 1];  // d etab per kept record (FeL<L>::NM each), fe_det_from_dg
   const int* slot2kept[FE_LMAX + 1];
   const double* dG[FE_LMAX + 1];
   int n_func;
@@ -241,6 +244,92 @@ __device__ __forceinline__ void fe_dg_to_dp_sp(const DPArgs& a, const int sp) {
   }
 }
 
+// 1-D Hermite coefficients E^{ab}_t, t = 0..L (zero above a + b), of
+// (u+xa)^a (u+xb)^b e^{-p u^2} for a, b <= 3, a + b <= L, by the recurrence of
+// fe_e1d_table raising a, then b; h = 1/(2p).  Compile-time bounds and
+// predicated steps keep e in registers.
+template <int L>
+__device__ __forceinline__ void fe_e1d_ab(const int a, const int b, const double xa,
+                                          const double xb, const double h, double* e) {
+  constexpr int LI = L < 3 ? L : 3;
+  e[0] = 1.0;
+#pragma unroll
+  for (int t = 1; t <= L; ++t) e[t] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 2 * LI; ++s) {
+    const bool on = s < LI ? s < a : s - LI < b;
+    const double x = s < LI ? xa : xb;
+    if (on) {
+      double n[L + 1];
+#pragma unroll
+      for (int t = 0; t <= L; ++t) {
+        double v = x * e[t];
+        if (t >= 1) v = fma(h, e[t - 1], v);
+        if (t + 1 <= L) v = fma((double)(t + 1), e[t + 1], v);
+        n[t] = v;
+      }
+#pragma unroll
+      for (int t = 0; t <= L; ++t) e[t] = n[t];
+    }
+  }
+}
+
+// One dL/dP task of class L: entry (mi, mj) of shell pair sp,
+//   sum over the kept primitive pairs of c_i c_j sum_t d etab_t Ex Ey Ez,
+// one lane each (no barriers: most shell pairs have one or two primitive
+// pairs, so the work is the latency of a few gathers).  Both triangles of P
+// receive it (the table folds P_ij + P_ji^T, fastesp/esp.py:185-187).
+template <int L>
+__device__ __forceinline__ void fe_dp_task(const DPArgs& a, const uint32_t task) {
+  constexpr int NM = FeL<L>::NM;
+  const int sp = (int)(task >> 8), mi = (int)(task >> 4) & 15, mj = (int)task & 15;
+  const int si = a.sp_i[sp], sj = a.sp_j[sp];
+  const int fi = a.sh_fn0[si] + mi, fj = a.sh_fn0[sj] + mj;
+  const int npi = a.sh_nprim[si], npj = a.sh_nprim[sj];
+  const int pi0 = a.sh_prim0[si], pj0 = a.sh_prim0[sj];
+  const int slot0 = a.sp_slot[sp];
+  const int ax = a.fn_lx[fi], ay = a.fn_ly[fi], az = a.fn_lz[fi];
+  const int bx = a.fn_lx[fj], by = a.fn_ly[fj], bz = a.fn_lz[fj];
+  const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
+  const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
+  const int* __restrict__ s2k = a.slot2kept[L];
+  const double* __restrict__ det = a.det[L];
+  double acc = 0.0;
+  for (int ii = 0; ii < npi; ++ii)
+    for (int jj = 0; jj < npj; ++jj) {
+      const int kept = s2k[slot0 + ii * npj + jj];
+      if (kept < 0) continue;
+      const double al = a.pexp[pi0 + ii], be = a.pexp[pj0 + jj];
+      const double p = al + be, ip = 1.0 / p, h = 0.5 * ip;
+      const double Px = (al * Ax + be * Bx) * ip;
+      const double Py = (al * Ay + be * By) * ip;
+      const double Pz = (al * Az + be * Bz) * ip;
+      double ex[L + 1], ey[L + 1], ez[L + 1];
+      fe_e1d_ab<L>(ax, bx, Px - Ax, Px - Bx, h, ex);
+      fe_e1d_ab<L>(ay, by, Py - Ay, Py - By, h, ey);
+      fe_e1d_ab<L>(az, bz, Pz - Az, Pz - Bz, h, ez);
+      const double* __restrict__ dt = det + (size_t)kept * NM;
+      // the |t| <= L simplex in its sorted (tx, ty, tz) order
+      double s = 0.0;
+      int q = 0;
+#pragma unroll
+      for (int tx = 0; tx <= L; ++tx) {
+        double sy = 0.0;
+#pragma unroll
+        for (int ty = 0; ty <= L - tx; ++ty) {
+          double sz = 0.0;
+#pragma unroll
+          for (int tz = 0; tz <= L - tx - ty; ++tz) sz = fma(ez[tz], dt[q++], sz);
+          sy = fma(ey[ty], sz, sy);
+        }
+        s = fma(ex[tx], sy, s);
+      }
+      acc = fma(a.pcoef[pi0 + ii] * a.pcoef[pj0 + jj], s, acc);
+    }
+  a.dP[(size_t)fi * a.n_func + fj] = acc;
+  if (si != sj) a.dP[(size_t)fj * a.n_func + fi] = acc;
+}
+
 // DPArgs of a table whose dG of class L starts at dG + dg_off[L]
 inline DPArgs fe_dp_args(const fe_table* t, double* dG, const size_t* dg_off, double* dP) {
   DPArgs pa{};
@@ -250,6 +339,8 @@ inline DPArgs fe_dp_args(const fe_table* t, double* dG, const size_t* dg_off, do
   pa.sp_i = t->b.sp_i; pa.sp_j = t->b.sp_j; pa.sp_slot = t->b.sp_slot;
   pa.sp_by_class = t->b.sp_by_class;
   for (int L = 0; L <= FE_LMAX + 1; ++L) pa.sp_cls_off[L] = t->sp_cls_off[L];
+  pa.tasks = t->b.tasks;
+  for (int L = 0; L <= FE_LMAX + 1; ++L) pa.task_cls_off[L] = t->task_cls_off[L];
   for (int L = 0; L <= FE_LMAX; ++L) {
     pa.slot2kept[L] = t->d_slot2kept[L];
     pa.dG[L] = dG + dg_off[L];

@@ -259,9 +259,37 @@ static void launch_dG_L(const fe_table* t, const DGPlan& d, const double* pw, in
 // ---------------------------------------------------------------------------
 // dL/dG -> dL/dP, one workgroup (64 lanes) per shell pair
 // ---------------------------------------------------------------------------
-// the shell pairs sp_list[0 .. gridDim.x) (one class: their dL/dG is ready)
-__global__ __launch_bounds__(64) void k_dg_to_dp(const DPArgs a, const int* __restrict__ sp_list) {
-  fe_dg_to_dp_sp(a, sp_list[blockIdx.x]);
+// d etab of every kept record of class L (one lane each) from its dL/dG
+template <int L>
+__global__ __launch_bounds__(256) void k_det(const double* __restrict__ rec,
+                                             const int* __restrict__ kept, const int cap,
+                                             const double* __restrict__ dG,
+                                             double* __restrict__ det) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= min(*kept, cap)) return;
+  fe_det_from_dg<L>(dG + (size_t)i * FeL<L>::NT, rec[(size_t)i * FeL<L>::STRIDE + 3],
+                    det + (size_t)i * FeL<L>::NM);
+}
+
+// the dL/dP tasks of class L (one lane each; their d etab is ready)
+template <int L>
+__global__ __launch_bounds__(256) void k_dp_tasks(const DPArgs a, const int n_tasks) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_tasks) return;
+  fe_dp_task<L>(a, a.tasks[a.task_cls_off[L] + i]);
+}
+
+template <int L>
+static void launch_dp_L(const fe_table* t, const DPArgs& pa, hipStream_t s) {
+  const int64_t nL = fe_plan_n(t, L);
+  const int nt = t->task_cls_off[L + 1] - t->task_cls_off[L];
+  if (nL > 0)
+    hipLaunchKernelGGL(k_det<L>, dim3((unsigned)((nL + 255) / 256)), dim3(256), 0, s,
+                       t->d_rec[L], t->d_stats + FE_ST_KEPT + L, (int)nL, pa.dG[L],
+                       (double*)pa.det[L]);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_dp_tasks<L>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, pa,
+                       nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -336,13 +364,25 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     if (rc) return rc;
     d_dP = (double*)c->ws[FE_WS_OUT];
   }
-  const DPArgs pa = dLdP ? fe_dp_args(t, dG, d.dg_off, d_dP) : DPArgs{};
+  DPArgs pa = dLdP ? fe_dp_args(t, dG, d.dg_off, d_dP) : DPArgs{};
+  // the d etab of class L go to its partial rows, which its dL/dG reduction
+  // has consumed by then (same stream); they hold at least NT >= NM values
+  // per record
+  for (int L = 0; L <= FE_LMAX; ++L) pa.det[L] = part + d.part_off[L];
   auto launch_dp = [&](int L, hipStream_t s) {
-    const int cnt = t->sp_cls_off[L + 1] - t->sp_cls_off[L];
-    if (!dLdP || cnt <= 0) return;
+    // classes without shell pairs have no entries of dL/dP
+    if (!dLdP || t->task_cls_off[L + 1] == t->task_cls_off[L]) return;
     const int pd = fe_prof_begin(c, FE_PH_DP, s);
-    hipLaunchKernelGGL(k_dg_to_dp, dim3((unsigned)cnt), dim3(64), 0, s, pa,
-                       t->b.sp_by_class + t->sp_cls_off[L]);
+    switch (L) {
+      case 0: launch_dp_L<0>(t, pa, s); break;
+      case 1: launch_dp_L<1>(t, pa, s); break;
+      case 2: launch_dp_L<2>(t, pa, s); break;
+      case 3: launch_dp_L<3>(t, pa, s); break;
+      case 4: launch_dp_L<4>(t, pa, s); break;
+      case 5: launch_dp_L<5>(t, pa, s); break;
+      case 6: launch_dp_L<6>(t, pa, s); break;
+      default: break;
+    }
     fe_prof_end(c, pd, s);
   };
   if (n == 0) {

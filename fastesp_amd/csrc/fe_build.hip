@@ -838,8 +838,9 @@ struct BasisSetup {
   size_t bytes = 0;
   size_t o_atom = 0, o_l = 0, o_fn0 = 0, o_nfn = 0, o_p0 = 0, o_np = 0, o_c = 0, o_pexp = 0,
          o_pcoef = 0, o_lx = 0, o_ly = 0, o_lz = 0, o_spi = 0, o_spj = 0, o_sps = 0, o_spg = 0,
-         o_spc = 0;
+         o_spc = 0, o_task = 0;
   int sp_cls_off[FE_LMAX + 2] = {};  // shell pairs of class L: sp_by_class[off[L], off[L+1])
+  int task_cls_off[FE_LMAX + 2] = {};  // dL/dP tasks of class L: tasks[off[L], off[L+1])
   ~BasisSetup() {
     if (mem) {
       (void)hipSetDevice(device);
@@ -946,6 +947,22 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
     for (int L = 0; L <= FE_LMAX + 1; ++L) bs->sp_cls_off[L] = cnt[L];
     for (size_t q = 0; q < nsp_all; ++q) sp_cls[cnt[sh[sp_i[q]].l + sh[sp_j[q]].l]++] = (int)q;
   }
+  // the dL/dP tasks: one per (shell pair, function of shell i, function of
+  // shell j), class by class in the order above, packed as sp << 8 | mi << 4 | mj
+  if (nsp_all >= (1u << 24))
+    return fe_set_error(FE_ERR_UNSUPPORTED, "too many shell pairs (%zu) for dL/dP", nsp_all);
+  std::vector<uint32_t> tasks;
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    bs->task_cls_off[L] = (int)tasks.size();
+    for (int q = bs->sp_cls_off[L]; q < bs->sp_cls_off[L + 1]; ++q) {
+      const int sp = sp_cls[q];
+      const int ni = sh[sp_i[sp]].nfn, nj = sh[sp_j[sp]].nfn;
+      for (int mi = 0; mi < ni; ++mi)
+        for (int mj = 0; mj < nj; ++mj)
+          tasks.push_back((uint32_t)sp << 8 | (uint32_t)mi << 4 | (uint32_t)mj);
+    }
+  }
+  bs->task_cls_off[FE_LMAX + 1] = (int)tasks.size();
   bs->device = c->device;
   bs->n_func = n_func;
   bs->n_prim = n_prim;
@@ -992,6 +1009,7 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
   bs->o_sps = push_bytes(hb, sp_slot.data(), nsp);
   bs->o_spg = push_bytes(hb, sp_gidx.data(), nsp);
   bs->o_spc = push_bytes(hb, sp_cls.data(), nsp);
+  bs->o_task = push_bytes(hb, tasks.data(), tasks.size());
   if (hipMalloc(&bs->mem, hb.size()) != hipSuccess) {
     (void)hipGetLastError();
     bs->mem = nullptr;
@@ -1074,7 +1092,9 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   t->b.sp_i = (const int*)(bm + bs->o_spi); t->b.sp_j = (const int*)(bm + bs->o_spj);
   t->b.sp_slot = (const int*)(bm + bs->o_sps); t->b.sp_gidx = (const int*)(bm + bs->o_spg);
   t->b.sp_by_class = (const int*)(bm + bs->o_spc);
+  t->b.tasks = (const uint32_t*)(bm + bs->o_task);
   for (int L = 0; L <= FE_LMAX + 1; ++L) t->sp_cls_off[L] = bs->sp_cls_off[L];
+  for (int L = 0; L <= FE_LMAX + 1; ++L) t->task_cls_off[L] = bs->task_cls_off[L];
   auto fail = [&](int rc) {
     (void)hipStreamSynchronize(st);
     fe_table_free(t);

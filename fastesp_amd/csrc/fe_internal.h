@@ -243,8 +243,10 @@ struct fe_table {
     const int* fn_lx; const int* fn_ly; const int* fn_lz;
     const int* sp_i; const int* sp_j; const int* sp_slot; const int* sp_gidx;
     const int* sp_by_class;  // shell-pair indices grouped by class L = l_i + l_j
+    const uint32_t* tasks;   // dL/dP tasks (sp << 8 | mi << 4 | mj) grouped by class
   } b{};
   int sp_cls_off[FE_LMAX + 2] = {};  // class L: sp_by_class[sp_cls_off[L], sp_cls_off[L+1])
+  int task_cls_off[FE_LMAX + 2] = {};  // class L: tasks[task_cls_off[L], task_cls_off[L+1])
   int64_t n_cand_L[FE_LMAX + 1] = {}; // candidate slots (every primitive pair) per class
   int64_t kvalid_L[FE_LMAX + 1] = {}; // K-valid slots per class
   int* d_slot2kept[FE_LMAX + 1] = {}; // candidate slot -> compact record (-1 dropped)

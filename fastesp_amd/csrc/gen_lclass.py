@@ -625,7 +625,36 @@ def gen_L(L):
                 e = f"fma((double)d{d}{m}, F[{m}], {e})"
         o.append(f"  G{d} = {e};")
     o.append("}")
+    o.extend(gen_det(L, keys, acc, order))
     return o, keys, order, parent, axis, aidx
+
+
+def gen_det(L, keys, acc, order):
+    """fe_det_from_dg<L>: the adjoint of G = (2 pi/p)(2p)^m M0 etab for one
+    record, d etab_t = (2 pi/p) sum_k (2p)^{m_k} M0[k, t] dL/dG_k over the
+    |t| <= L simplex in its sorted order (the order the etab columns use)."""
+    T = L + 1
+    o = [f"template <> __device__ __forceinline__ void fe_det_from_dg<{L}>("
+         "const double* __restrict__ g, const double p, double* __restrict__ det) {"]
+    ms = sorted({k[3] for k in keys})
+    o.append("  const double tp = p + p;")
+    o.append("  const double w0 = 6.283185307179586 / p;")
+    for m in range(1, max(ms) + 1):
+        o.append(f"  const double w{m} = w{m - 1} * tp;")
+    for ti, (tx, ty, tz) in enumerate(order):
+        col = (tx * T + ty) * T + tz
+        by_m = {}
+        for k, key in enumerate(keys):
+            v = acc[key].get(col, 0.0)
+            if v != 0.0:
+                by_m.setdefault(key[3], []).append((k, v))
+        parts = []
+        for m, lst in sorted(by_m.items()):
+            e = " + ".join((f"g[{k}]" if v == 1.0 else f"{v!r} * g[{k}]") for k, v in lst)
+            parts.append(f"w{m} * ({e})")
+        o.append(f"  det[{ti}] = " + (" + ".join(parts) if parts else "0.0") + ";")
+    o.append("}")
+    return o
 
 
 def main():
@@ -651,6 +680,7 @@ def main():
            "template <int L> __host__ __device__ inline void fe_fill_kelvin(double*);",
            "template <int L> __device__ __forceinline__ void fe_far_grad(const double, const double, const double, fe_gdptr __restrict__, double&, double&, double&, double&, double&);",
            "template <int L> __device__ __forceinline__ double fe_far_value(const double, const double, const double, fe_gdptr __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_det_from_dg(const double* __restrict__, const double, double* __restrict__);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, nds, koff, moff = [], [], [], [0], [0]
