@@ -4,15 +4,15 @@
 //   (x, y, z, gV) of [q0, q1) are wave-uniform scalar loads, so there is no
 //   cross-lane reduction.  fe_dg_block writes
 //   out[k * stride + pair] = -sum_i gV_i K R_i^a F_m(T_i).
-// fe_dg_to_dp_sp: dL/dG -> dL/dP for one shell pair (64 lanes):
-//   d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t], d blk_{mu nu} = c_i c_j
-//   sum_t d etab_t Ex Ey Ez; both triangles of P receive d blk because the
-//   block is folded P_ij + P_ji^T (fastesp/esp.py:185-187).
+// fe_det_from_dg<L> (generated) and fe_dp_task<L>: dL/dG -> dL/dP,
+//   d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k, t] per kept record, then per
+//   entry (mu, nu) of a shell pair d blk = sum over its primitive pairs of
+//   c_i c_j sum_t d etab_t Ex Ey Ez; both triangles of P receive d blk because
+//   the block is folded P_ij + P_ji^T (fastesp/esp.py:185-187).
 #pragma once
 
 #include "fe_internal.h"
 #include "fe_boys.h"
-#include "fe_hermite.h"
 #include "fe_lclass.inc"
 
 struct DPArgs {
@@ -29,7 +29,6 @@ struct DPArgs {
   const double* dG[FE_LMAX + 1];
   int n_func;
   double* dP;
-  LTabDev lt;
 };
 
 // acc[k] = sum_{i in [q0, q1)} gV_i K R_i^a F_m(T_i) for the lane's pair (the
@@ -147,103 +146,6 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
   }
 }
 
-// One shell pair `sp` of the table described by `a`; 64 lanes.  The caller
-// synchronises the workgroup before reusing the shared arrays for another
-// shell pair.
-__device__ __forceinline__ void fe_dg_to_dp_sp(const DPArgs& a, const int sp) {
-  __shared__ int s_ci[10][3], s_cj[10][3];
-  __shared__ double s_E[3][4 * 4 * 7];
-  __shared__ double s_det[84];
-  const int tid = threadIdx.x;
-  const int si = a.sp_i[sp], sj = a.sp_j[sp];
-  const int li = a.sh_l[si], lj = a.sh_l[sj];
-  const int L = li + lj, T = L + 1;
-  const int ni = a.sh_nfn[si], nj = a.sh_nfn[sj];
-  const int fi = a.sh_fn0[si], fj = a.sh_fn0[sj];
-  const int npi = a.sh_nprim[si], npj = a.sh_nprim[sj];
-  const int pi0 = a.sh_prim0[si], pj0 = a.sh_prim0[sj];
-  const int slot0 = a.sp_slot[sp];
-  const int NT = a.lt.key_off[L + 1] - a.lt.key_off[L];
-  const int nT = a.lt.mono_off[L + 1] - a.lt.mono_off[L];
-  const int* tl = a.lt.tlist + 3 * a.lt.mono_off[L];
-  const int* cp = a.lt.m0t_colptr + a.lt.mono_off[L];
-  const int k0 = a.lt.key_off[L];
-  const int* s2k = a.slot2kept[L];
-  const double* dGL = a.dG[L];
-  if (tid < ni) {
-    s_ci[tid][0] = a.fn_lx[fi + tid];
-    s_ci[tid][1] = a.fn_ly[fi + tid];
-    s_ci[tid][2] = a.fn_lz[fi + tid];
-  }
-  if (tid < nj) {
-    s_cj[tid][0] = a.fn_lx[fj + tid];
-    s_cj[tid][1] = a.fn_ly[fj + tid];
-    s_cj[tid][2] = a.fn_lz[fj + tid];
-  }
-  __syncthreads();
-  const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
-  const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
-  double acc0 = 0.0, acc1 = 0.0;  // entries e = tid and tid + 64 of the ni x nj block
-  // candidate slots are read 64 at a time, one lane each, and only the kept
-  // ones are visited, in slot order (a slot-by-slot loop would wait for every
-  // slot's load in turn, and contracted shell pairs have dozens of slots)
-  const int nslot = npi * npj;
-  for (int c0 = 0; c0 < nslot; c0 += 64) {
-    const int kl = c0 + tid < nslot ? s2k[slot0 + c0 + tid] : -1;
-    uint64_t live = __ballot(kl >= 0);
-    while (live) {
-      const int lane = __builtin_ctzll(live);
-      live &= live - 1;
-      const int kept = __builtin_amdgcn_readlane(kl, lane);
-      const int ii = (c0 + lane) / npj, jj = c0 + lane - ii * npj;
-      const double al = a.pexp[pi0 + ii], be = a.pexp[pj0 + jj];
-      const double p = al + be;
-      const double cc = a.pcoef[pi0 + ii] * a.pcoef[pj0 + jj];
-      const double Px = (al * Ax + be * Bx) / p;
-      const double Py = (al * Ay + be * By) / p;
-      const double Pz = (al * Az + be * Bz) / p;
-      if (tid < 3) {
-        const double xa = (tid == 0 ? Px - Ax : tid == 1 ? Py - Ay : Pz - Az);
-        const double xb = (tid == 0 ? Px - Bx : tid == 1 ? Py - By : Pz - Bz);
-        fe_e1d_table(s_E[tid], li, lj, xa, xb, p);
-      }
-      const double tp = 2.0 * p, cf = 2.0 * M_PI / p;
-      const double* g = dGL + (size_t)kept * NT;
-      for (int t = tid; t < nT; t += 64) {
-        double s = 0.0;
-        for (int q = cp[t]; q < cp[t + 1]; ++q) {
-          const int k = a.lt.m0t_k[q];
-          double pw = 1.0;
-          for (int m = 0; m < a.lt.key_m[k0 + k]; ++m) pw *= tp;
-          s += a.lt.m0t_val[q] * pw * g[k];
-        }
-        s_det[t] = cf * s;
-      }
-      __syncthreads();
-      for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
-        const int mi = e / nj, mj = e - mi * nj;
-        const int ex0 = (s_ci[mi][0] * (lj + 1) + s_cj[mj][0]) * T;
-        const int ey0 = (s_ci[mi][1] * (lj + 1) + s_cj[mj][1]) * T;
-        const int ez0 = (s_ci[mi][2] * (lj + 1) + s_cj[mj][2]) * T;
-        double s = 0.0;
-        for (int t = 0; t < nT; ++t) {
-          const int tx = tl[3 * t], ty = tl[3 * t + 1], tz = tl[3 * t + 2];
-          s += s_det[t] * s_E[0][ex0 + tx] * s_E[1][ey0 + ty] * s_E[2][ez0 + tz];
-        }
-        if (slot == 0) acc0 = fma(cc, s, acc0);
-        else acc1 = fma(cc, s, acc1);
-      }
-      __syncthreads();
-    }
-  }
-  for (int e = tid, slot = 0; e < ni * nj; e += 64, ++slot) {
-    const int mi = e / nj, mj = e - mi * nj;
-    const double v = slot == 0 ? acc0 : acc1;
-    a.dP[(size_t)(fi + mi) * a.n_func + (fj + mj)] = v;
-    if (si != sj) a.dP[(size_t)(fj + mj) * a.n_func + (fi + mi)] = v;
-  }
-}
-
 // 1-D Hermite coefficients E^{ab}_t, t = 0..L (zero above a + b), of
 // (u+xa)^a (u+xb)^b e^{-p u^2} for a, b <= 3, a + b <= L, by the recurrence of
 // fe_e1d_table raising a, then b; h = 1/(2p).  Compile-time bounds and
@@ -347,6 +249,5 @@ inline DPArgs fe_dp_args(const fe_table* t, double* dG, const size_t* dg_off, do
   }
   pa.n_func = t->n_func;
   pa.dP = dP;
-  pa.lt = t->ctx->ltab;
   return pa;
 }

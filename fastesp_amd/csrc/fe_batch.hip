@@ -122,6 +122,9 @@ struct fe_batch {
   DPArgs* dpargs = nullptr;
   int64_t* sp_off = nullptr;  // per class L: (n_conf + 1) offsets of its shell pairs
   int64_t n_sp[FE_LMAX + 1] = {}, n_sp_tot = 0;
+  int64_t* rec_off = nullptr;   // per class L: (n_conf + 1) offsets of the planned records
+  int64_t* task_off = nullptr;  // per class L: (n_conf + 1) offsets of the dL/dP tasks
+  int64_t n_rec[FE_LMAX + 1] = {}, n_task[FE_LMAX + 1] = {};
   std::vector<int64_t> dp_off;  // n_conf + 1
   // tuning
   int ppl[FE_LMAX + 1] = {2, 2, 2, 2, 2, 1, 1};
@@ -296,22 +299,46 @@ __global__ __launch_bounds__(256) void k_dG_reduce_batch(const DgSeg* __restrict
   dG[sg.dg + (int64_t)pair * NT + k] = s;
 }
 
-// the class-L shell pairs of every conformer (their dL/dG is reduced)
-__global__ __launch_bounds__(64) void k_dg_to_dp_batch(const DPArgs* __restrict__ args,
-                                                       const int64_t* __restrict__ sp_off,
-                                                       const int n_conf, const int64_t n_sp,
-                                                       const int L) {
-  for (int64_t s = blockIdx.x; s < n_sp; s += gridDim.x) {
-    int lo = 0, hi = n_conf - 1;  // last conformer with sp_off <= s
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (sp_off[mid] <= s) lo = mid;
-      else hi = mid - 1;
-    }
-    const DPArgs& a = args[lo];
-    fe_dg_to_dp_sp(a, a.sp_by_class[a.sp_cls_off[L] + (int)(s - sp_off[lo])]);
-    __syncthreads();
+// the last conformer c with off[c] <= x (off: n_conf + 1 ascending offsets)
+__device__ __forceinline__ int fe_conf_of(const int64_t* __restrict__ off, const int n_conf,
+                                          const int64_t x) {
+  int lo = 0, hi = n_conf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid - 1;
   }
+  return lo;
+}
+
+// d etab of every conformer's kept class-L records (one lane each; the
+// records of conformer c are [rec_off[c], rec_off[c+1]) of the launch)
+template <int L>
+__global__ __launch_bounds__(256) void k_det_batch(const DPArgs* __restrict__ args,
+                                                   const int64_t* __restrict__ rec_off,
+                                                   const int n_conf, const int64_t n_rec,
+                                                   const double* const* __restrict__ recs,
+                                                   const int* const* __restrict__ kept) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= n_rec) return;
+  const int cf = fe_conf_of(rec_off, n_conf, g);
+  const int r = (int)(g - rec_off[cf]);
+  if (r >= *kept[cf]) return;
+  const DPArgs& a = args[cf];
+  fe_det_from_dg<L>(a.dG[L] + (size_t)r * FeL<L>::NT, recs[cf][(size_t)r * FeL<L>::STRIDE + 3],
+                    (double*)a.det[L] + (size_t)r * FeL<L>::NM);
+}
+
+// the class-L dL/dP tasks of every conformer (their d etab is ready)
+template <int L>
+__global__ __launch_bounds__(256) void k_dp_tasks_batch(const DPArgs* __restrict__ args,
+                                                        const int64_t* __restrict__ task_off,
+                                                        const int n_conf, const int64_t n_task) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= n_task) return;
+  const int cf = fe_conf_of(task_off, n_conf, g);
+  const DPArgs& a = args[cf];
+  fe_dp_task<L>(a, a.tasks[a.task_cls_off[L] + (int)(g - task_off[cf])]);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,6 +681,12 @@ static int plan_bwd(fe_batch* b) {
   size_t part_total = 0, dg_total = 0;
   std::vector<DPArgs> args(nc);
   std::vector<int64_t> sp_off((size_t)(FE_LMAX + 1) * (nc + 1), 0);
+  // per class: (n_conf + 1) offsets of the conformers' planned records and
+  // of their dL/dP tasks; each (conformer, class)'s d etab goes to its
+  // dL/dG partial slots once the class's reduction has read them
+  std::vector<int64_t> rec_off((size_t)(FE_LMAX + 1) * (nc + 1), 0);
+  std::vector<int64_t> task_off((size_t)(FE_LMAX + 1) * (nc + 1), 0);
+  std::vector<std::array<size_t, FE_LMAX + 1>> det_off(nc);
   std::vector<size_t> dgbase(nc);
   std::vector<std::array<size_t, FE_LMAX + 1>> dg_off(nc);
   b->dp_off.assign(nc + 1, 0);
@@ -676,6 +709,7 @@ static int plan_bwd(fe_batch* b) {
         ppc = (n + ch - 1) / ch;
         ch = (n + ppc - 1) / ppc;
       }
+      det_off[i][L] = part_total;
       DgSeg sg{};
       sg.part = (int64_t)part_total;
       sg.dg = (int64_t)dg_total;
@@ -697,13 +731,18 @@ static int plan_bwd(fe_batch* b) {
           d.n_pairs = (int)nL;
           items[L].push_back(d);
         }
-      part_total += (size_t)ch * NT * nL;
+      // the slots also hold the class's d etab later (NM <= NT per record),
+      // also for a conformer without points (no chunks)
+      part_total += std::max((size_t)ch * NT * nL, (size_t)nL * fe_host::NMONO[L]);
       dg_total += (size_t)nL * NT;
       e_run[L] += nL * NT;
     }
-    for (int L = 0; L <= FE_LMAX; ++L)
-      sp_off[(size_t)L * (nc + 1) + i + 1] =
-          sp_off[(size_t)L * (nc + 1) + i] + (t->sp_cls_off[L + 1] - t->sp_cls_off[L]);
+    for (int L = 0; L <= FE_LMAX; ++L) {
+      const size_t o = (size_t)L * (nc + 1) + i;
+      sp_off[o + 1] = sp_off[o] + (t->sp_cls_off[L + 1] - t->sp_cls_off[L]);
+      rec_off[o + 1] = rec_off[o] + plan_pairs(b, t, L);
+      task_off[o + 1] = task_off[o] + (t->task_cls_off[L + 1] - t->task_cls_off[L]);
+    }
     b->dp_off[i + 1] = b->dp_off[i] + (int64_t)t->n_func * t->n_func;
   }
   for (int L = 0; L <= FE_LMAX; ++L) {
@@ -728,6 +767,7 @@ static int plan_bwd(fe_batch* b) {
   const size_t o_part = cv.take<double>(part_total), o_dG = cv.take<double>(dg_total);
   const size_t o_dP = cv.take<double>((size_t)b->dp_off[nc]);
   const size_t o_args = cv.take<DPArgs>(nc), o_spo = cv.take<int64_t>(sp_off.size());
+  const size_t o_reco = cv.take<int64_t>(rec_off.size()), o_tko = cv.take<int64_t>(task_off.size());
   FE_HIPCHK(hipSetDevice(c->device));
   if (int rc = batch_alloc(&b->bmem, cv.off, "backward buffers")) return rc;
   char* m = (char*)b->bmem;
@@ -736,6 +776,8 @@ static int plan_bwd(fe_batch* b) {
   b->dP = (double*)(m + o_dP);
   b->dpargs = (DPArgs*)(m + o_args);
   b->sp_off = (int64_t*)(m + o_spo);
+  b->rec_off = (int64_t*)(m + o_reco);
+  b->task_off = (int64_t*)(m + o_tko);
   for (int L = 0; L <= FE_LMAX; ++L) {
     b->dgitems[L] = (FeDgItem*)(m + o_it[L]);
     b->n_dgitems[L] = (int)items[L].size();
@@ -750,11 +792,21 @@ static int plan_bwd(fe_batch* b) {
       FE_HIPCHK(hipMemcpy(b->segs[L], segs[L].data(), segs[L].size() * sizeof(DgSeg),
                           hipMemcpyHostToDevice));
   }
-  for (int i = 0; i < nc; ++i)
+  for (int i = 0; i < nc; ++i) {
     args[i] = fe_dp_args(b->tabs[i], b->dG + dgbase[i], dg_off[i].data(), b->dP + b->dp_off[i]);
+    for (int L = 0; L <= FE_LMAX; ++L) args[i].det[L] = b->dg_part + det_off[i][L];
+  }
   FE_HIPCHK(hipMemcpy(b->dpargs, args.data(), nc * sizeof(DPArgs), hipMemcpyHostToDevice));
   FE_HIPCHK(hipMemcpy(b->sp_off, sp_off.data(), sp_off.size() * sizeof(int64_t),
                       hipMemcpyHostToDevice));
+  FE_HIPCHK(hipMemcpy(b->rec_off, rec_off.data(), rec_off.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice));
+  FE_HIPCHK(hipMemcpy(b->task_off, task_off.data(), task_off.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice));
+  for (int L = 0; L <= FE_LMAX; ++L) {
+    b->n_rec[L] = rec_off[(size_t)L * (nc + 1) + nc];
+    b->n_task[L] = task_off[(size_t)L * (nc + 1) + nc];
+  }
   b->n_sp_tot = 0;
   for (int L = 0; L <= FE_LMAX; ++L) {
     b->n_sp[L] = sp_off[(size_t)L * (nc + 1) + nc];
@@ -825,12 +877,30 @@ static void launch_dG_L(fe_batch* b, hipStream_t st) {
   }
 }
 
+template <int L>
+static void launch_dp_T(fe_batch* b, hipStream_t st) {
+  const size_t o = (size_t)L * (b->n_conf + 1);
+  if (b->n_rec[L])
+    hipLaunchKernelGGL(k_det_batch<L>, dim3((unsigned)((b->n_rec[L] + 255) / 256)), dim3(256), 0,
+                       st, b->dpargs, b->rec_off + o, b->n_conf, b->n_rec[L],
+                       (const double* const*)b->recs[L], (const int* const*)b->kept[L]);
+  hipLaunchKernelGGL(k_dp_tasks_batch<L>, dim3((unsigned)((b->n_task[L] + 255) / 256)),
+                     dim3(256), 0, st, b->dpargs, b->task_off + o, b->n_conf, b->n_task[L]);
+}
+
 static void launch_dp_L(fe_batch* b, int L, hipStream_t st) {
-  if (!b->n_sp[L]) return;
-  const int64_t grid = std::min<int64_t>(b->n_sp[L], (int64_t)b->ctx->n_cu * 32);
+  if (!b->n_task[L]) return;
   const int pd = prof_begin(b, FE_PH_DP, st);
-  hipLaunchKernelGGL(k_dg_to_dp_batch, dim3((unsigned)grid), dim3(64), 0, st, b->dpargs,
-                     b->sp_off + (size_t)L * (b->n_conf + 1), b->n_conf, b->n_sp[L], L);
+  switch (L) {
+    case 0: launch_dp_T<0>(b, st); break;
+    case 1: launch_dp_T<1>(b, st); break;
+    case 2: launch_dp_T<2>(b, st); break;
+    case 3: launch_dp_T<3>(b, st); break;
+    case 4: launch_dp_T<4>(b, st); break;
+    case 5: launch_dp_T<5>(b, st); break;
+    case 6: launch_dp_T<6>(b, st); break;
+    default: break;
+  }
   fe_prof_end(b->ctx, pd, st);
 }
 

@@ -248,3 +248,24 @@ def test_far_field_polynomial_is_harmonic_by_degree(L):
                     lap2[0] = lap2.get(0, 0.0) + b[d] * (b[d] - 1) * v
     if L >= 2:
         assert abs(lap2[0]) > 1e-6
+
+
+@pytest.mark.parametrize("L", range(7))
+def test_det_from_dg_is_the_adjoint_of_the_table_contraction(src, L):
+    """fe_det_from_dg<L> (dL/dG -> d etab of one record) is the transpose of
+    G_k = (2 pi/p)(2p)^m_k sum_t M0[k, t] etab_t: <det, etab> = <g, G(etab)>
+    for random etab and g, with M0 from the key map of gen_lclass."""
+    gen = _gen()
+    keys, acc = gen.keys_of(L)
+    order = gen.schedule_of(L)[0]
+    T = L + 1
+    rng = np.random.default_rng(120 + L)
+    p = 0.77
+    g = list(rng.normal(size=len(keys)))
+    det = [0.0] * len(order)
+    run(body(src, "fe_det_from_dg", L), dict(g=g, p=p, det=det))
+    etab = rng.normal(size=len(order))
+    pos = {((tx * T + ty) * T + tz): i for i, (tx, ty, tz) in enumerate(order)}
+    G = [2 * np.pi / p * (2 * p) ** m * sum(v * etab[pos[col]] for col, v in acc[(ax, ay, az, m)].items())
+         for (ax, ay, az, m) in keys]
+    assert np.dot(det, etab) == pytest.approx(np.dot(g, G), rel=1e-12)
