@@ -10,10 +10,12 @@
 //      pair and the points (x, y, z, gV) streamed as wave-uniform scalar
 //      loads (k_dG<L>; no cross-lane reduction), chunked over points with a
 //      fixed-order reduction (bitwise reproducible); then dL/dP by the adjoint
-//      of the table contraction, one workgroup per shell pair (k_dg_to_dp):
-//      d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k,t], d blk_{mu nu} = c_i c_j
-//      sum_t d etab_t Ex Ey Ez, and both triangles of P receive d blk because
-//      the block is folded P_ij + P_ji^T (fastesp/esp.py:185-187).
+//      of the table contraction: d etab_t = sum_k dG_k (2 pi/p)(2p)^m M0[k,t]
+//      per kept record (k_det<L>), then one lane per (shell pair, function
+//      pair) task (k_dp_tasks<L>): d blk_{mu nu} = c_i c_j sum over the kept
+//      primitive pairs and t of d etab_t Ex Ey Ez, and both triangles of P
+//      receive d blk because the block is folded P_ij + P_ji^T
+//      (fastesp/esp.py:185-187).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

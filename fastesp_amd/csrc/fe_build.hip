@@ -10,14 +10,17 @@
 //           every primitive pair gets a candidate slot in its class-L array
 //           and its reference-order index (shell pairs in order, primitives
 //           i-major).
-//  kernel : k_build_pairs, one 64-lane workgroup per shell pair.  Folded
-//           density block P_ij (+ P_ji^T), skip if max|blk| < 1e-15; per
-//           primitive pair p, P, K (skip K < 1e-30), 1-D Hermite E tables
-//           (three lanes, one axis each), etab over the |t| <= L simplex (lanes
-//           over t), G = (2 pi/p)(2p)^m M0 etab (lanes over terms), the
-//           screening bound (wave reduction), reff and the sum-rule partial.
-//  kernel : the screen (k_screen_boxes), one lane per candidate.  The points
-//           are Morton-sorted and cut into boxes of 64; the rules
+//  kernel : k_build_cands, one lane per candidate, the shell pairs grouped by
+//           (l_i, l_j) at basis set-up and compiled per pair of l's (two
+//           launches: l_i + l_j >= 4 on the context's second stream, beside
+//           the others and the point sort).  Folded density block P_ij
+//           (+ P_ji^T), skip if max|blk| < 1e-15; p, P, K (skip K < 1e-30);
+//           etab over the |t| <= L simplex from the block's polynomial about P
+//           (fe_pair_etab, the closed form of the 1-D Hermite recurrences);
+//           G = (2 pi/p)(2p)^m M0 etab (fe_g_from_etab); the screening bound
+//           (fe_etab_bound), reff and the candidate's sum-rule term.
+//  kernel : the screen (k_screen_boxes), one wavefront per open candidate.
+//           The points are Morton-sorted and cut into boxes of 64; the rules
 //           rule 0 (NumPy, table.py:190-194) keep = !(K bound/max(dmin,0.05) < tol)
 //           rule 1 (C++, build_pairs.cpp:318-328) keep = !(K bound/max(dmin-reff,0.05) < tol)
 //                  and G not all zero and c_i c_j != 0
@@ -27,14 +30,16 @@
 //           point passes it and rule 2 iff every point does, and a box whose
 //           distance bounds settle the test for all its points is decided
 //           (or skipped) without reading them.
-//  device : compaction and ordering.  Each candidate gets a 32-bit key
-//           (class, dropped flag, exponent bucket, Morton code of P), the keys
-//           are radix-sorted (stable), and the kept records of class L are the
-//           first ones of its range; a prefix sum over the candidates in
-//           reference order gives each kept record its flat term offset and
-//           its reference-order index.  Nothing goes back to the host but the
-//           counts (fe_table_build, one synchronisation to size the table);
-//           fe_table_rebuild does not synchronise at all.
+//  device : ordering and compaction.  Each candidate gets a 32-bit key
+//           (class, dropped flag, exponent bucket, Morton code of P); the keys
+//           are radix-sorted (stable) before the screen, which walks the open
+//           candidates in that order and sets the dropped flag of those it
+//           drops; one more stable pass over the top byte puts the kept
+//           records of class L first in its range.  A prefix sum over the
+//           candidates in reference order gives each kept record its flat term
+//           offset and its reference-order index.  Nothing goes back to the
+//           host but the counts (fe_table_build, one synchronisation to size
+//           the table); fe_table_rebuild does not synchronise at all.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,18 +50,10 @@
 #include "fe_boys.h"
 #include "fe_lclass.inc"
 #include "fe_internal.h"
-#include "fe_hermite.h"
 
 #define FE_BUILD_MAXPRIM 32
-#define FE_BUILD_MAXFN 10  // Cartesian components of an f shell
 #define FE_BOX 64          // sorted points per screening box
-#define FE_BOX_CHUNK 256   // boxes staged in LDS at a time
-
-// \int |He_n| e^{-x^2/2} dx as tabulated by the reference (fastesp/esp.py:25-28)
-__constant__ double c_mabs[12] = {2.5066282746310002, 2.0, 1.5957691216057308, 4.0,
-                                  10.212920779565312, 33.63284766793385, 136.5963625280984,
-                                  665.1416330443616, 3768.863619586618, 25506.330595911374,
-                                  202331.81842358326, 1872154.2572862155};
+#define FE_SENTINELS 32    // points the screen's first pass tests per candidate
 
 struct ClassInfo {
   int stride[FE_LMAX + 1], nterm[FE_LMAX + 1], nmono[FE_LMAX + 1];
@@ -64,16 +61,16 @@ struct ClassInfo {
 };
 
 struct BuildArgs {
-  const int* sh_atom; const int* sh_l; const int* sh_fn0; const int* sh_nfn; const int* sh_prim0;
-  const int* sh_nprim; const double* sh_c;
-  const double* pexp; const double* pcoef;
-  const int* fn_lx; const int* fn_ly; const int* fn_lz;
+  const int* sh_l; const int* sh_fn0; const int* sh_prim0; const int* sh_nprim;
+  const double* sh_c; const double* pexp; const double* pcoef;
+  const int* fn_cperm;
   const int* sp_i; const int* sp_j; const int* sp_slot; const int* sp_gidx;
+  const int* cl_sp; const int* cl_s;
+  int cmb_off[FE_NCOMBO + 1];  // candidates of shell-l pair c: cl_*[cmb_off[c], cmb_off[c+1])
   const double* dens; int n_func;
   double* cand[FE_LMAX + 1];
-  int* gidx;  // per candidate (class-concatenated slot): reference-order index
-  double* sp_srule;
-  LTabDev lt;
+  int* gidx;           // per candidate (class-concatenated slot): reference-order index
+  double* cand_srule;  // per candidate: its term of the sum rule
   ClassInfo ci;
 };
 
@@ -87,159 +84,134 @@ __device__ __forceinline__ double wave_sum(double v) {
 // terms.  flags: 1 candidate of the reference's pair list (K >= 1e-30 and a
 // non-empty density block), 2 some G non-zero, 4 c_i c_j != 0, 8 K >= 1e-30
 // (independent of the density: the capacity of a rebuildable table).
-__global__ __launch_bounds__(64) void k_build_pairs(const BuildArgs a) {
-  __shared__ double s_blk[FE_BUILD_MAXFN * FE_BUILD_MAXFN];
-  __shared__ int s_ci[FE_BUILD_MAXFN][3], s_cj[FE_BUILD_MAXFN][3];
-  __shared__ double s_ei[FE_BUILD_MAXPRIM], s_ej[FE_BUILD_MAXPRIM];
-  __shared__ double s_wi[FE_BUILD_MAXPRIM], s_wj[FE_BUILD_MAXPRIM];
-  __shared__ double s_E[3][4 * 4 * 7];
-  __shared__ double s_et[84];
-  __shared__ int s_nonzero;
-
-  const int sp = blockIdx.x;
-  const int tid = threadIdx.x;
+//
+// One lane per candidate (primitive pair) of a shell pair whose shells have
+// l = LI and LJ (LI >= LJ; a pair listed the other way round is handled with
+// the shells' roles swapped, which leaves the block's polynomial unchanged).
+// Everything is straight-line code in registers: the folded block, its
+// polynomial about P, etab (fe_pair_etab), G (fe_g_from_etab) and the
+// screening bound (fe_etab_bound).
+template <int LI, int LJ>
+__device__ __forceinline__ void fe_build_cand(const BuildArgs& a, const int q) {
+  constexpr int L = LI + LJ;
+  constexpr int NI = (LI + 1) * (LI + 2) / 2, NJ = (LJ + 1) * (LJ + 2) / 2;
+  constexpr int NT = FeL<L>::NT, NM = FeL<L>::NM, S = FeL<L>::KOFF;
+  const int sp = a.cl_sp[q], s = a.cl_s[q];
   const int si = a.sp_i[sp], sj = a.sp_j[sp];
-  const int li = a.sh_l[si], lj = a.sh_l[sj];
-  const int L = li + lj;
-  const int ni = a.sh_nfn[si], nj = a.sh_nfn[sj];
-  const int fi = a.sh_fn0[si], fj = a.sh_fn0[sj];
-  const int npi = a.sh_nprim[si], npj = a.sh_nprim[sj];
-  const int S = a.ci.stride[L];
-  const int nterm = a.ci.nterm[L];
-  const int nT = a.ci.nmono[L];
-  double* cand = a.cand[L];
-  const int slot0 = a.sp_slot[sp];
-
-  // reference-order index of every slot of this shell pair
-  for (int s = tid; s < npi * npj; s += 64)
-    a.gidx[a.ci.cand_off[L] + slot0 + s] = a.sp_gidx[sp] + s;
-  if (tid == 0) s_nonzero = 0;
-  __syncthreads();
-  for (int idx = tid; idx < ni * nj; idx += 64) {
-    const int mi = idx / nj, mj = idx - mi * nj;
-    double d = a.dens[(size_t)(fi + mi) * a.n_func + (fj + mj)];
-    if (si != sj) d += a.dens[(size_t)(fj + mj) * a.n_func + (fi + mi)];
-    s_blk[idx] = d;
-    if (!(fabs(d) < 1e-15)) s_nonzero = 1;  // np.abs(blk).max() < 1e-15 -> skip
-  }
-  if (tid < ni) {
-    s_ci[tid][0] = a.fn_lx[fi + tid];
-    s_ci[tid][1] = a.fn_ly[fi + tid];
-    s_ci[tid][2] = a.fn_lz[fi + tid];
-  }
-  if (tid < nj) {
-    s_cj[tid][0] = a.fn_lx[fj + tid];
-    s_cj[tid][1] = a.fn_ly[fj + tid];
-    s_cj[tid][2] = a.fn_lz[fj + tid];
-  }
-  if (tid < npi) {
-    s_ei[tid] = a.pexp[a.sh_prim0[si] + tid];
-    s_wi[tid] = a.pcoef[a.sh_prim0[si] + tid];
-  }
-  if (tid < npj) {
-    s_ej[tid] = a.pexp[a.sh_prim0[sj] + tid];
-    s_wj[tid] = a.pcoef[a.sh_prim0[sj] + tid];
-  }
-  __syncthreads();
+  const int ii = s / a.sh_nprim[sj], jj = s - ii * a.sh_nprim[sj];
+  const int slot = a.sp_slot[sp] + s;
+  const int64_t e = a.ci.cand_off[L] + slot;
+  a.gidx[e] = a.sp_gidx[sp] + s;
+  double* rec = a.cand[L] + (size_t)slot * S;
   const double Ax = a.sh_c[3 * si], Ay = a.sh_c[3 * si + 1], Az = a.sh_c[3 * si + 2];
   const double Bx = a.sh_c[3 * sj], By = a.sh_c[3 * sj + 1], Bz = a.sh_c[3 * sj + 2];
   const double dx = Ax - Bx, dy = Ay - By, dz = Az - Bz;
   const double ab2 = dx * dx + dy * dy + dz * dz;
-  if (!s_nonzero) {
-    // no candidates; only the K >= 1e-30 flag (the capacity) is recorded
-    for (int s = tid; s < npi * npj; s += 64) {
-      const double al = s_ei[s / npj], be = s_ej[s % npj];
-      const double K = exp(-al * be / (al + be) * ab2);
-      cand[(size_t)(slot0 + s) * S + 7] = K < 1e-30 ? 0.0 : 8.0;
-    }
-    if (tid == 0) a.sp_srule[sp] = 0.0;
+  const double al = a.pexp[a.sh_prim0[si] + ii], be = a.pexp[a.sh_prim0[sj] + jj];
+  const double p = al + be;
+  const double K = exp(-al * be / p * ab2);
+  if (K < 1e-30) {
+    rec[7] = 0.0;
+    a.cand_srule[e] = 0.0;
     return;
   }
-  const int T = L + 1;
-  const int* tl = a.lt.tlist + 3 * a.lt.mono_off[L];
-  const int k0 = a.lt.key_off[L];
-  double srule = 0.0;
-
-  for (int ii = 0; ii < npi; ++ii) {
-    for (int jj = 0; jj < npj; ++jj) {
-      double* rec = cand + (size_t)(slot0 + ii * npj + jj) * S;
-      const double al = s_ei[ii], be = s_ej[jj];
-      const double p = al + be;
-      const double K = exp(-al * be / p * ab2);
-      if (K < 1e-30) {
-        if (tid == 0) rec[7] = 0.0;
-        continue;
-      }
-      const double cc = s_wi[ii] * s_wj[jj];
-      const double Px = (al * Ax + be * Bx) / p;
-      const double Py = (al * Ay + be * By) / p;
-      const double Pz = (al * Az + be * Bz) / p;
-      if (tid < 3) {
-        // 1-D Hermite coefficients of axis `tid`
-        const double xa = (tid == 0 ? Px - Ax : tid == 1 ? Py - Ay : Pz - Az);
-        const double xb = (tid == 0 ? Px - Bx : tid == 1 ? Py - By : Pz - Bz);
-        fe_e1d_table(s_E[tid], li, lj, xa, xb, p);
-      }
-      __syncthreads();
-      // etab over the |t| <= L simplex
-      for (int t = tid; t < nT; t += 64) {
-        const int tx = tl[3 * t], ty = tl[3 * t + 1], tz = tl[3 * t + 2];
-        double s = 0.0;
-        for (int mi = 0; mi < ni; ++mi) {
-          const int ax = s_ci[mi][0], ay = s_ci[mi][1], az = s_ci[mi][2];
-          for (int mj = 0; mj < nj; ++mj) {
-            const double w = s_blk[mi * nj + mj];
-            if (w == 0.0) continue;
-            const double ex = s_E[0][(ax * (lj + 1) + s_cj[mj][0]) * T + tx];
-            const double ey = s_E[1][(ay * (lj + 1) + s_cj[mj][1]) * T + ty];
-            const double ez = s_E[2][(az * (lj + 1) + s_cj[mj][2]) * T + tz];
-            s += w * ex * ey * ez;
-          }
-        }
-        s_et[t] = s * cc;
-      }
-      __syncthreads();
-      const double tp = 2.0 * p;
-      const double cf = 2.0 * M_PI / p;
-      int anyg = 0;
-      for (int k = tid; k < nterm; k += 64) {
-        const int r = k0 + k;
-        double s = 0.0;
-        for (int q = a.lt.m0_rowptr[r]; q < a.lt.m0_rowptr[r + 1]; ++q)
-          s += a.lt.m0_val[q] * s_et[a.lt.m0_tidx[q]];
-        double pw = 1.0;
-        for (int m = 0; m < a.lt.key_m[r]; ++m) pw *= tp;
-        const double g = cf * pw * s;
-        rec[8 + k] = g;
-        anyg |= (g != 0.0);
-      }
-      const double sq = sqrt(tp);
-      double b = 0.0;
-      for (int t = tid; t < nT; t += 64) {
-        const int tx = tl[3 * t], ty = tl[3 * t + 1], tz = tl[3 * t + 2];
-        double pw = 1.0;
-        for (int q = 0; q < tx + ty + tz; ++q) pw *= sq;
-        b += fabs(s_et[t]) * (c_mabs[tx] * c_mabs[ty] * c_mabs[tz]) * pw;
-      }
-      b = wave_sum(b);
-      anyg = __any(anyg);
-      if (tid == 0) {
-        const double bound = pow(tp, -1.5) * b;
-        rec[0] = Px;
-        rec[1] = Py;
-        rec[2] = Pz;
-        rec[3] = p;
-        rec[4] = K;
-        rec[5] = K * bound;
-        rec[6] = 1.6 * sqrt((L + 2.0) / p);
-        rec[7] = 1.0 + (anyg ? 2.0 : 0.0) + (cc != 0.0 ? 4.0 : 0.0) + 8.0;
-        srule += K * pow(M_PI / p, 1.5) * s_et[0];
-      }
-      for (int k = nterm + tid; k < S - 8; k += 64) rec[8 + k] = 0.0;
-      __syncthreads();
+  // the folded block P_ij (+ P_ji^T), shell X (l = LI) by rows; skip the
+  // shell pair if max|blk| < 1e-15 (fastesp/esp.py:185-189)
+  const bool swap = LI != LJ && a.sh_l[si] != LI;
+  const int sx = swap ? sj : si, sy = swap ? si : sj;
+  const int fx = a.sh_fn0[sx], fy = a.sh_fn0[sy];
+  const size_t nf = (size_t)a.n_func;
+  double w[NI * NJ];
+  bool nz = false;
+#pragma unroll
+  for (int x = 0; x < NI; ++x) {
+    const int gx = fx + a.fn_cperm[fx + x];
+#pragma unroll
+    for (int y = 0; y < NJ; ++y) {
+      const int gy = fy + a.fn_cperm[fy + y];
+      double d = a.dens[(size_t)gx * nf + gy];
+      if (si != sj) d += a.dens[(size_t)gy * nf + gx];
+      w[x * NJ + y] = d;
+      nz |= !(fabs(d) < 1e-15);
     }
   }
-  if (tid == 0) a.sp_srule[sp] = srule;
+  if (!nz) {
+    rec[7] = 8.0;  // no candidate; only the capacity flag
+    a.cand_srule[e] = 0.0;
+    return;
+  }
+  const double cc = a.pcoef[a.sh_prim0[si] + ii] * a.pcoef[a.sh_prim0[sj] + jj];
+  const double Px = (al * Ax + be * Bx) / p;
+  const double Py = (al * Ay + be * By) / p;
+  const double Pz = (al * Az + be * Bz) / p;
+  const double* X = a.sh_c + 3 * sx;
+  const double* Y = a.sh_c + 3 * sy;
+  double et[NM];
+  fe_pair_etab<LI, LJ>(w, Px - X[0], Py - X[1], Pz - X[2], Px - Y[0], Py - Y[1], Pz - Y[2],
+                       0.5 / p, cc, et);
+  double g[NT];
+  fe_g_from_etab<L>(et, p, g);
+  bool anyg = false;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    rec[8 + k] = g[k];
+    anyg |= g[k] != 0.0;
+  }
+#pragma unroll
+  for (int k = 8 + NT; k < S; ++k) rec[k] = 0.0;
+  const double tp = 2.0 * p;
+  const double bound = pow(tp, -1.5) * fe_etab_bound<L>(et, sqrt(tp));
+  rec[0] = Px;
+  rec[1] = Py;
+  rec[2] = Pz;
+  rec[3] = p;
+  rec[4] = K;
+  rec[5] = K * bound;
+  rec[6] = 1.6 * sqrt((L + 2.0) / p);
+  rec[7] = 1.0 + (anyg ? 2.0 : 0.0) + (cc != 0.0 ? 4.0 : 0.0) + 8.0;
+  a.cand_srule[e] = K * pow(M_PI / p, 1.5) * et[0];
+}
+
+// The shell-l pairs of one launch, in launch order: range r covers the blocks
+// [blk_off[r], blk_off[r+1]) and the candidates of shell-l pair combo[r].
+struct CandLaunch {
+  int n_r;
+  int combo[FE_NCOMBO];
+  int blk_off[FE_NCOMBO + 1];
+};
+
+// The candidates of several shell-l pairs in one launch (the heavy ones first),
+// so that the short ranges run side by side rather than one launch after
+// another.  HEAVY: the pairs with l_i + l_j >= 4, which need up to 426 VGPRs;
+// the others run in a launch of their own at a higher occupancy.
+template <bool HEAVY>
+__global__ __launch_bounds__(256) void k_build_cands(const BuildArgs a, const CandLaunch pl) {
+  const int b = (int)blockIdx.x;
+  int r = 0;
+  while (r + 1 < pl.n_r && b >= pl.blk_off[r + 1]) ++r;
+  const int cb = pl.combo[r];
+  const int c0 = a.cmb_off[cb];
+  const int i = (b - pl.blk_off[r]) * 256 + (int)threadIdx.x;
+  if (i >= a.cmb_off[cb + 1] - c0) return;
+  if constexpr (HEAVY) {
+    switch (cb) {
+      case 5: fe_build_cand<2, 2>(a, c0 + i); break;
+      case 7: fe_build_cand<3, 1>(a, c0 + i); break;
+      case 8: fe_build_cand<3, 2>(a, c0 + i); break;
+      case 9: fe_build_cand<3, 3>(a, c0 + i); break;
+      default: break;
+    }
+  } else {
+    switch (cb) {
+      case 0: fe_build_cand<0, 0>(a, c0 + i); break;
+      case 1: fe_build_cand<1, 0>(a, c0 + i); break;
+      case 2: fe_build_cand<1, 1>(a, c0 + i); break;
+      case 3: fe_build_cand<2, 0>(a, c0 + i); break;
+      case 4: fe_build_cand<2, 1>(a, c0 + i); break;
+      case 6: fe_build_cand<3, 0>(a, c0 + i); break;
+      default: break;
+    }
+  }
 }
 
 // Bounding box of each run of FE_BOX sorted points: box[6 b ..] = min xyz, max xyz
@@ -304,11 +276,9 @@ struct ScreenArgs {
   ClassInfo ci;
   const int* gidx;
   unsigned long long* gflag;  // per reference-order index: (nterm << 32 | 1) if kept, else 0
-  uint32_t* key;              // final pair-order keys and candidate indices
+  uint32_t* key;              // pair-order keys and candidate indices (sorted in place)
   int32_t* val;
-  uint32_t* skey;             // screening order: keys and candidate indices
-  int32_t* sval;
-  double* thr;                // per candidate: the squared-distance threshold
+  double* thr;                // per candidate: the squared-distance threshold, < 0 if decided
   const double* pts;  // Morton-sorted
   int n_pts;
   const double* box;
@@ -318,7 +288,6 @@ struct ScreenArgs {
   double lo[3], kscale;  // Morton frame of P (8 bits per axis)
   double lpmin[FE_LMAX + 1], lpinv[FE_LMAX + 1];  // exponent buckets: 16 over the log range
   int* stats;  // FE_ST_* counters, zero before the launch
-  int* n_open;  // candidates left to the box walk (counted by k_screen_prep)
   int64_t n_all;
 };
 
@@ -340,20 +309,21 @@ __device__ __forceinline__ uint32_t fe_morton24(const double* P, const double* l
   return code;
 }
 
-// The screen's outputs for candidate e of class L: its pair-order key (class,
-// dropped flag, exponent bucket, Morton code of P) and its reference-order
-// flag (term count and 1 when kept, else 0).
-__device__ __forceinline__ void fe_screen_emit(const ScreenArgs& a, const int64_t e, const int L,
-                                               const bool kept, const double* r) {
+// The pair-order key of candidate e of class L: class, dropped flag, exponent
+// bucket, Morton code of P.  The screen sets the dropped flag of the
+// candidates it drops, and a stable pass over the top byte then puts each
+// class's kept candidates first, in the order of the full key.
+__device__ __forceinline__ uint32_t fe_pair_key(const ScreenArgs& a, const int L, const bool dropped,
+                                                const double* r) {
   uint32_t k = (uint32_t)L << 29;
-  if (!kept) {
-    k |= 1u << 28;
-  } else {
-    const int bucket = (int)fmin(fmax((log(r[3]) - a.lpmin[L]) * a.lpinv[L], 0.0), 15.0);
-    k |= ((uint32_t)bucket << 24) | fe_morton24(r, a.lo, a.kscale);
-  }
-  a.key[e] = k;
-  a.val[e] = (int32_t)e;
+  if (dropped) return k | 1u << 28;
+  const int bucket = (int)fmin(fmax((log(r[3]) - a.lpmin[L]) * a.lpinv[L], 0.0), 15.0);
+  return k | ((uint32_t)bucket << 24) | fe_morton24(r, a.lo, a.kscale);
+}
+
+// the reference-order flag of candidate e: term count and 1 when kept, else 0
+__device__ __forceinline__ void fe_emit_flag(const ScreenArgs& a, const int64_t e, const int L,
+                                             const bool kept) {
   a.gflag[a.gidx[e]] = kept ? (((unsigned long long)a.ci.nterm[L] << 32) | 1ull) : 0ull;
 }
 
@@ -364,10 +334,10 @@ __device__ __forceinline__ void fe_flush_counts(int* s_cnt, int* stats) {
 }
 
 // Every candidate slot: the counters of candidates and K-valid pairs, the
-// squared-distance threshold of the test, and either the final decision (no
-// candidate, rule 1's G/coefficient test, no points, or a threshold that
-// settles every distance) or a screening key that groups candidates with
-// alike thresholds and nearby P, so the box walk of a wavefront is coherent.
+// squared-distance threshold of the test, its pair-order key, and, where no
+// box walk is needed (no candidate, rule 1's G/coefficient test, no points,
+// or a threshold that settles every distance), the final decision; decided
+// candidates get a negative threshold.
 __global__ __launch_bounds__(256) void k_screen_prep(const ScreenArgs a) {
   __shared__ int s_cnt[FE_ST_N];
   if (threadIdx.x < FE_ST_N) s_cnt[threadIdx.x] = 0;
@@ -396,88 +366,116 @@ __global__ __launch_bounds__(256) void k_screen_prep(const ScreenArgs a) {
         else if (thr == INFINITY) decided = 1;    // every distance passes
       }
     }
-    a.thr[e] = thr;
-    if (decided >= 0) {
-      fe_screen_emit(a, e, L, decided == 1, r);
-      if (decided == 1) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
-      a.skey[e] = 0xFFFFFFFFu;  // sorts after every open candidate
-    } else {
-      atomicAdd(a.n_open, 1);
-      // 254 at most: the key 0xFFFFFFFF marks a decided candidate
-      const int tb = (int)fmin(fmax(floor(log2(thr)) + 128.0, 0.0), 254.0);
-      a.skey[e] = ((uint32_t)tb << 24) | fe_morton24(r, a.lo, a.kscale);
+    if (decided < 0) {
+      // FE_SENTINELS points spread over the Morton order: one that passes
+      // (rules 0, 1) or fails (rule 2) settles the candidate here, which is
+      // most of them; the box walk gets the rest
+      const double Px = r[0], Py = r[1], Pz = r[2];
+      const int ns = min(a.n_pts, FE_SENTINELS);
+      for (int k = 0; k < ns; ++k) {
+        const int q = (int)(((int64_t)k * a.n_pts) / ns);
+        const double x = a.pts[3 * (size_t)q] - Px;
+        const double y = a.pts[3 * (size_t)q + 1] - Py;
+        const double z = a.pts[3 * (size_t)q + 2] - Pz;
+        if ((fma(x, x, fma(y, y, z * z)) <= thr) == any_mode) {
+          decided = any_mode ? 1 : 0;
+          break;
+        }
+      }
     }
-    a.sval[e] = (int32_t)e;
+    a.thr[e] = decided >= 0 ? -1.0 : thr;
+    a.key[e] = fe_pair_key(a, L, decided == 0, r);
+    a.val[e] = (int32_t)e;
+    if (decided >= 0) {
+      fe_emit_flag(a, e, L, decided == 1);
+      if (decided == 1) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
+    }
   }
   fe_flush_counts(s_cnt, a.stats);
 }
 
-// The open candidates, one wavefront each (persistent: wavefront w takes the
-// sorted positions w, w + W, ... below the open count).  The rules 0, 1 keep
-// a candidate iff some point passes the test (the event is a pass); rule 2 iff
-// every point passes (the event is a fail); without an event the decision is
-// the opposite.  The lanes take 64 boxes at a time: a box whose distance
-// bounds settle the test for all its points decides at once; each lane whose
-// box is ambiguous tests that box's points itself.  So no candidate's work
-// runs on a single lane, and the slowest candidate costs a few box steps.
+// The open candidates in pair order, one wavefront each (persistent: wavefront
+// w takes the sorted positions w, w + W, ..., reading 64 of them at a time and
+// skipping the decided ones).  The rules 0, 1 keep a candidate iff some point
+// passes the test (the event is a pass); rule 2 iff every point passes (the
+// event is a fail); without an event the decision is the opposite.  The lanes
+// take 64 boxes at a time: a box whose distance bounds settle the test for all
+// its points decides at once; each lane whose box is ambiguous tests that
+// box's points itself.  So no candidate's work runs on a single lane, and the
+// slowest candidate costs a few box steps.  A dropped candidate gets the
+// dropped flag in its (sorted) key.
 __global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
   __shared__ int s_cnt[FE_ST_N];
   if (threadIdx.x < FE_ST_N) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int n_open = *a.n_open;
-  const int W = (int)gridDim.x * 4;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t w = (int64_t)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   const bool any_mode = a.rule != 2;
-  for (int i = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < n_open; i += W) {
-    const int64_t e = a.sval[i];
-    const int L = fe_class_of(a.ci, e);
-    const double* r = a.cand[L] + (size_t)(e - a.ci.cand_off[L]) * a.ci.stride[L];
-    const double Px = r[0], Py = r[1], Pz = r[2];
-    const double thr = a.thr[e];  // fe_pass(d2) == (d2 <= thr)
-    bool keep = !any_mode;
-    for (int b0 = 0; b0 < a.n_box; b0 += 64) {
-      const int b = b0 + lane;
-      const bool vb = b < a.n_box;
-      const double* bx = a.box + 6 * (size_t)(vb ? b : 0);
-      // bounds of every point's squared distance, widened by 1e-12 relative
-      // so that rounding cannot move a point across them
-      const double gx = fmax(fmax(bx[0] - Px, Px - bx[3]), 0.0);
-      const double gy = fmax(fmax(bx[1] - Py, Py - bx[4]), 0.0);
-      const double gz = fmax(fmax(bx[2] - Pz, Pz - bx[5]), 0.0);
-      const double hx = fmax(Px - bx[0], bx[3] - Px);
-      const double hy = fmax(Py - bx[1], bx[4] - Py);
-      const double hz = fmax(Pz - bx[2], bx[5] - Pz);
-      const double lo2 = fma(gx, gx, fma(gy, gy, gz * gz)) * (1.0 - 1e-12);
-      const double hi2 = fma(hx, hx, fma(hy, hy, hz * hz)) * (1.0 + 1e-12) + 1e-300;
-      const bool pass_far = hi2 <= thr;   // every point of the box passes
-      const bool pass_near = lo2 <= thr;  // some point of the box may pass
-      // rules 0/1: a box whose every point passes keeps; rule 2: a box whose
-      // every point fails drops
-      if (__any(vb && (any_mode ? pass_far : !pass_near))) {
-        keep = any_mode;
-        break;
-      }
-      const bool amb = vb && pass_near && !pass_far;
-      if (!__any(amb)) continue;
-      bool hit = false;  // a point of this lane's box passes (rules 0/1) or fails (rule 2)
-      if (amb) {
-        const int q0 = b * FE_BOX, q1 = min(a.n_pts, q0 + FE_BOX);
+  for (int64_t k0 = 0; w + k0 * W < a.n_all; k0 += 64) {
+    const int64_t my_pos = w + (k0 + lane) * W;
+    int my_e = 0;
+    bool open = false;
+    if (my_pos < a.n_all) {
+      my_e = a.val[my_pos];
+      open = a.thr[my_e] >= 0.0;
+    }
+    uint64_t todo = __ballot(open);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t e = __builtin_amdgcn_readlane(my_e, j);
+      const int64_t pos = w + (k0 + j) * W;
+      const int L = fe_class_of(a.ci, e);
+      const double* r = a.cand[L] + (size_t)(e - a.ci.cand_off[L]) * a.ci.stride[L];
+      const double Px = r[0], Py = r[1], Pz = r[2];
+      const double thr = a.thr[e];  // fe_pass(d2) == (d2 <= thr)
+      bool keep = !any_mode;
+      for (int b0 = 0; b0 < a.n_box; b0 += 64) {
+        const int b = b0 + lane;
+        const bool vb = b < a.n_box;
+        const double* bx = a.box + 6 * (size_t)(vb ? b : 0);
+        // bounds of every point's squared distance, widened by 1e-12 relative
+        // so that rounding cannot move a point across them
+        const double gx = fmax(fmax(bx[0] - Px, Px - bx[3]), 0.0);
+        const double gy = fmax(fmax(bx[1] - Py, Py - bx[4]), 0.0);
+        const double gz = fmax(fmax(bx[2] - Pz, Pz - bx[5]), 0.0);
+        const double hx = fmax(Px - bx[0], bx[3] - Px);
+        const double hy = fmax(Py - bx[1], bx[4] - Py);
+        const double hz = fmax(Pz - bx[2], bx[5] - Pz);
+        const double lo2 = fma(gx, gx, fma(gy, gy, gz * gz)) * (1.0 - 1e-12);
+        const double hi2 = fma(hx, hx, fma(hy, hy, hz * hz)) * (1.0 + 1e-12) + 1e-300;
+        const bool pass_far = hi2 <= thr;   // every point of the box passes
+        const bool pass_near = lo2 <= thr;  // some point of the box may pass
+        // rules 0/1: a box whose every point passes keeps; rule 2: a box whose
+        // every point fails drops
+        if (__any(vb && (any_mode ? pass_far : !pass_near))) {
+          keep = any_mode;
+          break;
+        }
+        const bool amb = vb && pass_near && !pass_far;
+        if (!__any(amb)) continue;
+        bool hit = false;  // a point of this lane's box passes (rules 0/1) or fails (rule 2)
+        if (amb) {
+          const int q0 = b * FE_BOX, q1 = min(a.n_pts, q0 + FE_BOX);
 #pragma unroll 8
-        for (int q = q0; q < q1; ++q) {
-          const double x = a.pts[3 * (size_t)q] - Px;
-          const double y = a.pts[3 * (size_t)q + 1] - Py;
-          const double z = a.pts[3 * (size_t)q + 2] - Pz;
-          hit |= (fma(x, x, fma(y, y, z * z)) <= thr) == any_mode;
+          for (int q = q0; q < q1; ++q) {
+            const double x = a.pts[3 * (size_t)q] - Px;
+            const double y = a.pts[3 * (size_t)q + 1] - Py;
+            const double z = a.pts[3 * (size_t)q + 2] - Pz;
+            hit |= (fma(x, x, fma(y, y, z * z)) <= thr) == any_mode;
+          }
+        }
+        if (__any(hit)) {
+          keep = any_mode;
+          break;
         }
       }
-      if (__any(hit)) {
-        keep = any_mode;
-        break;
+      if (lane == 0) {
+        fe_emit_flag(a, e, L, keep);
+        if (keep) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
+        else a.key[pos] |= 1u << 28;
       }
-    }
-    if (lane == 0) {
-      fe_screen_emit(a, e, L, keep, r);
-      if (keep) atomicAdd(&s_cnt[FE_ST_KEPT + L], 1);
     }
   }
   fe_flush_counts(s_cnt, a.stats);
@@ -488,17 +486,17 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, const int
   if (i < n) a[i] = v;
 }
 
-// Sum rule in two fixed-order steps: block b adds the shell-pair partials
+// Sum rule in two fixed-order steps: block b adds the candidates' terms
 // [b * per, (b + 1) * per) (k_srule_part), then one block adds the block sums
 // and copies the counters into the table's statistics (k_build_stats).
 #define FE_SRULE_BLOCKS 256
-__global__ __launch_bounds__(256) void k_srule_part(const double* __restrict__ srule, const int nsp,
-                                                    double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_srule_part(const double* __restrict__ srule,
+                                                    const int64_t n, double* __restrict__ part) {
   __shared__ double s_w[4];
-  const int per = (nsp + FE_SRULE_BLOCKS - 1) / FE_SRULE_BLOCKS;
-  const int i0 = blockIdx.x * per, i1 = min(nsp, i0 + per);
+  const int64_t per = (n + FE_SRULE_BLOCKS - 1) / FE_SRULE_BLOCKS;
+  const int64_t i0 = blockIdx.x * per, i1 = std::min(n, i0 + per);
   double s = 0.0;
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) s += srule[i];
+  for (int64_t i = i0 + (int)threadIdx.x; i < i1; i += 256) s += srule[i];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -518,43 +516,70 @@ __global__ __launch_bounds__(256) void k_build_stats(const double* __restrict__ 
   if (threadIdx.x < FE_ST_N && stats) stats[threadIdx.x] = ws_stats[threadIdx.x];
 }
 
+// The compaction of every class in three launches (slot map cleared, records
+// copied, derived parts filled), each class a range of the launch's grid.
+struct CompactArgs {
+  const double* cand[FE_LMAX + 1];
+  double* rec[FE_LMAX + 1];
+  int* flatoff[FE_LMAX + 1];
+  int32_t* orig[FE_LMAX + 1];
+  int* slot2kept[FE_LMAX + 1];
+  int64_t cap[FE_LMAX + 1], ncand[FE_LMAX + 1], cand_off[FE_LMAX + 1];
+  int64_t e_off[FE_LMAX + 2];  // copied doubles: class L covers [e_off[L], e_off[L+1])
+  int s_blk[FE_LMAX + 2];      // slot-map blocks of class L: [s_blk[L], s_blk[L+1])
+  int r_blk[FE_LMAX + 2];      // record blocks of class L: [r_blk[L], r_blk[L+1])
+  const int32_t* sorted;       // candidate indices, each class's kept ones first
+  const int* gidx;
+  const unsigned long long* gscan;
+  const int* stats;            // the kept counts
+};
+
+__device__ __forceinline__ int fe_range_of(const int* off, const int b) {
+  int L = 0;
+  while (L < FE_LMAX && b >= off[L + 1]) ++L;
+  return L;
+}
+
+__global__ __launch_bounds__(256) void k_slotmap_clear(const CompactArgs a) {
+  const int L = fe_range_of(a.s_blk, (int)blockIdx.x);
+  const int64_t i = (int64_t)(blockIdx.x - a.s_blk[L]) * 256 + threadIdx.x;
+  if (i < a.ncand[L]) a.slot2kept[L][i] = -1;
+}
+
 // Kept record k of class L: candidate sorted[k] (k < the kept count), with
 // slots 6/7 holding the far-field constants 1/(2p) and K sqrt(pi/p)/2 instead
 // of reff and the flags; column 0's thread also writes the flat term offset,
 // the reference-order index and the slot -> record map.
-__global__ __launch_bounds__(256) void k_compact(const double* __restrict__ cand, const int CS,
-                                                 const int S, const int64_t slot_base,
-                                                 const int32_t* __restrict__ sorted,
-                                                 const int* __restrict__ kept_p, const int cap,
-                                                 const int* __restrict__ gidx,
-                                                 const unsigned long long* __restrict__ gscan,
-                                                 double* __restrict__ rec, int* __restrict__ flatoff,
-                                                 int32_t* __restrict__ orig,
-                                                 int* __restrict__ slot2kept) {
-  const int n = min(*kept_p, cap);
+__global__ __launch_bounds__(256) void k_compact(const CompactArgs a) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)n * CS) return;
-  const int r = (int)(e / CS), c = (int)(e - (int64_t)r * CS);
-  const int ge = sorted[r];
-  const int slot = (int)(ge - slot_base);
-  const double* s = cand + (size_t)slot * CS;
-  rec[(size_t)r * S + c] =
+  if (e >= a.e_off[FE_LMAX + 1]) return;
+  int L = 0;
+  while (L < FE_LMAX && e >= a.e_off[L + 1]) ++L;
+  const int CS = fe_cstride(L), S = fe_stride(L);
+  const int64_t el = e - a.e_off[L];
+  const int n = min(a.stats[FE_ST_KEPT + L], (int)a.cap[L]);
+  if (el >= (int64_t)n * CS) return;
+  const int r = (int)(el / CS), c = (int)(el - (int64_t)r * CS);
+  const int ge = a.sorted[a.cand_off[L] + r];
+  const int slot = (int)(ge - a.cand_off[L]);
+  const double* s = a.cand[L] + (size_t)slot * CS;
+  a.rec[L][(size_t)r * S + c] =
       c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
   if (c == 0) {
-    const unsigned long long g = gscan[gidx[ge]];
-    flatoff[r] = (int)(g >> 32);
-    orig[r] = (int32_t)(g & 0xffffffffull);
-    slot2kept[slot] = r;
+    const unsigned long long g = a.gscan[a.gidx[ge]];
+    a.flatoff[L][r] = (int)(g >> 32);
+    a.orig[L][r] = (int32_t)(g & 0xffffffffull);
+    a.slot2kept[L][slot] = r;
   }
 }
 
-// the derived parts of each kept record of class L (one lane per record): the
-// scaled coefficients c g_k and the far-field block
-__global__ __launch_bounds__(256) void k_fill_record(double* __restrict__ rec, const int S,
-                                                     const int* __restrict__ kept_p, const int cap,
-                                                     const int L) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < min(*kept_p, cap)) fe_fill_record_any(L, rec + (size_t)i * S);
+// the derived parts of each kept record (one lane per record): the scaled
+// coefficients c g_k and the far-field block
+__global__ __launch_bounds__(256) void k_fill_record(const CompactArgs a) {
+  const int L = fe_range_of(a.r_blk, (int)blockIdx.x);
+  const int i = (int)(blockIdx.x - a.r_blk[L]) * 256 + (int)threadIdx.x;
+  if (i < min(a.stats[FE_ST_KEPT + L], (int)a.cap[L]))
+    fe_fill_record_any(L, a.rec[L] + (size_t)i * fe_stride(L));
 }
 
 // ---------------------------------------------------------------------------
@@ -595,7 +620,7 @@ int check_build_args(int n_points, const double* pts, const double* dens, double
 // boxes, sort and scan scratch); host inputs also stage the density and points.
 struct BuildLayout {
   size_t o_dens = 0, o_pts = 0, o_cand[FE_LMAX + 1] = {}, o_gidx = 0, o_key = 0, o_val = 0,
-         o_gflag = 0, o_skey = 0, o_sval = 0, o_thr = 0, o_srule = 0, o_stats = 0, o_spart = 0,
+         o_gflag = 0, o_thr = 0, o_srule = 0, o_stats = 0, o_spart = 0,
          o_box = 0, o_psort = 0, sort_pts = 0, o_sort = 0, total = 0;
 };
 
@@ -622,14 +647,10 @@ static BuildLayout build_layout(const fe_table* t, int n_points, bool host_in) {
   wo += al256(na * sizeof(int32_t));
   y.o_gflag = wo;
   wo += al256(na * sizeof(unsigned long long));
-  y.o_skey = wo;
-  wo += al256(na * sizeof(uint32_t));
-  y.o_sval = wo;
-  wo += al256(na * sizeof(int32_t));
   y.o_thr = wo;
   wo += al256(na * sizeof(double));
-  y.o_srule = wo;
-  wo += al256((size_t)std::max(t->n_sp, 1) * sizeof(double));
+  y.o_srule = wo;  // per candidate
+  wo += al256(na * sizeof(double));
   y.o_stats = wo;  // FE_ST_N counters, the open count, the sum rule
   wo += al256(FE_ST_SUMRULE_BYTE + sizeof(double));
   y.o_spart = wo;
@@ -648,13 +669,29 @@ static BuildLayout build_layout(const fe_table* t, int n_points, bool host_in) {
   return y;
 }
 
+// The two candidate launches, each over its shell-l pairs (c = LI (LI + 1) / 2
+// + LJ) with candidates, heavy-first
+CandLaunch cand_launch(const fe_table* t, const bool heavy) {
+  static const int kHeavy[] = {9, 8, 5, 7}, kLight[] = {6, 4, 3, 2, 1, 0};
+  CandLaunch pl{};
+  const int* order = heavy ? kHeavy : kLight;
+  const int n = heavy ? 4 : 6;
+  for (int k = 0; k < n; ++k) {
+    const int cb = order[k], m = t->cmb_off[cb + 1] - t->cmb_off[cb];
+    if (m <= 0) continue;
+    pl.combo[pl.n_r] = cb;
+    pl.blk_off[pl.n_r + 1] = pl.blk_off[pl.n_r] + (m + 255) / 256;
+    ++pl.n_r;
+  }
+  return pl;
+}
+
 // The build proper.  `ws` / `ws_bytes`: a caller-owned workspace of at least
 // fe_build_ws_bytes bytes (device inputs only), else the context's.
 int build_on_device(fe_table* t, const double* dens, int n_points, const double* pts, double tol,
                     int rule, int ptr_kind, hipStream_t st, bool sized, char* ws = nullptr,
                     size_t ws_bytes = 0) {
   fe_ctx* c = t->ctx;
-  const int nsp = t->n_sp;
   ClassInfo ci{};
   for (int L = 0; L <= FE_LMAX; ++L) {
     ci.stride[L] = fe_host::CSTRIDE[L];
@@ -667,7 +704,7 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   const int n_box = (n_points + FE_BOX - 1) / FE_BOX;
   const BuildLayout y = build_layout(t, n_points, host_in);
   const size_t o_dens = y.o_dens, o_pts = y.o_pts, o_gidx = y.o_gidx, o_key = y.o_key,
-               o_val = y.o_val, o_gflag = y.o_gflag, o_skey = y.o_skey, o_sval = y.o_sval,
+               o_val = y.o_val, o_gflag = y.o_gflag,
                o_thr = y.o_thr, o_srule = y.o_srule, o_stats = y.o_stats, o_spart = y.o_spart,
                o_box = y.o_box, o_psort = y.o_psort, sort_pts = y.sort_pts, o_sort = y.o_sort;
   const size_t* o_cand = y.o_cand;
@@ -699,19 +736,30 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
 
   // ---- candidates
   BuildArgs ba{};
-  ba.sh_atom = t->b.atom; ba.sh_l = t->b.l; ba.sh_fn0 = t->b.fn0; ba.sh_nfn = t->b.nfn;
-  ba.sh_prim0 = t->b.prim0; ba.sh_nprim = t->b.nprim; ba.sh_c = t->b.c;
-  ba.pexp = t->b.pexp; ba.pcoef = t->b.pcoef;
-  ba.fn_lx = t->b.fn_lx; ba.fn_ly = t->b.fn_ly; ba.fn_lz = t->b.fn_lz;
+  ba.sh_l = t->b.l; ba.sh_fn0 = t->b.fn0; ba.sh_prim0 = t->b.prim0; ba.sh_nprim = t->b.nprim;
+  ba.sh_c = t->b.c; ba.pexp = t->b.pexp; ba.pcoef = t->b.pcoef; ba.fn_cperm = t->b.fn_cperm;
   ba.sp_i = t->b.sp_i; ba.sp_j = t->b.sp_j; ba.sp_slot = t->b.sp_slot; ba.sp_gidx = t->b.sp_gidx;
+  ba.cl_sp = t->b.cl_sp; ba.cl_s = t->b.cl_s;
+  for (int q = 0; q <= FE_NCOMBO; ++q) ba.cmb_off[q] = t->cmb_off[q];
   ba.dens = d_dens;
   ba.n_func = t->n_func;
   for (int L = 0; L <= FE_LMAX; ++L) ba.cand[L] = (double*)(wb + o_cand[L]);
   ba.gidx = (int*)(wb + o_gidx);
-  ba.sp_srule = (double*)(wb + o_srule);
-  ba.lt = c->ltab;
+  ba.cand_srule = (double*)(wb + o_srule);
   ba.ci = ci;
-  if (nsp > 0) hipLaunchKernelGGL(k_build_pairs, dim3(nsp), dim3(64), 0, st, ba);
+  // the heavy shell-l pairs on the context's second stream, beside the light
+  // ones and the point sort on the caller's stream
+  const CandLaunch pl_heavy = cand_launch(t, true), pl_light = cand_launch(t, false);
+  hipStream_t side = st;
+  if (pl_heavy.n_r) {
+    if (int rc = fe_fork(c, st, &side)) return rc;
+    hipLaunchKernelGGL(k_build_cands<true>, dim3((unsigned)pl_heavy.blk_off[pl_heavy.n_r]),
+                       dim3(256), 0, side, ba, pl_heavy);
+  }
+  if (pl_light.n_r)
+    hipLaunchKernelGGL(k_build_cands<false>, dim3((unsigned)pl_light.blk_off[pl_light.n_r]),
+                       dim3(256), 0, st, ba, pl_light);
+  FE_HIPCHK(hipGetLastError());
 
   // ---- points: Morton order, boxes of FE_BOX
   const double* sorted = nullptr;
@@ -721,6 +769,9 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
       return rc;
     hipLaunchKernelGGL(k_boxes, dim3(n_box), dim3(64), 0, st, sorted, n_points,
                        (double*)(wb + o_box));
+  }
+  if (pl_heavy.n_r) {
+    if (int rc = fe_join(c, st)) return rc;
   }
 
   // ---- screen, sort keys, reference-order flags
@@ -738,8 +789,6 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   sa.gflag = gflag;
   sa.key = key;
   sa.val = val;
-  sa.skey = (uint32_t*)(wb + o_skey);
-  sa.sval = (int32_t*)(wb + o_sval);
   sa.thr = (double*)(wb + o_thr);
   sa.pts = sorted;
   sa.n_pts = n_points;
@@ -750,23 +799,24 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   for (int a = 0; a < 3; ++a) sa.lo[a] = t->key_lo[a];
   sa.kscale = t->key_scale;
   sa.stats = ws_stats;
-  sa.n_open = ws_stats + FE_ST_N;
   sa.n_all = n_all;
   if (n_all > 0) {
     const dim3 g((unsigned)((n_all + 255) / 256));
     hipLaunchKernelGGL(k_screen_prep, g, dim3(256), 0, st, sa);
-    if (int rc = fe_radix_sort(sa.skey, sa.sval, (int)n_all, 32, wb + o_sort, st)) return rc;
+    // pair order; then the box walk in that order, which flags the keys of the
+    // candidates it drops; then a stable pass over the class and flag byte
+    if (int rc = fe_radix_sort(key, val, (int)n_all, 32, wb + o_sort, st)) return rc;
     // persistent: 16 workgroups (64 wavefronts) per CU walk the open candidates
     hipLaunchKernelGGL(k_screen_boxes, dim3((unsigned)std::max(1, c->n_cu * 16)), dim3(256), 0,
                        st, sa);
-    if (int rc = fe_radix_sort(key, val, (int)n_all, 32, wb + o_sort, st)) return rc;
+    if (int rc = fe_radix_sort_bits(key, val, (int)n_all, 24, 32, wb + o_sort, st)) return rc;
     if (int rc = fe_scan_excl_u64(gflag, gflag, (int)n_all, wb + o_sort, st)) return rc;
   }
   FE_HIPCHK(hipGetLastError());
 
   double* spart = (double*)(wb + o_spart);
-  hipLaunchKernelGGL(k_srule_part, dim3(FE_SRULE_BLOCKS), dim3(256), 0, st, ba.sp_srule, nsp,
-                     spart);
+  hipLaunchKernelGGL(k_srule_part, dim3(FE_SRULE_BLOCKS), dim3(256), 0, st, ba.cand_srule,
+                     n_all, spart);
   if (!sized) {
     // the one synchronisation of a build: the counts size the table
     hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(256), 0, st, spart, ws_stats, (int*)nullptr,
@@ -796,20 +846,31 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
                      t->d_sum_rule);
 
   // ---- compaction into the table, in key order
+  CompactArgs ca{};
   for (int L = 0; L <= FE_LMAX; ++L) {
-    if (!t->n_cand_L[L]) continue;
-    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((t->n_cand_L[L] + 255) / 256)), dim3(256), 0,
-                       st, t->d_slot2kept[L], t->n_cand_L[L], -1);
-    if (!t->cap_L[L]) continue;
-    const int S = fe_host::STRIDE[L], CS = fe_host::CSTRIDE[L];
-    const int* kept = t->d_stats + FE_ST_KEPT + L;
-    const int64_t ne = t->cap_L[L] * CS;
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st,
-                       ba.cand[L], CS, S, ci.cand_off[L], val + ci.cand_off[L], kept,
-                       (int)t->cap_L[L], ba.gidx, gflag, t->d_rec[L], t->d_flatoff[L],
-                       t->d_orig[L], t->d_slot2kept[L]);
-    hipLaunchKernelGGL(k_fill_record, dim3((unsigned)((t->cap_L[L] + 255) / 256)), dim3(256), 0,
-                       st, t->d_rec[L], S, kept, (int)t->cap_L[L], L);
+    const int64_t cap = t->n_cand_L[L] ? t->cap_L[L] : 0;
+    ca.cand[L] = ba.cand[L];
+    ca.rec[L] = t->d_rec[L];
+    ca.flatoff[L] = t->d_flatoff[L];
+    ca.orig[L] = t->d_orig[L];
+    ca.slot2kept[L] = t->d_slot2kept[L];
+    ca.cap[L] = cap;
+    ca.ncand[L] = t->n_cand_L[L];
+    ca.cand_off[L] = ci.cand_off[L];
+    ca.e_off[L + 1] = ca.e_off[L] + cap * fe_host::CSTRIDE[L];
+    ca.s_blk[L + 1] = ca.s_blk[L] + (int)((t->n_cand_L[L] + 255) / 256);
+    ca.r_blk[L + 1] = ca.r_blk[L] + (int)((cap + 255) / 256);
+  }
+  ca.sorted = val;
+  ca.gidx = ba.gidx;
+  ca.gscan = gflag;
+  ca.stats = t->d_stats;
+  if (ca.s_blk[FE_LMAX + 1] > 0)
+    hipLaunchKernelGGL(k_slotmap_clear, dim3((unsigned)ca.s_blk[FE_LMAX + 1]), dim3(256), 0, st, ca);
+  if (ca.e_off[FE_LMAX + 1] > 0) {
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)((ca.e_off[FE_LMAX + 1] + 255) / 256)), dim3(256),
+                       0, st, ca);
+    hipLaunchKernelGGL(k_fill_record, dim3((unsigned)ca.r_blk[FE_LMAX + 1]), dim3(256), 0, st, ca);
   }
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
@@ -838,7 +899,8 @@ struct BasisSetup {
   size_t bytes = 0;
   size_t o_atom = 0, o_l = 0, o_fn0 = 0, o_nfn = 0, o_p0 = 0, o_np = 0, o_c = 0, o_pexp = 0,
          o_pcoef = 0, o_lx = 0, o_ly = 0, o_lz = 0, o_spi = 0, o_spj = 0, o_sps = 0, o_spg = 0,
-         o_spc = 0, o_task = 0;
+         o_spc = 0, o_task = 0, o_cperm = 0, o_clsp = 0, o_cls = 0;
+  int cmb_off[FE_NCOMBO + 1] = {};  // build candidates of shell-l pair c: cl_*[off[c], off[c+1])
   int sp_cls_off[FE_LMAX + 2] = {};  // shell pairs of class L: sp_by_class[off[L], off[L+1])
   int task_cls_off[FE_LMAX + 2] = {};  // dL/dP tasks of class L: tasks[off[L], off[L+1])
   ~BasisSetup() {
@@ -872,6 +934,7 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
                 const double* fc, const double* pexp, const double* pcoef,
                 std::shared_ptr<BasisSetup>& out) {
   std::vector<HostShell> sh;
+  std::vector<int> cperm(n_func, 0);
   for (int i = 0; i < n_func;) {
     int j = i;
     while (j < n_func && fn_shell[j] == fn_shell[i]) ++j;
@@ -899,6 +962,22 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
           return fe_set_error(FE_ERR_FORMAT, "function %d: primitives differ within its shell", q);
       if (!(pexp[fn_off[q]] > 0.0))
         return fe_set_error(FE_ERR_FORMAT, "function %d: exponents must be > 0", q);
+    }
+    // every Cartesian component of the shell exactly once (in any order)
+    {
+      unsigned seen = 0;
+      for (int q = i; q < j; ++q) {
+        // canonical index: x-major, (l,0,0), (l-1,1,0), (l-1,0,1), ...
+        const bool neg = fn_lx[q] < 0 || fn_ly[q] < 0 || fn_lz[q] < 0;
+        const int lyz = fn_ly[q] + fn_lz[q];
+        const int cidx = neg ? 0 : lyz * (lyz + 1) / 2 + fn_lz[q];
+        if (neg || cidx >= s.nfn || (seen >> cidx & 1u))
+          return fe_set_error(FE_ERR_FORMAT,
+                              "shell at function %d: components are not the %d of l=%d", i, s.nfn,
+                              s.l);
+        seen |= 1u << cidx;
+        cperm[i + cidx] = q - i;
+      }
     }
     sh.push_back(s);
     i = j;
@@ -963,6 +1042,29 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
     }
   }
   bs->task_cls_off[FE_LMAX + 1] = (int)tasks.size();
+  // the build candidates grouped by shell-l pair (l_i >= l_j, either order of
+  // the two shells), reference order within a group
+  std::vector<int> cl_sp((size_t)gtot), cl_s((size_t)gtot);
+  {
+    auto combo = [&](int sp) {
+      const int la = sh[sp_i[sp]].l, lb = sh[sp_j[sp]].l;
+      const int hi = std::max(la, lb), lo = std::min(la, lb);
+      return hi * (hi + 1) / 2 + lo;
+    };
+    int64_t at[FE_NCOMBO + 1] = {};
+    for (int sp = 0; sp < nsp; ++sp)
+      at[combo(sp) + 1] += (int64_t)sh[sp_i[sp]].nprim * sh[sp_j[sp]].nprim;
+    for (int cb = 0; cb < FE_NCOMBO; ++cb) at[cb + 1] += at[cb];
+    for (int cb = 0; cb <= FE_NCOMBO; ++cb) bs->cmb_off[cb] = (int)at[cb];
+    for (int sp = 0; sp < nsp; ++sp) {
+      int64_t& o = at[combo(sp)];
+      const int np = sh[sp_i[sp]].nprim * sh[sp_j[sp]].nprim;
+      for (int q = 0; q < np; ++q, ++o) {
+        cl_sp[o] = sp;
+        cl_s[o] = q;
+      }
+    }
+  }
   bs->device = c->device;
   bs->n_func = n_func;
   bs->n_prim = n_prim;
@@ -1010,6 +1112,9 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
   bs->o_spg = push_bytes(hb, sp_gidx.data(), nsp);
   bs->o_spc = push_bytes(hb, sp_cls.data(), nsp);
   bs->o_task = push_bytes(hb, tasks.data(), tasks.size());
+  bs->o_cperm = push_bytes(hb, cperm.data(), cperm.size());
+  bs->o_clsp = push_bytes(hb, cl_sp.data(), cl_sp.size());
+  bs->o_cls = push_bytes(hb, cl_s.data(), cl_s.size());
   if (hipMalloc(&bs->mem, hb.size()) != hipSuccess) {
     (void)hipGetLastError();
     bs->mem = nullptr;
@@ -1093,6 +1198,10 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
   t->b.sp_slot = (const int*)(bm + bs->o_sps); t->b.sp_gidx = (const int*)(bm + bs->o_spg);
   t->b.sp_by_class = (const int*)(bm + bs->o_spc);
   t->b.tasks = (const uint32_t*)(bm + bs->o_task);
+  t->b.fn_cperm = (const int*)(bm + bs->o_cperm);
+  t->b.cl_sp = (const int*)(bm + bs->o_clsp);
+  t->b.cl_s = (const int*)(bm + bs->o_cls);
+  for (int q = 0; q <= FE_NCOMBO; ++q) t->cmb_off[q] = bs->cmb_off[q];
   for (int L = 0; L <= FE_LMAX + 1; ++L) t->sp_cls_off[L] = bs->sp_cls_off[L];
   for (int L = 0; L <= FE_LMAX + 1; ++L) t->task_cls_off[L] = bs->task_cls_off[L];
   auto fail = [&](int rc) {

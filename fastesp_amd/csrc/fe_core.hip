@@ -124,57 +124,6 @@ static std::vector<double> make_boys_tables() {
 }
 
 // ---------------------------------------------------------------------------
-// per-L tables on the device
-// ---------------------------------------------------------------------------
-static int upload_ltab(fe_ctx* c) {
-  using namespace fe_host;
-  const int nkeys = KEY_OFF[FE_LMAX + 1];
-  const int nmono = MONO_OFF[FE_LMAX + 1];
-  const int nrow = (int)(sizeof(M0_ROWPTR) / sizeof(int));
-  const int nnz = (int)(sizeof(M0_TIDX) / sizeof(int));
-  std::vector<int> iv;
-  auto put = [&](const int* p, int n) {
-    const size_t o = iv.size();
-    iv.insert(iv.end(), p, p + n);
-    return o;
-  };
-  std::vector<int> key_m(nkeys), tl(3 * nmono);
-  for (int k = 0; k < nkeys; ++k) key_m[k] = KEYS[k][3];
-  for (int m = 0; m < nmono; ++m)
-    for (int a = 0; a < 3; ++a) tl[3 * m + a] = MONOS[m][a];
-  const size_t o_koff = put(KEY_OFF, FE_LMAX + 2);
-  const size_t o_km = put(key_m.data(), nkeys);
-  const size_t o_rp = put(M0_ROWPTR, nrow);
-  const size_t o_ti = put(M0_TIDX, nnz);
-  const size_t o_mo = put(MONO_OFF, FE_LMAX + 2);
-  const size_t o_tl = put(tl.data(), 3 * nmono);
-  const int ncp = (int)(sizeof(M0T_COLPTR) / sizeof(int));
-  const int nnzt = (int)(sizeof(M0T_K) / sizeof(int));
-  const size_t o_cp = put(M0T_COLPTR, ncp);
-  const size_t o_tk = put(M0T_K, nnzt);
-  const size_t ibytes = ((iv.size() * sizeof(int)) + 255) & ~(size_t)255;
-  const size_t bytes = ibytes + (nnz + nnzt) * sizeof(double);
-  FE_HIPCHK(hipMalloc(&c->d_ltab_mem, bytes));
-  FE_HIPCHK(hipMemcpy(c->d_ltab_mem, iv.data(), iv.size() * sizeof(int), hipMemcpyHostToDevice));
-  FE_HIPCHK(hipMemcpy((char*)c->d_ltab_mem + ibytes, M0_VAL, nnz * sizeof(double),
-                      hipMemcpyHostToDevice));
-  FE_HIPCHK(hipMemcpy((char*)c->d_ltab_mem + ibytes + nnz * sizeof(double), M0T_VAL,
-                      nnzt * sizeof(double), hipMemcpyHostToDevice));
-  const int* ib = (const int*)c->d_ltab_mem;
-  c->ltab.key_off = ib + o_koff;
-  c->ltab.key_m = ib + o_km;
-  c->ltab.m0_rowptr = ib + o_rp;
-  c->ltab.m0_tidx = ib + o_ti;
-  c->ltab.mono_off = ib + o_mo;
-  c->ltab.tlist = ib + o_tl;
-  c->ltab.m0_val = (const double*)((const char*)c->d_ltab_mem + ibytes);
-  c->ltab.m0t_colptr = ib + o_cp;
-  c->ltab.m0t_k = ib + o_tk;
-  c->ltab.m0t_val = c->ltab.m0_val + nnz;
-  return FE_OK;
-}
-
-// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes) {
@@ -290,15 +239,23 @@ extern "C" int fe_init(int device, fe_ctx** out) {
     delete c;
     return fe_set_error(FE_ERR_HIP, "fe_init: Boys table upload failed");
   }
-  if (int rc = upload_ltab(c)) {
-    if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
+  c->target_wg = 16 * c->n_cu;
+  // the second stream of the context (table build, dL/dG classes), created
+  // here so that no later call creates one (say, while a graph is captured)
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side) (void)hipStreamDestroy(c->side);
     (void)hipFree(c->d_boys);
     delete c;
-    return rc;
+    return fe_set_error(FE_ERR_HIP, "fe_init: second stream creation failed");
   }
-  c->target_wg = 16 * c->n_cu;
   if (int rc = fe_noalloc_init(c)) {
-    (void)hipFree(c->d_ltab_mem);
+    (void)hipEventDestroy(c->ev_join);
+    (void)hipEventDestroy(c->ev_fork);
+    (void)hipStreamDestroy(c->side);
     (void)hipFree(c->d_boys);
     delete c;
     return rc;
@@ -321,7 +278,6 @@ extern "C" int fe_finalize(fe_ctx* c) {
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   c->basis_cache.reset();
   if (c->d_boys) (void)hipFree(c->d_boys);
-  if (c->d_ltab_mem) (void)hipFree(c->d_ltab_mem);
   for (FeProfEv& e : c->prof) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);

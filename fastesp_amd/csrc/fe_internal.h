@@ -68,13 +68,15 @@ int fe_set_error(int code, const char* fmt, ...);
 
 // Device primitives (fe_prim.hip): deterministic, no memsets, no host
 // synchronisation.  Stable LSD radix sort of (key, value) pairs on the low
-// key_bits bits, in place, with fe_radix_sort_scratch(n) bytes of scratch;
-// exclusive prefix sums (out may equal in) with fe_scan_scratch(n) bytes.
+// key_bits bits (or on the bits [lo_bit, hi_bit), the others left as they
+// are), in place, with fe_radix_sort_scratch(n) bytes of scratch; exclusive
+// prefix sums (out may equal in) with fe_scan_scratch(n) bytes.
 size_t fe_radix_sort_scratch(int64_t n);
 int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scratch,
                   hipStream_t st);
+int fe_radix_sort_bits(uint32_t* keys, int32_t* vals, int n, int lo_bit, int hi_bit,
+                       void* scratch, hipStream_t st);
 size_t fe_scan_scratch(int64_t n);
-int fe_scan_excl_i32(const int* in, int* out, int n, void* scratch, hipStream_t st);
 int fe_scan_excl_u64(const unsigned long long* in, unsigned long long* out, int n,
                      void* scratch, hipStream_t st);
 int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes);
@@ -101,21 +103,6 @@ enum {
   FE_WS_SORT,
   FE_WS_BUILD,  // scratch of fe_table_build (kept for the next build)
   FE_WS_SLOTS
-};
-
-// device copies of the per-L tables of fe_lclass.inc (used by the table build
-// and the adjoint contraction)
-struct LTabDev {
-  const int* key_off;    // [FE_LMAX+2]
-  const int* key_m;      // Boys order of every key, KEYS order
-  const int* m0_rowptr;  // CSR rows = keys
-  const int* m0_tidx;    // simplex index of t
-  const double* m0_val;
-  const int* mono_off;   // [FE_LMAX+2]
-  const int* tlist;      // (tx, ty, tz) of every simplex entry, MONOS order
-  const int* m0t_colptr; // transpose: per simplex entry (MONOS order) its keys
-  const int* m0t_k;      // key index within the class
-  const double* m0t_val;
 };
 
 // phases timed by fe_profile (HIP events on the launch stream)
@@ -147,8 +134,6 @@ struct fe_ctx {
   int device = 0;
   int n_cu = 256;
   double* d_boys = nullptr;  // (FE_BOYS_MMAX+1) x NR x COLS
-  void* d_ltab_mem = nullptr;
-  LTabDev ltab{};
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
   void* h_pin = nullptr;  // pinned host staging of the table build (hipHostMalloc)
@@ -201,6 +186,9 @@ enum {
   FE_ST_SUMRULE_BYTE = 128           // byte offset of the sum rule (a double)
 };
 
+// shell-l pairs (LI >= LJ, LI <= 3) of the table build: c = LI (LI + 1) / 2 + LJ
+#define FE_NCOMBO 10
+
 struct fe_table {
   fe_ctx* ctx = nullptr;
   int n_pairs = 0;
@@ -244,7 +232,11 @@ struct fe_table {
     const int* sp_i; const int* sp_j; const int* sp_slot; const int* sp_gidx;
     const int* sp_by_class;  // shell-pair indices grouped by class L = l_i + l_j
     const uint32_t* tasks;   // dL/dP tasks (sp << 8 | mi << 4 | mj) grouped by class
+    const int* fn_cperm;     // function fn0 + c: offset in its shell of canonical component c
+    const int* cl_sp;        // build candidates grouped by shell-l pair: shell pair,
+    const int* cl_s;         //   primitive pair within it (i-major)
   } b{};
+  int cmb_off[FE_NCOMBO + 1] = {};  // shell-l pair (l_i >= l_j) c: cl_*[cmb_off[c], cmb_off[c+1])
   int sp_cls_off[FE_LMAX + 2] = {};  // class L: sp_by_class[sp_cls_off[L], sp_cls_off[L+1])
   int task_cls_off[FE_LMAX + 2] = {};  // class L: tasks[task_cls_off[L], task_cls_off[L+1])
   int64_t n_cand_L[FE_LMAX + 1] = {}; // candidate slots (every primitive pair) per class

@@ -7,13 +7,15 @@
 // rocPRIM clear their counters with hipMemsetAsync, which a captured graph
 // must not contain).
 //
-// Radix sort: 8-bit digits, one pass per digit.  A pass is a digit histogram
-// per tile of 1024 keys (LDS counters), an exclusive scan of the histograms in
-// digit-major order (digit d of tile b lands after every tile's smaller digits
-// and after digit d of tiles < b), and a scatter that ranks equal digits
-// within a tile in input order: per wavefront by ballots over the digit's
-// eight bits, across the four wavefronts and the four rounds of a tile by LDS
-// counters.  So the sort is stable, and the output does not depend on timing.
+// Radix sort: 8-bit digits, one pass per digit, three launches a pass.  A
+// pass is a digit histogram per tile of 1024 keys (LDS counters); an exclusive
+// scan of each digit's row of tile counts, one block per digit, which also
+// leaves the digit's total; and a scatter that starts digit d of tile b after
+// every key with a smaller digit (a block scan of the 256 totals) and after
+// digit d of the tiles < b, and ranks equal digits within a tile in input
+// order: per wavefront by ballots over the digit's eight bits, across the four
+// wavefronts and the four rounds of a tile by LDS counters.  So the sort is
+// stable, and the output does not depend on timing.
 #include <hip/hip_runtime.h>
 
 #include "fe_internal.h"
@@ -28,6 +30,24 @@ constexpr int SC_TILE = 256 * SC_ITEMS;
 size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 }  // namespace
+
+template <typename T>
+__device__ __forceinline__ T fe_block_excl_scan(const T v, T* s_w, T& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  T pre = 0;
+  for (int w = 0; w < wave; ++w) pre += s_w[w];
+  total = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  __syncthreads();
+  return pre + x - v;
+}
 
 // hist[d * gridDim.x + tile] = number of keys of the tile with digit d
 __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ keys, const int n,
@@ -45,17 +65,52 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* __restrict__ ke
   hist[threadIdx.x * gridDim.x + blockIdx.x] = s_cnt[threadIdx.x];
 }
 
-// stable scatter of one tile to the offsets of the scanned histogram
+// hist[d * nt + b] -> the count of digit d in tiles < b (exclusive scan of the
+// row, one block per digit d); tot[d] = the row's total
+__global__ __launch_bounds__(256) void k_rs_rowscan(int* __restrict__ hist, const int nt,
+                                                    int* __restrict__ tot) {
+  __shared__ int s_w[4];
+  int* row = hist + (size_t)blockIdx.x * nt;
+  int carry = 0;
+  for (int c0 = 0; c0 < nt; c0 += SC_TILE) {
+    // thread t holds the SC_ITEMS consecutive entries c0 + t*SC_ITEMS + j
+    int v[SC_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; ++j) {
+      const int i = c0 + (int)threadIdx.x * SC_ITEMS + j;
+      v[j] = i < nt ? row[i] : 0;
+      s += v[j];
+    }
+    int total;
+    int pre = fe_block_excl_scan<int>(s, s_w, total) + carry;
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; ++j) {
+      const int i = c0 + (int)threadIdx.x * SC_ITEMS + j;
+      if (i < nt) row[i] = pre;
+      pre += v[j];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// stable scatter of one tile: digit d starts after every key of a smaller
+// digit (exclusive scan of tot) and after digit d of the tiles before it
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                     const int32_t* __restrict__ vin, const int n,
                                                     const int shift,
-                                                    const int* __restrict__ hist_scan,
+                                                    const int* __restrict__ row_scan,
+                                                    const int* __restrict__ tot,
                                                     uint32_t* __restrict__ kout,
                                                     int32_t* __restrict__ vout) {
   __shared__ int s_base[256];   // next output position of digit d for this tile
   __shared__ int s_wc[4][256];  // keys of digit d in wavefront w of the current round
+  __shared__ int s_w[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  s_base[tid] = hist_scan[tid * gridDim.x + blockIdx.x];
+  int total;
+  s_base[tid] = fe_block_excl_scan<int>(tot[tid], s_w, total) +
+                row_scan[tid * gridDim.x + blockIdx.x];
 #pragma unroll
   for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
   __syncthreads();
@@ -106,24 +161,6 @@ __global__ __launch_bounds__(256) void k_copy_pairs(const T* __restrict__ a, con
 // ---------------------------------------------------------------------------
 // exclusive prefix sums (integer: any grouping gives the same result)
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T fe_block_excl_scan(const T v, T* s_w, T& total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  T x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const T y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_w[wave] = x;
-  __syncthreads();
-  T pre = 0;
-  for (int w = 0; w < wave; ++w) pre += s_w[w];
-  total = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
-  __syncthreads();
-  return pre + x - v;
-}
-
 // sums[b] = sum of tile b
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_reduce(const T* __restrict__ in, const int n,
@@ -141,39 +178,20 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const T* __restrict__ in, c
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// exclusive scan of sums[0..nb) in place by one block
-template <typename T>
-__global__ __launch_bounds__(256) void k_scan_top(T* __restrict__ sums, const int nb) {
-  __shared__ T s_w[4];
-  T carry = 0;
-  for (int c0 = 0; c0 < nb; c0 += SC_TILE) {
-    // thread t holds the SC_ITEMS consecutive entries c0 + t*SC_ITEMS + j
-    T v[SC_ITEMS];
-    T s = 0;
-#pragma unroll
-    for (int j = 0; j < SC_ITEMS; ++j) {
-      const int i = c0 + (int)threadIdx.x * SC_ITEMS + j;
-      v[j] = i < nb ? sums[i] : (T)0;
-      s += v[j];
-    }
-    T total;
-    T pre = fe_block_excl_scan<T>(s, s_w, total) + carry;
-#pragma unroll
-    for (int j = 0; j < SC_ITEMS; ++j) {
-      const int i = c0 + (int)threadIdx.x * SC_ITEMS + j;
-      if (i < nb) sums[i] = pre;
-      pre += v[j];
-    }
-    carry += total;
-  }
-}
-
-// out[i] = sums[tile] + exclusive prefix of in within the tile
+// out[i] = (sum of the tiles before this one) + exclusive prefix of in within
+// the tile; each block adds the tile sums before it itself (a few hundred)
 template <typename T>
 __global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, const int n,
                                                    const T* __restrict__ sums,
                                                    T* __restrict__ out) {
   __shared__ T s_w[4];
+  T before = 0;
+  for (int b = (int)threadIdx.x; b < (int)blockIdx.x; b += 256) before += sums[b];
+  {
+    T all;
+    (void)fe_block_excl_scan<T>(before, s_w, all);
+    before = all;
+  }
   const int b0 = blockIdx.x * SC_TILE + (int)threadIdx.x * SC_ITEMS;
   T v[SC_ITEMS];
   T s = 0;
@@ -183,7 +201,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const T* __restrict__ in, con
     s += v[j];
   }
   T total;
-  T pre = fe_block_excl_scan<T>(s, s_w, total) + sums[blockIdx.x];
+  T pre = fe_block_excl_scan<T>(s, s_w, total) + before;
 #pragma unroll
   for (int j = 0; j < SC_ITEMS; ++j) {
     if (b0 + j < n) out[b0 + j] = pre;
@@ -206,14 +224,9 @@ static int scan_excl(const T* in, T* out, int n, void* scratch, hipStream_t st) 
   const int nb = (n + SC_TILE - 1) / SC_TILE;
   T* sums = (T*)scratch;
   hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(256), 0, st, in, n, sums);
-  hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(256), 0, st, sums, nb);
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(256), 0, st, in, n, sums, out);
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
-}
-
-int fe_scan_excl_i32(const int* in, int* out, int n, void* scratch, hipStream_t st) {
-  return scan_excl<int>(in, out, n, scratch, st);
 }
 
 int fe_scan_excl_u64(const unsigned long long* in, unsigned long long* out, int n, void* scratch,
@@ -224,14 +237,14 @@ int fe_scan_excl_u64(const unsigned long long* in, unsigned long long* out, int 
 size_t fe_radix_sort_scratch(int64_t n) {
   const int64_t nt = std::max<int64_t>((n + RS_TILE - 1) / RS_TILE, 1);
   return 2 * al256((size_t)std::max<int64_t>(n, 1) * 4) + al256((size_t)256 * nt * sizeof(int)) +
-         fe_scan_scratch_t<int>(256 * nt);
+         al256(256 * sizeof(int));
 }
 
-int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scratch,
-                  hipStream_t st) {
+int fe_radix_sort_bits(uint32_t* keys, int32_t* vals, int n, int lo_bit, int hi_bit,
+                       void* scratch, hipStream_t st) {
   if (n <= 1) return FE_OK;
-  if (key_bits < 1 || key_bits > 32)
-    return fe_set_error(FE_ERR_ARG, "radix sort: key_bits %d out of range", key_bits);
+  if (lo_bit < 0 || hi_bit > 32 || hi_bit <= lo_bit)
+    return fe_set_error(FE_ERR_ARG, "radix sort: key bits [%d, %d) out of range", lo_bit, hi_bit);
   const int nt = (n + RS_TILE - 1) / RS_TILE;
   char* s = (char*)scratch;
   uint32_t* kalt = (uint32_t*)s;
@@ -240,17 +253,18 @@ int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scra
   s += al256((size_t)n * 4);
   int* hist = (int*)s;
   s += al256((size_t)256 * nt * sizeof(int));
-  void* scan_tmp = s;
-  const int passes = (key_bits + 7) / 8;
+  int* tot = (int*)s;
+  const int passes = (hi_bit - lo_bit + 7) / 8;
   uint32_t* ks = keys;
   int32_t* vs = vals;
   uint32_t* kd = kalt;
   int32_t* vd = valt;
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
+    const int shift = lo_bit + 8 * p;
     hipLaunchKernelGGL(k_rs_hist, dim3(nt), dim3(256), 0, st, ks, n, shift, hist);
-    if (int rc = scan_excl<int>(hist, hist, 256 * nt, scan_tmp, st)) return rc;
-    hipLaunchKernelGGL(k_rs_scatter, dim3(nt), dim3(256), 0, st, ks, vs, n, shift, hist, kd, vd);
+    hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, st, hist, nt, tot);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nt), dim3(256), 0, st, ks, vs, n, shift, hist, tot, kd,
+                       vd);
     std::swap(ks, kd);
     std::swap(vs, vd);
   }
@@ -259,4 +273,11 @@ int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scra
                        keys, vals);
   FE_HIPCHK(hipGetLastError());
   return FE_OK;
+}
+
+int fe_radix_sort(uint32_t* keys, int32_t* vals, int n, int key_bits, void* scratch,
+                  hipStream_t st) {
+  if (key_bits < 1 || key_bits > 32)
+    return fe_set_error(FE_ERR_ARG, "radix sort: key_bits %d out of range", key_bits);
+  return fe_radix_sort_bits(keys, vals, n, 0, key_bits, scratch, st);
 }

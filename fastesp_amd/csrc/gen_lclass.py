@@ -626,6 +626,7 @@ def gen_L(L):
         o.append(f"  G{d} = {e};")
     o.append("}")
     o.extend(gen_det(L, keys, acc, order))
+    o.extend(gen_g_from_etab(L, keys, acc, order))
     return o, keys, order, parent, axis, aidx
 
 
@@ -657,6 +658,203 @@ def gen_det(L, keys, acc, order):
     return o
 
 
+def cart_order(l):
+    """The Cartesian components of an l-shell, x-major ((l,0,0), (l-1,1,0), ...)."""
+    return [(lx, ly, l - lx - ly) for lx in range(l, -1, -1) for ly in range(l - lx, -1, -1)]
+
+
+def _fac(n):
+    f = 1
+    for k in range(2, n + 1):
+        f *= k
+    return f
+
+
+def _binom(n, k):
+    return _fac(n) // (_fac(k) * _fac(n - k))
+
+
+def _lit(v):
+    return repr(float(v))
+
+
+def gen_pair_etab(LI, LJ):
+    """fe_pair_etab<LI, LJ>: etab over the |t| <= L simplex of one primitive pair
+    from its folded density block, by polynomial algebra instead of the 1-D
+    Hermite recurrences.
+
+    With u = r - P, (x - A)^a = sum_al C(a, al) PA^(a - al) u^al, so the block's
+    polynomial sum_ij w_ij (r - A)^a_i (r - B)^b_j is a polynomial D(u) of
+    degree L (D_n over the simplex), formed as sum_i Phi_i(u) Q_i(u) with
+    Q_i = sum_j w_ij Psi_j.  A monomial u^n e^{-p u^2} expands in Hermite
+    Gaussians with E^n_t = n! / (t! k!) (1/(4p))^k (1/(2p))^t for n = t + 2k
+    (the E recurrence at PA = 0), so etab_t = cc sum_k D_{t+2k} prod_axes of
+    those.  w is the block in the canonical component order of both shells
+    (i-major), PA = P - A (shell i), PB = P - B (shell j), h = 1/(2p)."""
+    L = LI + LJ
+    ci, cj = cart_order(LI), cart_order(LJ)
+    NJ = len(cj)
+    simplex = sorted((a, b, c) for a in range(L + 1) for b in range(L + 1 - a)
+                     for c in range(L + 1 - a - b))
+    tpos = {t: i for i, t in enumerate(simplex)}
+    qmon = [m for m in simplex if sum(m) <= LJ]
+    o = [f"template <> __device__ __forceinline__ void fe_pair_etab<{LI}, {LJ}>("
+         "const double* __restrict__ w, const double pax, const double pay, const double paz, "
+         "const double pbx, const double pby, const double pbz, const double h, const double cc, "
+         "double* __restrict__ et) {"]
+    ax = "xyz"
+    for d in range(3):
+        for k in range(2, LI + 1):
+            o.append(f"  const double pa{ax[d]}{k} = pa{ax[d]}{k - 1 if k > 2 else ''} * pa{ax[d]};")
+        for k in range(2, LJ + 1):
+            o.append(f"  const double pb{ax[d]}{k} = pb{ax[d]}{k - 1 if k > 2 else ''} * pb{ax[d]};")
+
+    def pw(side, d, e):
+        return None if e == 0 else f"p{side}{ax[d]}{e if e > 1 else ''}"
+
+    def factor(side, comp, mon):
+        """prod_d C(comp_d, mon_d) P{side}_d^(comp_d - mon_d) as (integer, [powers])."""
+        c, ps = 1, []
+        for d in range(3):
+            c *= _binom(comp[d], mon[d])
+            p = pw(side, d, comp[d] - mon[d])
+            if p:
+                ps.append(p)
+        return c, ps
+
+    def prod_expr(c, ps):
+        if not ps:
+            return _lit(c)
+        e = " * ".join(ps)
+        return e if c == 1 else f"{_lit(c)} * {e}"
+
+    # Psi_j monomial coefficients (named when not a constant)
+    psi = {}
+    for j, bj in enumerate(cj):
+        for m in qmon:
+            if all(m[d] <= bj[d] for d in range(3)):
+                c, ps = factor("b", bj, m)
+                if ps and len(ps) + (c != 1) > 1:
+                    nm = f"s{j}_{tpos[m]}"
+                    o.append(f"  const double {nm} = {prod_expr(c, ps)};")
+                    psi[(j, m)] = (1, [nm])
+                else:
+                    psi[(j, m)] = (c, ps)
+    # per component i of shell A: Q_i over the monomials of degree <= LJ, then
+    # D_n += Phi_i[al] Q_i[n - al] (D held as running sums, so only one Q_i
+    # is live at a time)
+    started = set()
+    for i, a in enumerate(ci):
+        for m in qmon:
+            terms = []
+            for j in range(NJ):
+                if (j, m) in psi:
+                    c, ps = psi[(j, m)]
+                    wv = f"w[{i * NJ + j}]"
+                    terms.append(wv if (c == 1 and not ps) else
+                                 f"{wv} * {prod_expr(c, ps)}")
+            o.append(f"  const double q{i}_{tpos[m]} = " + (" + ".join(terms) if terms else "0.0") + ";")
+        dterms = {}
+        for al in simplex:
+            if not all(al[d] <= a[d] for d in range(3)):
+                continue
+            c, ps = factor("a", a, al)
+            if ps and len(ps) + (c != 1) > 1:
+                nm = f"f{i}_{tpos[al]}"
+                o.append(f"  const double {nm} = {prod_expr(c, ps)};")
+                fe = nm
+            else:
+                fe = prod_expr(c, ps)
+            for m in qmon:
+                n = tuple(al[d] + m[d] for d in range(3))
+                qv = f"q{i}_{tpos[m]}"
+                dterms.setdefault(n, []).append(qv if fe == "1.0" else f"{fe} * {qv}")
+        for n in simplex:
+            if n not in dterms:
+                continue
+            e = " + ".join(dterms[n])
+            if n in started:
+                o.append(f"  D{tpos[n]} += {e};")
+            else:
+                o.append(f"  double D{tpos[n]} = {e};")
+                started.add(n)
+    for n in simplex:
+        if n not in started:
+            o.append(f"  const double D{tpos[n]} = 0.0;")
+    # powers of h = 1/(2p)
+    for e in range(2, L + 1):
+        o.append(f"  const double h{e} = h{e - 1 if e > 2 else ''} * h;")
+
+    def hname(e):
+        return None if e == 0 else ("h" if e == 1 else f"h{e}")
+
+    for t in simplex:
+        by_e = {}
+        nt_ = sum(t)
+        for kx in range((L - nt_) // 2 + 1):
+            for ky in range((L - nt_) // 2 + 1 - kx):
+                for kz in range((L - nt_) // 2 + 1 - kx - ky):
+                    k = (kx, ky, kz)
+                    n = tuple(t[d] + 2 * k[d] for d in range(3))
+                    # E^n_t = n!/(t! k!) (1/(4p))^k (1/(2p))^t = n!/(t! k! 2^k) h^(t+k)
+                    c = 1.0
+                    for d in range(3):
+                        c *= _fac(n[d]) / (_fac(t[d]) * _fac(k[d]) * 2 ** k[d])
+                    by_e.setdefault(nt_ + sum(k), []).append((c, tpos[n]))
+        parts = []
+        for e, lst in sorted(by_e.items()):
+            s = " + ".join(f"D{q}" if c == 1.0 else f"{_lit(c)} * D{q}" for c, q in lst)
+            hn = hname(e)
+            parts.append(s if hn is None and len(lst) == 1 else
+                         f"({s})" if hn is None else f"{hn} * ({s})")
+        o.append(f"  et[{tpos[t]}] = cc * (" + " + ".join(parts) + ");")
+    o.append("}")
+    return o
+
+
+# \int |He_n(x)| e^{-x^2/2} dx, n = 0..11, as tabulated by the reference
+# (fastesp/esp.py:25-28): the screening bound's per-axis weights.
+MABS = [2.5066282746310002, 2.0, 1.5957691216057308, 4.0, 10.212920779565312,
+        33.63284766793385, 136.5963625280984, 665.1416330443616, 3768.863619586618,
+        25506.330595911374, 202331.81842358326, 1872154.2572862155]
+
+
+def gen_g_from_etab(L, keys, acc, order):
+    """fe_g_from_etab<L>: G_k = (2 pi/p)(2p)^m_k sum_t M0[k, t] etab_t for one
+    primitive pair (the etab columns in the simplex order), and
+    fe_etab_bound<L>: sum_t |etab_t| mabs(tx) mabs(ty) mabs(tz) (2p)^(|t|/2),
+    the screening bound before its (2p)^-3/2 factor (fastesp/esp.py:248-252)."""
+    T = L + 1
+    o = [f"template <> __device__ __forceinline__ void fe_g_from_etab<{L}>("
+         "const double* __restrict__ et, const double p, double* __restrict__ g) {"]
+    ms = sorted({k[3] for k in keys})
+    o.append("  const double tp = p + p;")
+    o.append("  const double w0 = 6.283185307179586 / p;")
+    for m in range(1, max(ms) + 1):
+        o.append(f"  const double w{m} = w{m - 1} * tp;")
+    pos = {((tx * T + ty) * T + tz): i for i, (tx, ty, tz) in enumerate(order)}
+    for k, key in enumerate(keys):
+        terms = [(pos[col], v) for col, v in sorted(acc[key].items()) if v != 0.0]
+        e = " + ".join((f"et[{q}]" if v == 1.0 else f"{v!r} * et[{q}]") for q, v in terms)
+        o.append(f"  g[{k}] = w{key[3]} * ({e if terms else '0.0'});")
+    o.append("}")
+    o.append(f"template <> __device__ __forceinline__ double fe_etab_bound<{L}>("
+             "const double* __restrict__ et, const double sq) {")
+    for e in range(2, L + 1):
+        o.append(f"  const double sq{e} = sq{e - 1 if e > 2 else ''} * sq;")
+    by_n = {}
+    for q, t in enumerate(order):
+        c = MABS[t[0]] * MABS[t[1]] * MABS[t[2]]
+        by_n.setdefault(sum(t), []).append(f"{c!r} * fabs(et[{q}])")
+    parts = []
+    for n, lst in sorted(by_n.items()):
+        s = " + ".join(lst)
+        parts.append(f"({s})" if n == 0 else f"{'sq' if n == 1 else f'sq{n}'} * ({s})")
+    o.append("  return " + " + ".join(parts) + ";")
+    o.append("}")
+    return o
+
+
 def main():
     lmax = int(sys.argv[1]) if len(sys.argv) > 1 else 6
     out = ["// GENERATED by gen_lclass.py -- do not edit by hand.",
@@ -681,6 +879,9 @@ def main():
            "template <int L> __device__ __forceinline__ void fe_far_grad(const double, const double, const double, fe_gdptr __restrict__, double&, double&, double&, double&, double&);",
            "template <int L> __device__ __forceinline__ double fe_far_value(const double, const double, const double, fe_gdptr __restrict__);",
            "template <int L> __device__ __forceinline__ void fe_det_from_dg(const double* __restrict__, const double, double* __restrict__);",
+           "template <int L> __device__ __forceinline__ void fe_g_from_etab(const double* __restrict__, const double, double* __restrict__);",
+           "template <int L> __device__ __forceinline__ double fe_etab_bound(const double* __restrict__, const double);",
+           "template <int LI, int LJ> __device__ __forceinline__ void fe_pair_etab(const double* __restrict__, const double, const double, const double, const double, const double, const double, const double, const double, double* __restrict__);",
            ""]
     host = ["// host-side canonical tables", "namespace fe_host {"]
     nts, nms, nds, koff, moff = [], [], [], [0], [0]
@@ -698,6 +899,22 @@ def main():
             allmono.append((*order[j], parent[j], axis[j]))
         koff.append(koff[-1] + len(keys))
         moff.append(moff[-1] + len(order))
+    out.append("// etab of a primitive pair from its folded density block, per shell-l pair")
+    out.append("// (l_i >= l_j; the build swaps the shells otherwise)")
+    for LI in range(min(3, lmax) + 1):
+        for LJ in range(LI + 1):
+            if LI + LJ <= lmax:
+                out.extend(gen_pair_etab(LI, LJ))
+    out.append("")
+    out.append("// record and candidate-record strides of a class known at run time")
+    for fn, field in (("fe_stride", "STRIDE"), ("fe_cstride", "KOFF")):
+        out.append(f"__host__ __device__ inline int {fn}(const int L) {{")
+        out.append("  switch (L) {")
+        for L in range(lmax + 1):
+            out.append(f"    case {L}: return FeL<{L}>::{field};")
+        out.append("    default: return 0;")
+        out.append("  }")
+        out.append("}")
     out.append("// the derived parts of a record of a class known at run time: the integer")
     out.append("// multiples c g_k (fe_fill_dcoef<L>) and the far-field block (fe_fill_kelvin<L>)")
     out.append("__host__ __device__ inline void fe_fill_record_any(const int L, double* r) {")
@@ -729,40 +946,6 @@ def main():
     host.append(f"static const int MONOS[{len(allmono)}][5] = {{")
     host.extend("  {" + ", ".join(map(str, k)) + "}," for k in allmono)
     host.append("};")
-    # sparse map etab_t -> G_k per class: rows in KEYS order, t as an index into
-    # the class's simplex list (the MONOS order: all t with |t| <= L)
-    rowptr, tidx, vals = [0], [], []
-    for L in range(lmax + 1):
-        keys, acc = keys_of(L)
-        order = schedule_of(L)[0]
-        T = L + 1
-        pos = {((tx * T + ty) * T + tz): i for i, (tx, ty, tz) in enumerate(order)}
-        for k in keys:
-            for col, v in sorted(acc[k].items()):
-                if v != 0.0:
-                    tidx.append(pos[col])
-                    vals.append(v)
-            rowptr.append(len(tidx))
-    # transpose (per simplex t: the keys k it feeds), for the adjoint dG -> d etab
-    colptr, kidx, tvals = [0], [], []
-    for L in range(lmax + 1):
-        keys, acc = keys_of(L)
-        order = schedule_of(L)[0]
-        T = L + 1
-        for (tx, ty, tz) in order:
-            col = (tx * T + ty) * T + tz
-            for k, key in enumerate(keys):
-                v = acc[key].get(col, 0.0)
-                if v != 0.0:
-                    kidx.append(k)
-                    tvals.append(v)
-            colptr.append(len(kidx))
-    host.append(f"static const int M0T_COLPTR[{len(colptr)}] = {{{', '.join(map(str, colptr))}}};")
-    host.append(f"static const int M0T_K[{len(kidx)}] = {{{', '.join(map(str, kidx))}}};")
-    host.append(f"static const double M0T_VAL[{len(tvals)}] = {{{', '.join(repr(v) for v in tvals)}}};")
-    host.append(f"static const int M0_ROWPTR[{len(rowptr)}] = {{{', '.join(map(str, rowptr))}}};")
-    host.append(f"static const int M0_TIDX[{len(tidx)}] = {{{', '.join(map(str, tidx))}}};")
-    host.append(f"static const double M0_VAL[{len(vals)}] = {{{', '.join(repr(v) for v in vals)}}};")
     host.append("}  // namespace fe_host")
     out.extend(host)
     dst = Path(__file__).with_name("fe_lclass.inc")

@@ -269,3 +269,64 @@ def test_det_from_dg_is_the_adjoint_of_the_table_contraction(src, L):
     G = [2 * np.pi / p * (2 * p) ** m * sum(v * etab[pos[col]] for col, v in acc[(ax, ay, az, m)].items())
          for (ax, ay, az, m) in keys]
     assert np.dot(det, etab) == pytest.approx(np.dot(g, G), rel=1e-12)
+
+
+PAIR_LS = [(li, lj) for li in range(4) for lj in range(li + 1)]
+
+
+@pytest.mark.parametrize("li,lj", PAIR_LS)
+def test_pair_etab_matches_the_hermite_recurrence(src, li, lj):
+    """fe_pair_etab<li, lj> (the polynomial-algebra etab of the device build)
+    equals cc sum_ij w_ij Ex Ey Ez with the 1-D tables of the oracle's E
+    recurrence (oracle/ints.py e1d), over the simplex in the class's order."""
+    from fastesp_amd.basis import cart_order
+    from oracle.ints import e1d
+    gen = _gen()
+    L = li + lj
+    order = gen.schedule_of(L)[0]
+    ci, cj = cart_order(li), cart_order(lj)
+    rng = np.random.default_rng(300 + 10 * li + lj)
+    for trial in range(3):
+        p = float(rng.uniform(0.2, 9.0))
+        pa = rng.uniform(-1.2, 1.2, 3)
+        pb = rng.uniform(-1.2, 1.2, 3) if trial else -pa * 0.4
+        cc = float(rng.normal())
+        w = rng.normal(size=(len(ci), len(cj)))
+        et = [0.0] * len(order)
+        env = dict(w=list(w.ravel()), pax=pa[0], pay=pa[1], paz=pa[2], pbx=pb[0], pby=pb[1],
+                   pbz=pb[2], h=0.5 / p, cc=cc, et=et)
+        run(body(src, "fe_pair_etab", f"{li}, {lj}"), env)
+        E = [e1d(li, lj, p, pa[d], pb[d])[0] for d in range(3)]
+        want = np.zeros(len(order))
+        for i, a in enumerate(ci):
+            for j, b in enumerate(cj):
+                for q, t in enumerate(order):
+                    want[q] += w[i, j] * E[0][a[0], b[0], t[0]] * E[1][a[1], b[1], t[1]] \
+                        * E[2][a[2], b[2], t[2]]
+        want *= cc
+        np.testing.assert_allclose(et, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("L", range(7))
+def test_g_from_etab_and_bound_match_the_oracle_pair_type(src, L):
+    """fe_g_from_etab<L> is G = (2 pi/p)(2p)^m M0 etab and fe_etab_bound<L> the
+    screening bound's sum over |etab| (before (2p)^-3/2), both with the M0,
+    moments and |t| of oracle/ints.py pair_type."""
+    from oracle.ints import pair_type
+    gen = _gen()
+    order = gen.schedule_of(L)[0]
+    T = L + 1
+    keys, M0, wmom, abs_t = pair_type(L)
+    rng = np.random.default_rng(400 + L)
+    p = 1.37
+    et = rng.normal(size=len(order))
+    full = np.zeros(T ** 3)
+    for q, (tx, ty, tz) in enumerate(order):
+        full[(tx * T + ty) * T + tz] = et[q]
+    g = [0.0] * len(keys)
+    run(body(src, "fe_g_from_etab", L), dict(et=list(et), p=p, g=g))
+    want = 2 * np.pi / p * (2 * p) ** keys[:, 3] * (M0 @ full)
+    np.testing.assert_allclose(g, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+    b = run(body(src, "fe_etab_bound", L), dict(et=list(et), sq=np.sqrt(2 * p), fabs=abs))
+    wb = (np.abs(full) * wmom * (2 * p) ** (abs_t / 2.0)).sum()
+    assert b == pytest.approx(wb, rel=1e-13)
