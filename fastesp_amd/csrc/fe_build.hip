@@ -249,12 +249,23 @@ __device__ __forceinline__ bool fe_pass(const double d2, const double kb, const 
 // The largest non-negative double T with fe_pass(T) (-1 if none passes), so
 // that fe_pass(d2) == (d2 <= T) for every d2 >= 0: fe_pass is monotone, and
 // the bit patterns of non-negative doubles are ordered like their values, so
-// a bisection over the patterns finds the switch exactly (63 steps).
+// a bisection over the patterns finds the switch exactly.  The switch lies
+// near (kb/tol + reff)^2; when the test confirms a bracket of 1e-13 relative
+// around that, the bisection starts there (about 10 steps instead of 63).
 __device__ double fe_pass_threshold(const double kb, const double reff, const double tol,
                                     const int rule) {
   if (!fe_pass(0.0, kb, reff, tol, rule)) return -1.0;
   unsigned long long lo = 0ull, hi = 0x7FF0000000000000ull;  // pass(lo), +inf
   if (fe_pass(INFINITY, kb, reff, tol, rule)) return INFINITY;
+  const double d = kb / tol + (rule != 0 ? reff : 0.0);
+  const double est = d * d;
+  if (est > 0.0 && est < 1e300) {
+    const double a = est * (1.0 - 1e-13), b = est * (1.0 + 1e-13);
+    if (fe_pass(a, kb, reff, tol, rule) && !fe_pass(b, kb, reff, tol, rule)) {
+      lo = (unsigned long long)__double_as_longlong(a);
+      hi = (unsigned long long)__double_as_longlong(b);
+    }
+  }
   while (hi - lo > 1ull) {
     const unsigned long long mid = lo + (hi - lo) / 2;
     if (fe_pass(__longlong_as_double((long long)mid), kb, reff, tol, rule)) lo = mid;
@@ -400,10 +411,9 @@ __global__ __launch_bounds__(256) void k_screen_prep(const ScreenArgs a) {
 // passes the test (the event is a pass); rule 2 iff every point passes (the
 // event is a fail); without an event the decision is the opposite.  The lanes
 // take 64 boxes at a time: a box whose distance bounds settle the test for all
-// its points decides at once; each lane whose box is ambiguous tests that
-// box's points itself.  So no candidate's work runs on a single lane, and the
-// slowest candidate costs a few box steps.  A dropped candidate gets the
-// dropped flag in its (sorted) key.
+// its points decides at once; the ambiguous ones are then read one at a time,
+// a point per lane.  So no candidate's work runs on a single lane.  A dropped
+// candidate gets the dropped flag in its (sorted) key.
 __global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
   __shared__ int s_cnt[FE_ST_N];
   if (threadIdx.x < FE_ST_N) s_cnt[threadIdx.x] = 0;
@@ -453,20 +463,23 @@ __global__ __launch_bounds__(256) void k_screen_boxes(const ScreenArgs a) {
           keep = any_mode;
           break;
         }
-        const bool amb = vb && pass_near && !pass_far;
-        if (!__any(amb)) continue;
-        bool hit = false;  // a point of this lane's box passes (rules 0/1) or fails (rule 2)
-        if (amb) {
-          const int q0 = b * FE_BOX, q1 = min(a.n_pts, q0 + FE_BOX);
-#pragma unroll 8
-          for (int q = q0; q < q1; ++q) {
+        // the ambiguous boxes one at a time, the 64 lanes over its 64 points,
+        // until a point passes (rules 0/1) or fails (rule 2)
+        uint64_t amb = __ballot(vb && pass_near && !pass_far);
+        bool hit = false;
+        while (amb && !hit) {
+          const int q = (b0 + __builtin_ctzll(amb)) * FE_BOX + lane;
+          amb &= amb - 1;
+          bool h = false;
+          if (q < a.n_pts) {
             const double x = a.pts[3 * (size_t)q] - Px;
             const double y = a.pts[3 * (size_t)q + 1] - Py;
             const double z = a.pts[3 * (size_t)q + 2] - Pz;
-            hit |= (fma(x, x, fma(y, y, z * z)) <= thr) == any_mode;
+            h = (fma(x, x, fma(y, y, z * z)) <= thr) == any_mode;
           }
+          hit = __any(h);
         }
-        if (__any(hit)) {
+        if (hit) {
           keep = any_mode;
           break;
         }

@@ -812,8 +812,9 @@ def gen_pair_etab(LI, LJ):
     return o
 
 
-# \int |He_n(x)| e^{-x^2/2} dx, n = 0..11, as tabulated by the reference
-# (fastesp/esp.py:25-28): the screening bound's per-axis weights.
+# The screening bound's per-axis weights for n = 0..11, the reference's table
+# (fastesp/esp.py:25-28), kept as it is because it decides which pairs exist
+# (it is labelled \int |He_n(x)| e^{-x^2/2} dx but equals that only for n <= 1).
 MABS = [2.5066282746310002, 2.0, 1.5957691216057308, 4.0, 10.212920779565312,
         33.63284766793385, 136.5963625280984, 665.1416330443616, 3768.863619586618,
         25506.330595911374, 202331.81842358326, 1872154.2572862155]

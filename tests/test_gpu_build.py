@@ -139,3 +139,56 @@ def test_build_rejects_bad_input():
     wfn, basis, P, _ = oh_toy()
     with pytest.raises(_lib.HipBackendError):
         build_table_hip(wfn, basis, P, np.zeros((1, 3)), tol=-1.0)
+
+
+def _permuted_payload(pl, perm):
+    """The same basis and density with the functions in the order `perm`."""
+    q = dict(pl)
+    for k in ("fn_atom", "fn_nprim", "fn_off", "fn_lx", "fn_ly", "fn_lz", "fn_shell"):
+        q[k] = np.ascontiguousarray(pl[k][perm])
+    q["fc"] = np.ascontiguousarray(pl["fc"].reshape(-1, 3)[perm].ravel())
+    q["dens"] = np.ascontiguousarray(pl["dens"][np.ix_(perm, perm)])
+    return q
+
+
+def test_build_takes_cartesian_components_in_any_order():
+    """A basis whose shells list their Cartesian components in another order
+    (here reversed in every shell with l >= 1), with the density permuted to
+    match, gives the same pairs, G and ESP: the build reads each shell's
+    block through its components, not their positions."""
+    from fastesp_amd.builder import build_table_payload
+    wfn, basis, P, _ = cf_toy()
+    pts = np.random.default_rng(21).normal(0, 3.0, (200, 3))
+    pl = basis_payload(wfn, basis, P, pts)
+    perm = []
+    for s in np.unique(pl["fn_shell"]):
+        idx = np.flatnonzero(pl["fn_shell"] == s)
+        perm.extend(idx[::-1])
+    perm = np.array(perm)
+    assert not np.array_equal(perm, np.arange(len(perm)))
+    say("cf toy build, components reversed in every shell")
+    a = build_table_payload(pl, tol=0.0)
+    b = build_table_payload(_permuted_payload(pl, perm), tol=0.0)
+    fa, fb = export_flat(a), export_flat(b)
+    assert fa.n_pairs == fb.n_pairs and fa.n_terms == fb.n_terms
+    np.testing.assert_array_equal(fa.pp_exp, fb.pp_exp)
+    np.testing.assert_array_equal(fa.pp_K, fb.pp_K)
+    np.testing.assert_allclose(fb.tm_g, fa.tm_g, rtol=1e-14, atol=1e-14 * np.abs(fa.tm_g).max())
+    np.testing.assert_allclose(b.esp(), a.esp(), rtol=0, atol=1e-13)
+
+
+def test_build_rejects_a_shell_with_a_repeated_component():
+    """Every Cartesian component of a shell exactly once: a d shell listing xx
+    twice (and no yy) is a malformed basis."""
+    from fastesp_amd.builder import build_table_payload
+    wfn, basis, P, _ = oh_toy()
+    pl = basis_payload(wfn, basis, P, np.zeros((4, 3)))
+    d = np.flatnonzero((pl["fn_lx"] + pl["fn_ly"] + pl["fn_lz"]) == 2)
+    assert len(d) == 6
+    pl["fn_lx"] = pl["fn_lx"].copy()
+    pl["fn_ly"] = pl["fn_ly"].copy()
+    yy = d[(pl["fn_ly"][d] == 2)][0]
+    pl["fn_lx"][yy], pl["fn_ly"][yy] = 2, 0
+    say("oh toy build with a repeated d component")
+    with pytest.raises(_lib.HipBackendError, match="components"):
+        build_table_payload(pl, tol=0.0)
