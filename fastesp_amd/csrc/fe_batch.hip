@@ -34,6 +34,8 @@
 #include "fe_adjoint.h"
 #include "fe_reduce.h"
 
+#define FE_RB_LANES 4  // streams of the in-graph table rebuilds
+
 namespace {
 
 struct FeDgItem {
@@ -98,8 +100,12 @@ struct fe_batch {
   int rb_rule = 0;
   void* rbmem = nullptr;
   double* dens = nullptr;
-  char* bws = nullptr;
+  char* bws = nullptr;      // FE_RB_LANES build workspaces of bws_bytes each
   size_t bws_bytes = 0;
+  // the rebuilds run on FE_RB_LANES streams of the batch's own (table i on
+  // lane i mod FE_RB_LANES), forked from and joined into the step's stream
+  hipStream_t rb_st[FE_RB_LANES] = {};
+  hipEvent_t rb_fork = nullptr, rb_join[FE_RB_LANES] = {};
   std::vector<int64_t> dens_off;  // n_conf + 1
   // forward plan
   bool fplan = false, f_grad = false, f_mixed = false;
@@ -532,6 +538,11 @@ extern "C" int fe_batch_free(fe_batch* b) {
   drop_fplan(b);
   drop_bplan(b);
   if (b->rbmem) (void)hipFree(b->rbmem);
+  for (int k = 0; k < FE_RB_LANES; ++k) {
+    if (b->rb_st[k]) (void)hipStreamDestroy(b->rb_st[k]);
+    if (b->rb_join[k]) (void)hipEventDestroy(b->rb_join[k]);
+  }
+  if (b->rb_fork) (void)hipEventDestroy(b->rb_fork);
   if (b->mem) (void)hipFree(b->mem);
   if (b->cs) (void)hipStreamDestroy(b->cs);
   delete b;
@@ -995,13 +1006,24 @@ static int prepare_step(fe_batch* b, int precision, int flags, hipStream_t st) {
 // every table built again from the batch's densities and current points (the
 // screen reads the sorted copy; it does not depend on the order)
 static int launch_rebuilds(fe_batch* b, hipStream_t st) {
+  const int K = std::min(FE_RB_LANES, b->n_conf);
+  FE_HIPCHK(hipEventRecord(b->rb_fork, st));
+  for (int k = 0; k < K; ++k) FE_HIPCHK(hipStreamWaitEvent(b->rb_st[k], b->rb_fork, 0));
   for (int i = 0; i < b->n_conf; ++i) {
+    const int k = i % K;
     const int64_t pt0 = b->pt_off[i], n = b->pt_off[i + 1] - pt0;
     if (int rc = fe_table_rebuild_ws(b->tabs[i], b->dens + b->dens_off[i], (int)n,
-                                     b->pts + 3 * pt0, b->rb_tol, b->rb_rule, st, b->bws,
-                                     b->bws_bytes))
+                                     b->pts + 3 * pt0, b->rb_tol, b->rb_rule, b->rb_st[k],
+                                     b->bws + (size_t)k * b->bws_bytes, b->bws_bytes, false))
       return rc;
   }
+  for (int k = 0; k < K; ++k) {
+    FE_HIPCHK(hipEventRecord(b->rb_join[k], b->rb_st[k]));
+    FE_HIPCHK(hipStreamWaitEvent(st, b->rb_join[k], 0));
+  }
+  // the counts are complete once st is: host queries wait on st, not on a
+  // lane stream of this batch
+  for (int i = 0; i < b->n_conf; ++i) b->tabs[i]->count_stream = st;
   return FE_OK;
 }
 
@@ -1126,12 +1148,19 @@ extern "C" int fe_batch_set_density(fe_batch* b, const double* dens, double tol,
       b->dens_off[i + 1] = b->dens_off[i] + (int64_t)t->n_func * t->n_func;
       ws = std::max(ws, fe_build_ws_bytes(t, (int)(b->pt_off[i + 1] - b->pt_off[i])));
     }
+    ws = up256(ws);
     const size_t db = up256((size_t)b->dens_off[nc] * sizeof(double));
-    if (int rc = batch_alloc(&b->rbmem, db + ws, "densities and build workspace")) return rc;
+    if (int rc = batch_alloc(&b->rbmem, db + FE_RB_LANES * ws, "densities and build workspaces"))
+      return rc;
     b->dens = (double*)b->rbmem;
     b->bws = (char*)b->rbmem + db;
     b->bws_bytes = ws;
   }
+  for (int k = 0; k < FE_RB_LANES; ++k) {
+    if (!b->rb_st[k]) FE_HIPCHK(hipStreamCreateWithFlags(&b->rb_st[k], hipStreamNonBlocking));
+    if (!b->rb_join[k]) FE_HIPCHK(hipEventCreateWithFlags(&b->rb_join[k], hipEventDisableTiming));
+  }
+  if (!b->rb_fork) FE_HIPCHK(hipEventCreateWithFlags(&b->rb_fork, hipEventDisableTiming));
   if (b->dens_off[nc])
     FE_HIPCHK(hipMemcpyAsync(b->dens, dens, (size_t)b->dens_off[nc] * sizeof(double),
                              ptr_kind == FE_PTR_HOST ? hipMemcpyHostToDevice

@@ -703,7 +703,7 @@ CandLaunch cand_launch(const fe_table* t, const bool heavy) {
 // fe_build_ws_bytes bytes (device inputs only), else the context's.
 int build_on_device(fe_table* t, const double* dens, int n_points, const double* pts, double tol,
                     int rule, int ptr_kind, hipStream_t st, bool sized, char* ws = nullptr,
-                    size_t ws_bytes = 0) {
+                    size_t ws_bytes = 0, bool fork = true) {
   fe_ctx* c = t->ctx;
   ClassInfo ci{};
   for (int L = 0; L <= FE_LMAX; ++L) {
@@ -761,11 +761,14 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   ba.cand_srule = (double*)(wb + o_srule);
   ba.ci = ci;
   // the heavy shell-l pairs on the context's second stream, beside the light
-  // ones and the point sort on the caller's stream
+  // ones and the point sort on the caller's stream (`fork` false: all on the
+  // caller's stream, for callers that run several builds side by side)
   const CandLaunch pl_heavy = cand_launch(t, true), pl_light = cand_launch(t, false);
   hipStream_t side = st;
+  fork = fork && pl_heavy.n_r > 0;
   if (pl_heavy.n_r) {
-    if (int rc = fe_fork(c, st, &side)) return rc;
+    if (fork)
+      if (int rc = fe_fork(c, st, &side)) return rc;
     hipLaunchKernelGGL(k_build_cands<true>, dim3((unsigned)pl_heavy.blk_off[pl_heavy.n_r]),
                        dim3(256), 0, side, ba, pl_heavy);
   }
@@ -783,7 +786,7 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
     hipLaunchKernelGGL(k_boxes, dim3(n_box), dim3(64), 0, st, sorted, n_points,
                        (double*)(wb + o_box));
   }
-  if (pl_heavy.n_r) {
+  if (fork) {
     if (int rc = fe_join(c, st)) return rc;
   }
 
@@ -1235,13 +1238,14 @@ extern "C" int fe_table_build(fe_ctx* c, int n_func, int n_prim, const int32_t* 
 // nothing here touches the context's workspace, so a graph that records it
 // stays valid whatever the context does later.
 int fe_table_rebuild_ws(fe_table* t, const double* dens, int n_points, const double* pts,
-                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes) {
+                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes,
+                        bool fork) {
   if (!t || !t->has_basis || !t->capacity_layout)
     return fe_set_error(FE_ERR_STATE, "rebuild: the table has no capacity layout yet");
   if (int rc = check_build_args(n_points, pts, dens, tol, rule, FE_PTR_DEVICE, "rebuild"))
     return rc;
   if (int rc = build_on_device(t, dens, n_points, pts, tol, rule, FE_PTR_DEVICE, st, true, ws,
-                               ws_bytes))
+                               ws_bytes, fork))
     return rc;
   if (int rc = fe_refresh_g32(t, st)) return rc;
   t->counts_stale = true;

@@ -263,10 +263,12 @@ int fe_table_sync_counts(fe_table* t);
 int fe_refresh_g32(fe_table* t, hipStream_t st);
 // workspace bytes of a build of table t on n_points device points, and the
 // rebuild on device inputs into a caller-owned workspace (the table must be in
-// its capacity layout: a first fe_table_rebuild has run)
+// its capacity layout: a first fe_table_rebuild has run); `fork` false keeps
+// every launch on st (no use of the context's second stream)
 size_t fe_build_ws_bytes(const fe_table* t, int n_points);
 int fe_table_rebuild_ws(fe_table* t, const double* dens, int n_points, const double* pts,
-                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes);
+                        double tol, int rule, hipStream_t st, char* ws, size_t ws_bytes,
+                        bool fork = true);
 
 
 // Partial sums: pair chunk c of every class adds into row c.  The classes run
