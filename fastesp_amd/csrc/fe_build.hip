@@ -595,6 +595,33 @@ __global__ __launch_bounds__(256) void k_fill_record(const CompactArgs a) {
     fe_fill_record_any(L, a.rec[L] + (size_t)i * fe_stride(L));
 }
 
+// The per-shell-pair lists of a basis, one lane per shell pair: its dL/dP
+// tasks (sp << 8 | mi << 4 | mj, from toff[sp]) and its build candidates
+// (shell pair and primitive pair i-major, from coff[sp]).
+__global__ __launch_bounds__(256) void k_expand_lists(const int* __restrict__ sp_i,
+                                                      const int* __restrict__ sp_j,
+                                                      const int* __restrict__ nfn,
+                                                      const int* __restrict__ nprim,
+                                                      const int* __restrict__ toff,
+                                                      const int* __restrict__ coff, const int nsp,
+                                                      uint32_t* __restrict__ tasks,
+                                                      int* __restrict__ cl_sp,
+                                                      int* __restrict__ cl_s) {
+  const int sp = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (sp >= nsp) return;
+  const int si = sp_i[sp], sj = sp_j[sp];
+  const int ni = nfn[si], nj = nfn[sj];
+  uint32_t* t = tasks + toff[sp];
+  for (int mi = 0; mi < ni; ++mi)
+    for (int mj = 0; mj < nj; ++mj)
+      t[mi * nj + mj] = (uint32_t)sp << 8 | (uint32_t)mi << 4 | (uint32_t)mj;
+  const int np = nprim[si] * nprim[sj], c0 = coff[sp];
+  for (int q = 0; q < np; ++q) {
+    cl_sp[c0 + q] = sp;
+    cl_s[c0 + q] = q;
+  }
+}
+
 // ---------------------------------------------------------------------------
 namespace {
 struct HostShell {
@@ -1046,21 +1073,24 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
   // shell j), class by class in the order above, packed as sp << 8 | mi << 4 | mj
   if (nsp_all >= (1u << 24))
     return fe_set_error(FE_ERR_UNSUPPORTED, "too many shell pairs (%zu) for dL/dP", nsp_all);
-  std::vector<uint32_t> tasks;
-  for (int L = 0; L <= FE_LMAX; ++L) {
-    bs->task_cls_off[L] = (int)tasks.size();
-    for (int q = bs->sp_cls_off[L]; q < bs->sp_cls_off[L + 1]; ++q) {
-      const int sp = sp_cls[q];
-      const int ni = sh[sp_i[sp]].nfn, nj = sh[sp_j[sp]].nfn;
-      for (int mi = 0; mi < ni; ++mi)
-        for (int mj = 0; mj < nj; ++mj)
-          tasks.push_back((uint32_t)sp << 8 | (uint32_t)mi << 4 | (uint32_t)mj);
+  // (expanded on the device from per-shell-pair offsets, k_expand_lists)
+  std::vector<int> sp_toff(nsp_all), sp_coff(nsp_all);
+  {
+    int64_t nt = 0;
+    for (int L = 0; L <= FE_LMAX; ++L) {
+      bs->task_cls_off[L] = (int)nt;
+      for (int q = bs->sp_cls_off[L]; q < bs->sp_cls_off[L + 1]; ++q) {
+        const int sp = sp_cls[q];
+        sp_toff[sp] = (int)nt;
+        nt += (int64_t)sh[sp_i[sp]].nfn * sh[sp_j[sp]].nfn;
+      }
     }
+    if (nt > INT32_MAX) return fe_set_error(FE_ERR_UNSUPPORTED, "too many dL/dP tasks");
+    bs->task_cls_off[FE_LMAX + 1] = (int)nt;
   }
-  bs->task_cls_off[FE_LMAX + 1] = (int)tasks.size();
   // the build candidates grouped by shell-l pair (l_i >= l_j, either order of
-  // the two shells), reference order within a group
-  std::vector<int> cl_sp((size_t)gtot), cl_s((size_t)gtot);
+  // the two shells), reference order within a group: sp_coff[sp] is where the
+  // shell pair's candidates start
   {
     auto combo = [&](int sp) {
       const int la = sh[sp_i[sp]].l, lb = sh[sp_j[sp]].l;
@@ -1074,11 +1104,8 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
     for (int cb = 0; cb <= FE_NCOMBO; ++cb) bs->cmb_off[cb] = (int)at[cb];
     for (int sp = 0; sp < nsp; ++sp) {
       int64_t& o = at[combo(sp)];
-      const int np = sh[sp_i[sp]].nprim * sh[sp_j[sp]].nprim;
-      for (int q = 0; q < np; ++q, ++o) {
-        cl_sp[o] = sp;
-        cl_s[o] = q;
-      }
+      sp_coff[sp] = (int)o;
+      o += (int64_t)sh[sp_i[sp]].nprim * sh[sp_j[sp]].nprim;
     }
   }
   bs->device = c->device;
@@ -1110,6 +1137,8 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
     for (int q = 0; q < 3; ++q) v_c[3 * i + q] = sh[i].c[q];
   }
   std::vector<char> hb;
+  hb.reserve(256 * 24 + 4 * (6 * (size_t)nsh + 4 * (size_t)n_func + 7 * nsp_all) +
+             8 * (3 * (size_t)nsh + 2 * (size_t)n_prim));
   bs->o_atom = push_bytes(hb, v_atom.data(), nsh);
   bs->o_l = push_bytes(hb, v_l.data(), nsh);
   bs->o_fn0 = push_bytes(hb, v_fn0.data(), nsh);
@@ -1127,20 +1156,41 @@ int basis_setup(fe_ctx* c, int n_func, int n_prim, const int32_t* fn_atom,
   bs->o_sps = push_bytes(hb, sp_slot.data(), nsp);
   bs->o_spg = push_bytes(hb, sp_gidx.data(), nsp);
   bs->o_spc = push_bytes(hb, sp_cls.data(), nsp);
-  bs->o_task = push_bytes(hb, tasks.data(), tasks.size());
   bs->o_cperm = push_bytes(hb, cperm.data(), cperm.size());
-  bs->o_clsp = push_bytes(hb, cl_sp.data(), cl_sp.size());
-  bs->o_cls = push_bytes(hb, cl_s.data(), cl_s.size());
-  if (hipMalloc(&bs->mem, hb.size()) != hipSuccess) {
+  const size_t o_toff = push_bytes(hb, sp_toff.data(), nsp_all);
+  const size_t o_coff = push_bytes(hb, sp_coff.data(), nsp_all);
+  // device-only tail: the task and candidate lists, filled by k_expand_lists
+  size_t tail = al256(hb.size());
+  bs->o_task = tail;
+  tail += al256((size_t)bs->task_cls_off[FE_LMAX + 1] * sizeof(uint32_t));
+  bs->o_clsp = tail;
+  tail += al256((size_t)gtot * sizeof(int));
+  bs->o_cls = tail;
+  tail += al256((size_t)gtot * sizeof(int));
+  if (hipMalloc(&bs->mem, tail) != hipSuccess) {
     (void)hipGetLastError();
     bs->mem = nullptr;
     return fe_set_error(FE_ERR_ALLOC, "fe_table_build: basis allocation failed");
   }
-  bs->bytes = hb.size();
+  bs->bytes = tail;
   // synchronous: hb is a local buffer
   if (hipMemcpy(bs->mem, hb.data(), hb.size(), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipGetLastError();
     return fe_set_error(FE_ERR_HIP, "fe_table_build: basis upload (hipMemcpy) failed");
+  }
+  {
+    char* m = (char*)bs->mem;
+    if (nsp > 0)
+      hipLaunchKernelGGL(k_expand_lists, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, nullptr,
+                         (const int*)(m + bs->o_spi), (const int*)(m + bs->o_spj),
+                         (const int*)(m + bs->o_nfn), (const int*)(m + bs->o_np),
+                         (const int*)(m + o_toff), (const int*)(m + o_coff), nsp,
+                         (uint32_t*)(m + bs->o_task), (int*)(m + bs->o_clsp),
+                         (int*)(m + bs->o_cls));
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+      (void)hipGetLastError();
+      return fe_set_error(FE_ERR_HIP, "fe_table_build: task and candidate lists (k_expand_lists)");
+    }
   }
   const int32_t* iv[7] = {fn_atom, fn_nprim, fn_off, fn_lx, fn_ly, fn_lz, fn_shell};
   basis_key(n_func, n_prim, iv, fc, pexp, pcoef, bs->ikey, bs->dkey);
