@@ -166,7 +166,7 @@ class Context:
         check(self.lib.fe_set_tuning(self.handle, key, int(value)), "fe_set_tuning")
 
     def trim(self) -> None:
-        """Release the context's workspace and pinned staging (``fe_ctx_trim``)."""
+        """Release the context's workspace and cached basis set-up (``fe_ctx_trim``)."""
         check(self.lib.fe_ctx_trim(self.handle), "fe_ctx_trim")
 
     def profile_start(self) -> None:

@@ -538,9 +538,9 @@ struct CompactArgs {
   int32_t* orig[FE_LMAX + 1];
   int* slot2kept[FE_LMAX + 1];
   int64_t cap[FE_LMAX + 1], ncand[FE_LMAX + 1], cand_off[FE_LMAX + 1];
-  int64_t e_off[FE_LMAX + 2];  // copied doubles: class L covers [e_off[L], e_off[L+1])
+  int e_blk[FE_LMAX + 2];      // copy blocks of class L: [e_blk[L], e_blk[L+1])
   int s_blk[FE_LMAX + 2];      // slot-map blocks of class L: [s_blk[L], s_blk[L+1])
-  int r_blk[FE_LMAX + 2];      // record blocks of class L: [r_blk[L], r_blk[L+1])
+  int r_blk[FE_LMAX + 2];      // record blocks of class FE_LMAX - k: [r_blk[k], r_blk[k+1])
   const int32_t* sorted;       // candidate indices, each class's kept ones first
   const int* gidx;
   const unsigned long long* gscan;
@@ -562,22 +562,21 @@ __global__ __launch_bounds__(256) void k_slotmap_clear(const CompactArgs a) {
 // Kept record k of class L: candidate sorted[k] (k < the kept count), with
 // slots 6/7 holding the far-field constants 1/(2p) and K sqrt(pi/p)/2 instead
 // of reff and the flags; column 0's thread also writes the flat term offset,
-// the reference-order index and the slot -> record map.
-__global__ __launch_bounds__(256) void k_compact(const CompactArgs a) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= a.e_off[FE_LMAX + 1]) return;
-  int L = 0;
-  while (L < FE_LMAX && e >= a.e_off[L + 1]) ++L;
-  const int CS = fe_cstride(L), S = fe_stride(L);
-  const int64_t el = e - a.e_off[L];
+// the reference-order index and the slot -> record map.  One thread per
+// copied double, the class's strides known at compile time.
+template <int L>
+__device__ __forceinline__ void fe_compact_cls(const CompactArgs& a, const int el) {
+  constexpr int CS = FeL<L>::KOFF, S = FeL<L>::STRIDE;
   const int n = min(a.stats[FE_ST_KEPT + L], (int)a.cap[L]);
-  if (el >= (int64_t)n * CS) return;
-  const int r = (int)(el / CS), c = (int)(el - (int64_t)r * CS);
+  const int r = el / CS, c = el - r * CS;
+  if (r >= n) return;
   const int ge = a.sorted[a.cand_off[L] + r];
   const int slot = (int)(ge - a.cand_off[L]);
   const double* s = a.cand[L] + (size_t)slot * CS;
-  a.rec[L][(size_t)r * S + c] =
-      c == 6 ? 0.5 / s[3] : c == 7 ? s[4] * (0.88622692545275801365 / sqrt(s[3])) : s[c];
+  double v = s[c];
+  if (c == 6) v = 0.5 / s[3];
+  else if (c == 7) v = s[4] * (0.88622692545275801365 / sqrt(s[3]));
+  a.rec[L][(size_t)r * S + c] = v;
   if (c == 0) {
     const unsigned long long g = a.gscan[a.gidx[ge]];
     a.flatoff[L][r] = (int)(g >> 32);
@@ -586,11 +585,28 @@ __global__ __launch_bounds__(256) void k_compact(const CompactArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_compact(const CompactArgs a) {
+  const int L = fe_range_of(a.e_blk, (int)blockIdx.x);
+  const int el = (int)(blockIdx.x - a.e_blk[L]) * 256 + (int)threadIdx.x;
+  switch (L) {
+    case 0: fe_compact_cls<0>(a, el); break;
+    case 1: fe_compact_cls<1>(a, el); break;
+    case 2: fe_compact_cls<2>(a, el); break;
+    case 3: fe_compact_cls<3>(a, el); break;
+    case 4: fe_compact_cls<4>(a, el); break;
+    case 5: fe_compact_cls<5>(a, el); break;
+    case 6: fe_compact_cls<6>(a, el); break;
+    default: break;
+  }
+}
+
 // the derived parts of each kept record (one lane per record): the scaled
-// coefficients c g_k and the far-field block
+// coefficients c g_k and the far-field block; the highest class first, whose
+// lanes run longest
 __global__ __launch_bounds__(256) void k_fill_record(const CompactArgs a) {
-  const int L = fe_range_of(a.r_blk, (int)blockIdx.x);
-  const int i = (int)(blockIdx.x - a.r_blk[L]) * 256 + (int)threadIdx.x;
+  const int k = fe_range_of(a.r_blk, (int)blockIdx.x);
+  const int L = FE_LMAX - k;
+  const int i = (int)(blockIdx.x - a.r_blk[k]) * 256 + (int)threadIdx.x;
   if (i < min(a.stats[FE_ST_KEPT + L], (int)a.cap[L]))
     fe_fill_record_any(L, a.rec[L] + (size_t)i * fe_stride(L));
 }
@@ -900,9 +916,16 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
     ca.cap[L] = cap;
     ca.ncand[L] = t->n_cand_L[L];
     ca.cand_off[L] = ci.cand_off[L];
-    ca.e_off[L + 1] = ca.e_off[L] + cap * fe_host::CSTRIDE[L];
+    if (cap * fe_host::CSTRIDE[L] > INT32_MAX)
+      return fe_set_error(FE_ERR_UNSUPPORTED, "table: %lld records of class %d",
+                          (long long)cap, L);
+    ca.e_blk[L + 1] = ca.e_blk[L] + (int)((cap * fe_host::CSTRIDE[L] + 255) / 256);
     ca.s_blk[L + 1] = ca.s_blk[L] + (int)((t->n_cand_L[L] + 255) / 256);
-    ca.r_blk[L + 1] = ca.r_blk[L] + (int)((cap + 255) / 256);
+  }
+  for (int k = 0; k <= FE_LMAX; ++k) {
+    const int L = FE_LMAX - k;
+    const int64_t cap = t->n_cand_L[L] ? t->cap_L[L] : 0;
+    ca.r_blk[k + 1] = ca.r_blk[k] + (int)((cap + 255) / 256);
   }
   ca.sorted = val;
   ca.gidx = ba.gidx;
@@ -910,9 +933,8 @@ int build_on_device(fe_table* t, const double* dens, int n_points, const double*
   ca.stats = t->d_stats;
   if (ca.s_blk[FE_LMAX + 1] > 0)
     hipLaunchKernelGGL(k_slotmap_clear, dim3((unsigned)ca.s_blk[FE_LMAX + 1]), dim3(256), 0, st, ca);
-  if (ca.e_off[FE_LMAX + 1] > 0) {
-    hipLaunchKernelGGL(k_compact, dim3((unsigned)((ca.e_off[FE_LMAX + 1] + 255) / 256)), dim3(256),
-                       0, st, ca);
+  if (ca.e_blk[FE_LMAX + 1] > 0) {
+    hipLaunchKernelGGL(k_compact, dim3((unsigned)ca.e_blk[FE_LMAX + 1]), dim3(256), 0, st, ca);
     hipLaunchKernelGGL(k_fill_record, dim3((unsigned)ca.r_blk[FE_LMAX + 1]), dim3(256), 0, st, ca);
   }
   FE_HIPCHK(hipGetLastError());

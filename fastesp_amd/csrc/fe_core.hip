@@ -141,22 +141,6 @@ int fe_ctx_reserve(fe_ctx* c, int slot, size_t bytes) {
   return FE_OK;
 }
 
-int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes) {
-  if (c->h_pin_bytes >= bytes) return FE_OK;
-  if (c->h_pin) FE_HIPCHK(hipHostFree(c->h_pin));
-  c->h_pin = nullptr;
-  c->h_pin_bytes = 0;
-  size_t b = std::max<size_t>(bytes, 4096);
-  b = (b + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
-  if (hipHostMalloc(&c->h_pin, b, hipHostMallocDefault) != hipSuccess) {
-    (void)hipGetLastError();
-    c->h_pin = nullptr;
-    return fe_set_error(FE_ERR_ALLOC, "pinned host allocation of %zu bytes failed", b);
-  }
-  c->h_pin_bytes = b;
-  return FE_OK;
-}
-
 int fe_prof_begin(fe_ctx* c, int phase, hipStream_t st) {
   if (!c->prof_on) return -1;
   if (c->prof_n == c->prof.size()) {
@@ -275,7 +259,6 @@ extern "C" int fe_finalize(fe_ctx* c) {
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   for (int s = 0; s < FE_WS_SLOTS; ++s)
     if (c->ws[s]) (void)hipFree(c->ws[s]);
-  if (c->h_pin) (void)hipHostFree(c->h_pin);
   c->basis_cache.reset();
   if (c->d_boys) (void)hipFree(c->d_boys);
   for (FeProfEv& e : c->prof) {
@@ -297,9 +280,6 @@ extern "C" int fe_ctx_trim(fe_ctx* c) {
     c->ws[s] = nullptr;
     c->ws_bytes[s] = 0;
   }
-  if (c->h_pin) FE_HIPCHK(hipHostFree(c->h_pin));
-  c->h_pin = nullptr;
-  c->h_pin_bytes = 0;
   c->basis_cache.reset();
   return FE_OK;
 }

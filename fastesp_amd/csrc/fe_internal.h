@@ -79,7 +79,6 @@ int fe_radix_sort_bits(uint32_t* keys, int32_t* vals, int n, int lo_bit, int hi_
 size_t fe_scan_scratch(int64_t n);
 int fe_scan_excl_u64(const unsigned long long* in, unsigned long long* out, int n,
                      void* scratch, hipStream_t st);
-int fe_ctx_reserve_pinned(fe_ctx* c, size_t bytes);
 
 #define FE_HIPCHK(x)                                                                         \
   do {                                                                                       \
@@ -136,8 +135,6 @@ struct fe_ctx {
   double* d_boys = nullptr;  // (FE_BOYS_MMAX+1) x NR x COLS
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
-  void* h_pin = nullptr;  // pinned host staging of the table build (hipHostMalloc)
-  size_t h_pin_bytes = 0;
   int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 2};  // points per lane, per class
   int target_wg = 4096; // workgroups per class launch
   bool sort_points = true;  // Morton-order the points before evaluation

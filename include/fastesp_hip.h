@@ -74,8 +74,8 @@ int fe_finalize(fe_ctx* ctx);
 
 /* Release the context's workspace (the scratch that table builds and
  * evaluations grow on demand and keep for the next call: about 0.1 GB after a
- * 1634-function build, 0.7 GB after a 4538-function one) and its pinned host
- * staging buffer.  Waits for all work on the device first.  The next call
+ * 1634-function build, 0.7 GB after a 4538-function one) and the cached
+ * basis set-up.  Waits for all work on the device first.  The next call
  * allocates again what it needs.  No counterpart in the reference, whose
  * processes free everything at exit (fastesp/cli.py:143-163). */
 int fe_ctx_trim(fe_ctx* ctx);
@@ -101,7 +101,9 @@ int fe_table_from_pairs(fe_ctx* ctx, int n_pairs, int n_terms,
  * matrix and the ESP points (used only for the screen).  Arguments are the
  * basis.bin fields of fastesp/table.py:268-316 (one entry per Cartesian
  * component, primitives duplicated per component, components of a shell
- * contiguous and in cart_order).  Replaces `build_pairs <basis.bin> <pairs.bin>
+ * contiguous: every component of the shell's l exactly once, in cart_order or
+ * any other order; a shell that repeats or lacks a component is
+ * FE_ERR_FORMAT).  Replaces `build_pairs <basis.bin> <pairs.bin>
  * [tol]` (fastesp/native/build_pairs.cpp:162-425) and the NumPy builder
  * ESPEngine + build_table (fastesp/esp.py:168-257, fastesp/table.py:176-259).
  * rule 0: the NumPy screen, keep = !(K bound / max(dmin, 0.05) < tol)
