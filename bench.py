@@ -96,7 +96,7 @@ def parse():
                     help="timed steps of the autograd / graph-captured leg (0 = skip it)")
     ap.add_argument("--eval-streams", type=int, choices=[1, 2], default=1,
                     help="streams of the evaluation class launches (1 = library default)")
-    ap.add_argument("--dg-streams", type=int, choices=[1, 2], default=2,
+    ap.add_argument("--dg-streams", type=int, choices=[1, 2, 3], default=2,
                     help="streams of the dL/dG class launches (2 = library default; 1 runs "
                          "them one at a time, so per-kernel counters and times are their own)")
     return ap.parse_args()
@@ -737,7 +737,7 @@ def main() -> int:
     log(f"generating workload (config 1, {a.points} points)")
     w = synth.config1(a.points)
     ctx = _lib.Context.get(0)
-    ctx.set_tuning(6, int(a.dg_streams == 2))
+    ctx.set_tuning(6, 0 if a.dg_streams == 1 else a.dg_streams)
     ctx.set_tuning(7, int(a.eval_streams == 2))
     if a.ppl:
         ctx.set_tuning(0, a.ppl)
@@ -940,7 +940,7 @@ def main() -> int:
     traffic_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_hbm_traffic.md"
     counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
     # with two streams the dL/dG classes run two at a time
-    overlapped = frozenset(16 + L for L in range(7)) if grad and a.dg_streams == 2 else frozenset()
+    overlapped = frozenset(16 + L for L in range(7)) if grad and a.dg_streams >= 2 else frozenset()
     if a.eval_streams == 2:
         overlapped |= frozenset(8 + L for L in range(7))
     roof = roofline(head["_phases"], counts, grad, shape["ppl_L"], shape["fp32_classes"],

@@ -397,33 +397,39 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     for (int L = 0; L <= FE_LMAX; ++L) launch_dp(L, st);
   } else {
     // The classes are independent (own partial rows, own reduction, own
-    // shell pairs), so they run on two streams with about the same work
-    // each: the tail of one class's launch fills with the other stream's
-    // workgroups, and the latency-bound dL/dP of a class runs beside the
-    // other stream's dL/dG.
-    hipStream_t s2 = st;
-    if (c->split_dg) {
-      rc = fe_fork(c, st, &s2);
-      if (rc) return rc;
+    // shell pairs), so they run on the context's dL/dG streams (tuning key
+    // 6) with about the same work each: the tail of one class's launch fills
+    // with another stream's workgroups, and the latency-bound dL/dP of a
+    // class runs beside another stream's dL/dG.
+    hipStream_t ss[3];
+    const int ns = c->dg_streams;
+    rc = fe_fork_n(c, st, ns, ss);
+    if (rc) return rc;
+    // Largest class first, each to the stream with the least estimated work
+    // so far; a class's dL/dG per (pair, point) costs about 20 + 2.2 NT
+    // instructions (NT its term count), measured on config 1.
+    double work[FE_LMAX + 1];
+    for (int L = 0; L <= FE_LMAX; ++L)
+      work[L] = (double)fe_plan_n(t, L) * (20.0 + 2.2 * fe_host::NTERM[L]);
+    int order[FE_LMAX + 1], q[FE_LMAX + 1];
+    fe_lpt(work, ns, order, q);
+    for (int k = 0; k <= FE_LMAX; ++k) {
+      const int L = order[k];
+      hipStream_t s = ss[q[k]];
+      switch (L) {
+        case 0: launch_dG_L<0>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 1: launch_dG_L<1>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 2: launch_dG_L<2>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 3: launch_dG_L<3>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 4: launch_dG_L<4>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 5: launch_dG_L<5>(t, d, pw, (int)n, part, dG, flat, s); break;
+        case 6: launch_dG_L<6>(t, d, pw, (int)n, part, dG, flat, s); break;
+        default: break;
+      }
+      launch_dp(L, s);
     }
-    launch_dG_L<1>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dp(1, st);
-    launch_dG_L<2>(t, d, pw, (int)n, part, dG, flat, s2);
-    launch_dp(2, s2);
-    launch_dG_L<3>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dp(3, st);
-    launch_dG_L<0>(t, d, pw, (int)n, part, dG, flat, s2);
-    launch_dp(0, s2);
-    launch_dG_L<5>(t, d, pw, (int)n, part, dG, flat, st);
-    launch_dp(5, st);
-    launch_dG_L<4>(t, d, pw, (int)n, part, dG, flat, s2);
-    launch_dp(4, s2);
-    launch_dG_L<6>(t, d, pw, (int)n, part, dG, flat, s2);
-    launch_dp(6, s2);
-    if (c->split_dg) {
-      rc = fe_join(c, st);
-      if (rc) return rc;
-    }
+    rc = fe_join_n(c, st, ns);
+    if (rc) return rc;
   }
   FE_HIPCHK(hipGetLastError());
   if (ptr_kind == FE_PTR_HOST) {

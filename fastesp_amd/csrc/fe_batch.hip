@@ -923,27 +923,35 @@ static int launch_backward(fe_batch* b, hipStream_t st) {
                        b->pts, b->gV, b->perm, (int)b->n_tot, b->pw);
     fe_prof_end(b->ctx, pp, st);
   }
-  // the classes on two streams (as in the single-table backward), each class's
+  // the classes on the context's dL/dG streams (as in the single-table
+  // backward: largest first, each to the least loaded stream), each class's
   // dL/dP right after its dL/dG
-  hipStream_t s2 = st;
-  if (b->ctx->split_dg)
-    if (int rc = fe_fork(b->ctx, st, &s2)) return rc;
-  launch_dG_L<1>(b, st);
-  launch_dp_L(b, 1, st);
-  launch_dG_L<2>(b, s2);
-  launch_dp_L(b, 2, s2);
-  launch_dG_L<3>(b, st);
-  launch_dp_L(b, 3, st);
-  launch_dG_L<0>(b, s2);
-  launch_dp_L(b, 0, s2);
-  launch_dG_L<5>(b, st);
-  launch_dp_L(b, 5, st);
-  launch_dG_L<4>(b, s2);
-  launch_dp_L(b, 4, s2);
-  launch_dG_L<6>(b, s2);
-  launch_dp_L(b, 6, s2);
-  if (b->ctx->split_dg)
-    if (int rc = fe_join(b->ctx, st)) return rc;
+  hipStream_t ss[3];
+  const int ns = b->ctx->dg_streams;
+  if (int rc = fe_fork_n(b->ctx, st, ns, ss)) return rc;
+  double work[FE_LMAX + 1] = {};
+  for (int i = 0; i < b->n_conf; ++i)
+    for (int L = 0; L <= FE_LMAX; ++L)
+      work[L] += (double)fe_plan_n(b->tabs[i], L) * (double)(b->pt_off[i + 1] - b->pt_off[i]);
+  for (int L = 0; L <= FE_LMAX; ++L) work[L] *= 20.0 + 2.2 * fe_host::NTERM[L];
+  int order[FE_LMAX + 1], q[FE_LMAX + 1];
+  fe_lpt(work, ns, order, q);
+  for (int k = 0; k <= FE_LMAX; ++k) {
+    const int L = order[k];
+    hipStream_t s = ss[q[k]];
+    switch (L) {
+      case 0: launch_dG_L<0>(b, s); break;
+      case 1: launch_dG_L<1>(b, s); break;
+      case 2: launch_dG_L<2>(b, s); break;
+      case 3: launch_dG_L<3>(b, s); break;
+      case 4: launch_dG_L<4>(b, s); break;
+      case 5: launch_dG_L<5>(b, s); break;
+      case 6: launch_dG_L<6>(b, s); break;
+      default: break;
+    }
+    launch_dp_L(b, L, s);
+  }
+  if (int rc = fe_join_n(b->ctx, st, ns)) return rc;
   return FE_OK;
 }
 

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -227,18 +228,24 @@ extern "C" int fe_init(int device, fe_ctx** out) {
   // the second stream of the context (table build, dL/dG classes), created
   // here so that no later call creates one (say, while a graph is captured)
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->side2) (void)hipStreamDestroy(c->side2);
     if (c->side) (void)hipStreamDestroy(c->side);
     (void)hipFree(c->d_boys);
     delete c;
     return fe_set_error(FE_ERR_HIP, "fe_init: second stream creation failed");
   }
   if (int rc = fe_noalloc_init(c)) {
+    (void)hipEventDestroy(c->ev_join2);
     (void)hipEventDestroy(c->ev_join);
     (void)hipEventDestroy(c->ev_fork);
+    (void)hipStreamDestroy(c->side2);
     (void)hipStreamDestroy(c->side);
     (void)hipFree(c->d_boys);
     delete c;
@@ -251,12 +258,14 @@ extern "C" int fe_init(int device, fe_ctx** out) {
 extern "C" int fe_finalize(fe_ctx* c) {
   if (!c) return fe_set_error(FE_ERR_ARG, "fe_finalize: null context");
   FE_HIPCHK(hipSetDevice(c->device));
-  if (c->side) {
-    (void)hipStreamSynchronize(c->side);  // its launches read the workspace slots
-    (void)hipStreamDestroy(c->side);
-  }
+  for (hipStream_t s : {c->side, c->side2})
+    if (s) {
+      (void)hipStreamSynchronize(s);  // its launches read the workspace slots
+      (void)hipStreamDestroy(s);
+    }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
   for (int s = 0; s < FE_WS_SLOTS; ++s)
     if (c->ws[s]) (void)hipFree(c->ws[s]);
   c->basis_cache.reset();
@@ -285,11 +294,6 @@ extern "C" int fe_ctx_trim(fe_ctx* c) {
 }
 
 int fe_fork(fe_ctx* c, hipStream_t st, hipStream_t* side) {
-  if (!c->side) {
-    FE_HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-    FE_HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    FE_HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-  }
   FE_HIPCHK(hipEventRecord(c->ev_fork, st));
   FE_HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
   *side = c->side;
@@ -300,6 +304,43 @@ int fe_join(fe_ctx* c, hipStream_t st) {
   FE_HIPCHK(hipEventRecord(c->ev_join, c->side));
   FE_HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
   return FE_OK;
+}
+
+int fe_fork_n(fe_ctx* c, hipStream_t st, int n, hipStream_t* s) {
+  s[0] = s[1] = s[2] = st;
+  if (n < 2) return FE_OK;
+  FE_HIPCHK(hipEventRecord(c->ev_fork, st));
+  FE_HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  s[1] = c->side;
+  if (n >= 3) {
+    FE_HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork, 0));
+    s[2] = c->side2;
+  }
+  return FE_OK;
+}
+
+int fe_join_n(fe_ctx* c, hipStream_t st, int n) {
+  if (n < 2) return FE_OK;
+  FE_HIPCHK(hipEventRecord(c->ev_join, c->side));
+  FE_HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
+  if (n >= 3) {
+    FE_HIPCHK(hipEventRecord(c->ev_join2, c->side2));
+    FE_HIPCHK(hipStreamWaitEvent(st, c->ev_join2, 0));
+  }
+  return FE_OK;
+}
+
+void fe_lpt(const double* work, int n, int* order, int* q) {
+  for (int L = 0; L <= FE_LMAX; ++L) order[L] = L;
+  std::stable_sort(order, order + FE_LMAX + 1, [&](int a, int b) { return work[a] > work[b]; });
+  double load[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k <= FE_LMAX; ++k) {
+    int best = 0;
+    for (int j = 1; j < n; ++j)
+      if (load[j] < load[best]) best = j;
+    load[best] += work[order[k]];
+    q[k] = best;
+  }
 }
 
 extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
@@ -328,7 +369,7 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       c->pair_order = value != 0;
       return FE_OK;
     case 6:
-      c->split_dg = value != 0;
+      c->dg_streams = value <= 0 ? 1 : value == 1 ? 2 : (int)std::min<int64_t>(value, 3);
       return FE_OK;
     case 7:
       c->split_eval = value != 0;

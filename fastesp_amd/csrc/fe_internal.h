@@ -153,20 +153,28 @@ struct fe_ctx {
   // the last basis fe_table_build set up (host layout + device copy), reused
   // when the next build passes the same basis arrays
   std::shared_ptr<void> basis_cache;
-  // A second stream for launches that are independent of each other (the
-  // dL/dG classes): fe_fork makes it wait for the caller's stream, fe_join
-  // makes the caller's stream wait for it.  Default priority, non-blocking;
-  // created at the first fork.  Under graph capture the fork and join become
-  // graph edges.
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool split_dg = true;    // run the dL/dG classes on two streams (tuning key 6)
+  // Two more streams for launches that are independent of each other (the
+  // dL/dG classes, the heavy candidates of a table build): fe_fork makes
+  // them wait for the caller's stream, fe_join makes the caller's stream
+  // wait for them.  Default priority, non-blocking; created by fe_init.
+  // Under graph capture the fork and join become graph edges.
+  hipStream_t side = nullptr, side2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
+  int dg_streams = 2;       // streams of the dL/dG classes, 1-3 (tuning key 6)
   bool split_eval = false;  // run the evaluation classes on two streams (tuning key 7)
 };
 
 // fork / join of the context's second stream with the caller's stream st
 int fe_fork(fe_ctx* c, hipStream_t st, hipStream_t* side);
 int fe_join(fe_ctx* c, hipStream_t st);
+// the same for n streams in all (1-3): s[0] = st, s[1..n-1] the context's
+int fe_fork_n(fe_ctx* c, hipStream_t st, int n, hipStream_t* s);
+int fe_join_n(fe_ctx* c, hipStream_t st, int n);
+// The order and stream of the class launches of a backward over n streams:
+// largest estimated work first, each to the stream with the least so far.
+// work[L] is the class's estimated cost; q[k] receives the stream index of
+// the k-th class of order.
+void fe_lpt(const double* work, int n, int* order, int* q);
 
 #define FE_CTX_LOCK(c) std::lock_guard<std::recursive_mutex> fe_ctx_lock_((c)->mu)
 

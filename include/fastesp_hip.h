@@ -231,9 +231,10 @@ int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
  *        (default 1e8, i.e. 1e-4 a.u.; 0 sends every pair to FP64);
  * key 5: order the pairs of each class by exponent bucket, then Morton code
  *        of P, at table creation (0/1, default 1; export keeps reference order);
- * key 6: run the dL/dG class launches of fe_esp_backward on two streams (the
- *        caller's and a second one the context owns, joined before dL/dP;
- *        0/1, default 1);
+ * key 6: streams of the dL/dG class launches of fe_esp_backward and of the
+ *        batched backward (the caller's and streams the context owns, joined
+ *        before the call returns; largest class first, each to the least
+ *        loaded stream): 0 one stream, 1 or 2 two (default), 3 three;
  * key 7: run the evaluation class launches on two streams, each with its own
  *        partial rows (0/1, default 0: at config 1 it measured no faster);
  * key 20 + L: class L takes the FP32 contraction under FE_PREC_MIXED (0/1;

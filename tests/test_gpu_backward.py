@@ -185,9 +185,9 @@ def test_trim_releases_workspace_and_next_call_works():
 
 
 def test_stream_modes_agree():
-    """The dL/dG classes on two streams (default) or one at a time give the
-    same bits; the evaluation classes on two streams (tuning key 7, own
-    partial rows) change only the summation order."""
+    """The dL/dG classes on two streams (default), three, or one at a time
+    give the same bits; the evaluation classes on two streams (tuning key 7,
+    own partial rows) change only the summation order."""
     from fastesp_amd import _lib
     ctx = _lib.Context.get(0)
     w = synth.config0(1500)
@@ -195,8 +195,8 @@ def test_stream_modes_agree():
     gV = np.linspace(-1.0, 1.0, len(w.points))
     out = {}
     try:
-        for dg2, ev2 in ((1, 0), (0, 0), (1, 1)):
-            say(f"dL/dG streams {1 + dg2}, evaluation streams {1 + ev2}")
+        for dg2, ev2 in ((1, 0), (0, 0), (3, 0), (1, 1)):
+            say(f"dL/dG tuning {dg2}, evaluation streams {1 + ev2}")
             ctx.set_tuning(6, dg2)
             ctx.set_tuning(7, ev2)
             V, F = esp_with_grad(t, w.points)
@@ -205,8 +205,8 @@ def test_stream_modes_agree():
     finally:
         ctx.set_tuning(6, 1)
         ctx.set_tuning(7, 0)
-    for a, b in zip(out[(1, 0)], out[(0, 0)]):
-        assert np.array_equal(a, b)
+    for a, b, c in zip(out[(1, 0)], out[(0, 0)], out[(3, 0)]):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
     V0, F0, dG0, dP0 = out[(1, 0)]
     V2, F2, dG2, dP2 = out[(1, 1)]
     assert np.abs(V2 - V0).max() < 1e-12 and np.abs(F2 - F0).max() < 1e-12
