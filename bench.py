@@ -48,6 +48,12 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# the library's default points per lane of the value kernels and of the value
+# + field kernels (fe_ctx in fastesp_amd/csrc/fe_internal.h), and its default
+# target workgroups per launch (32 per compute unit)
+LIB_PPL_FWD = [2, 2, 2, 2, 2, 2, 4]
+LIB_PPL_GRAD = [2, 2, 2, 2, 2, 1, 1]
+
 # published build + FP64 forward of the reference (RTX 5090 + 16 CPU cores),
 # ms per 1e4 points (benchmarks/README.md:50-52)
 REFERENCE_BUILD_PLUS_FWD = {300: 8959.0, 3000: 1598.0, 30000: 819.0}
@@ -836,14 +842,16 @@ def main() -> int:
     from fastesp_amd import autotune
     tuned = {}
 
-    def current_shape():
-        return dict(ppl_L=[ctx_ppl[L] for L in range(7)], target_wg=ctx_wg[0],
+    def current_shape(mode):
+        # without explicit values, the library's defaults for the mode
+        ppl = ctx_ppl or (LIB_PPL_FWD if mode == "fwd" else LIB_PPL_GRAD)
+        return dict(ppl_L=[ppl[L] for L in range(7)], target_wg=ctx_wg[0],
                     fp32_classes=[ctx_fp32[L] for L in range(7)])
 
-    ctx_ppl = [a.ppl or 2] * 7
+    ctx_ppl = [a.ppl] * 7 if a.ppl else None
     if a.ppl_classes:
         ctx_ppl = [int(x) for x in a.ppl_classes.split(",")]
-    ctx_wg = [a.wg or 16 * 256]
+    ctx_wg = [a.wg or 32 * 256]
     ctx_fp32 = [0, 0, 0, 1, 1, 1, 1]
     if a.fp32_classes:
         ctx_fp32 = [int(x) for x in a.fp32_classes.split(",")]
@@ -863,7 +871,7 @@ def main() -> int:
             log(f"autotune {mode} {prec}: best ppl_L={best['ppl_L']} wg={best['target_wg']} "
                 f"{best['ms']:.2f} ms (first {r['all'][0]['ms']:.2f} ms, {len(r['all'])} trials)")
         else:
-            shape = current_shape()
+            shape = current_shape(mode)
             if force32:
                 shape["fp32_classes"] = [1] * 7
         autotune.apply(ctx, shape["ppl_L"], shape["target_wg"],

@@ -84,7 +84,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
     if (rc) return rc;
     fe_prof_end(c, ps, st);
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg, c->split_eval);
+  FwdPlan pl = fe_plan(t, n, c->ppl_grad_L, c->target_wg, c->split_eval);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
   const int groups = fe_row_groups((int)n, pl.rows);
   const size_t gplane = (size_t)groups * pl.ld;
@@ -222,11 +222,14 @@ static DGPlan plan_dG(const fe_table* t, int64_t n, int target_wg) {
     d.dg_off[L] = d.dg_total;
     if (!nL) continue;
     const int NT = fe_host::NTERM[L];
-    const int64_t nblk = (nL + 255) / 256;
+    // the point chunks from the pairs expected to be kept (blocks past the
+    // kept count exit at once), the buffers for all nL
+    const int64_t nE = fe_plan_est(t, L);
+    const int64_t nblk = (nE + 255) / 256;
     int64_t ch = std::max<int64_t>(1, target_wg / nblk);
     // bound the partial buffer (written and re-read once) and keep >= 64
     // points per chunk so the reduction stays small next to the kernel
-    ch = std::min<int64_t>(ch, std::max<int64_t>(1, (16ll << 20) / (nL * NT)));
+    ch = std::min<int64_t>(ch, std::max<int64_t>(1, (16ll << 20) / (nE * NT)));
     ch = std::min<int64_t>(ch, std::max<int64_t>(1, n / 64));
     ch = std::min<int64_t>(ch, 65535);
     const int64_t ppc = (n + ch - 1) / ch;
@@ -410,7 +413,7 @@ extern "C" int fe_esp_backward(fe_table* t, const double* pts, int64_t n, const 
     // instructions (NT its term count), measured on config 1.
     double work[FE_LMAX + 1];
     for (int L = 0; L <= FE_LMAX; ++L)
-      work[L] = (double)fe_plan_n(t, L) * (20.0 + 2.2 * fe_host::NTERM[L]);
+      work[L] = (double)fe_plan_est(t, L) * (20.0 + 2.2 * fe_host::NTERM[L]);
     int order[FE_LMAX + 1], q[FE_LMAX + 1];
     fe_lpt(work, ns, order, q);
     for (int k = 0; k <= FE_LMAX; ++k) {
@@ -580,7 +583,7 @@ extern "C" int fe_eval_work(fe_table* t, const double* pts, int64_t n, int grad,
   unsigned long long* d_out = (unsigned long long*)c->ws[FE_WS_AUX3];
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, (double*)d_out,
                      (int64_t)(4 * (FE_LMAX + 1)), 0.0);
-  const FwdPlan pl = fe_plan(t, n, c->ppl_L, c->target_wg);
+  const FwdPlan pl = fe_plan(t, n, grad ? c->ppl_grad_L : c->ppl_L, c->target_wg);
   FE_FOR_EACH_CLASS(pl, count_eval_LP, t, pl, s_pts, (int)n, grad != 0, d_out, st);
   const DGPlan d = plan_dG(t, n, c->target_wg);
   count_dg_L<0>(t, d, d_pts, (int)n, d_out, st);

@@ -932,7 +932,7 @@ static int launch_backward(fe_batch* b, hipStream_t st) {
   double work[FE_LMAX + 1] = {};
   for (int i = 0; i < b->n_conf; ++i)
     for (int L = 0; L <= FE_LMAX; ++L)
-      work[L] += (double)fe_plan_n(b->tabs[i], L) * (double)(b->pt_off[i + 1] - b->pt_off[i]);
+      work[L] += (double)fe_plan_est(b->tabs[i], L) * (double)(b->pt_off[i + 1] - b->pt_off[i]);
   for (int L = 0; L <= FE_LMAX; ++L) work[L] *= 20.0 + 2.2 * fe_host::NTERM[L];
   int order[FE_LMAX + 1], q[FE_LMAX + 1];
   fe_lpt(work, ns, order, q);

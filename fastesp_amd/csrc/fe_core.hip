@@ -224,7 +224,7 @@ extern "C" int fe_init(int device, fe_ctx** out) {
     delete c;
     return fe_set_error(FE_ERR_HIP, "fe_init: Boys table upload failed");
   }
-  c->target_wg = 16 * c->n_cu;
+  c->target_wg = 32 * c->n_cu;
   // the second stream of the context (table build, dL/dG classes), created
   // here so that no later call creates one (say, while a graph is captured)
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
@@ -350,13 +350,13 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
     case 0:  // points per lane of every class
       if (value != 1 && value != 2 && value != 4)
         return fe_set_error(FE_ERR_ARG, "points per lane must be 1, 2 or 4");
-      for (int L = 0; L <= FE_LMAX; ++L) c->ppl_L[L] = (int)value;
+      for (int L = 0; L <= FE_LMAX; ++L) c->ppl_L[L] = c->ppl_grad_L[L] = (int)value;
       return FE_OK;
     case 10: case 11: case 12: case 13: case 14: case 15: case 16:  // per class L = key-10
       if (key - 10 > FE_LMAX) return fe_set_error(FE_ERR_ARG, "class out of range");
       if (value != 1 && value != 2 && value != 4)
         return fe_set_error(FE_ERR_ARG, "points per lane must be 1, 2 or 4");
-      c->ppl_L[key - 10] = (int)value;
+      c->ppl_L[key - 10] = c->ppl_grad_L[key - 10] = (int)value;
       return FE_OK;
     case 1:
       if (value < 1 || value > (1 << 24)) return fe_set_error(FE_ERR_ARG, "bad target workgroups");

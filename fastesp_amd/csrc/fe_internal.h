@@ -1,5 +1,7 @@
 // Internal types shared by the translation units of libfastesp_hip.
 #pragma once
+
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -135,7 +137,10 @@ struct fe_ctx {
   double* d_boys = nullptr;  // (FE_BOYS_MMAX+1) x NR x COLS
   void* ws[FE_WS_SLOTS] = {};
   size_t ws_bytes[FE_WS_SLOTS] = {};
-  int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 2};  // points per lane, per class
+  // points per lane, per class, of the value kernels and of the value + field
+  // kernels (the shapes the autotuner picks at config 1)
+  int ppl_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 2, 4};
+  int ppl_grad_L[FE_LMAX + 1] = {2, 2, 2, 2, 2, 1, 1};
   int target_wg = 4096; // workgroups per class launch
   bool sort_points = true;  // Morton-order the points before evaluation
   bool pair_order = true;   // order each class by exponent bucket, then Morton code of P
@@ -258,6 +263,13 @@ struct fe_table {
 // knows it, else the allocated capacity (the kernels read the kept count)
 inline int64_t fe_plan_n(const fe_table* t, int L) {
   return t->counts_stale ? t->cap_L[L] : t->n_L[L];
+}
+// the pairs of class L a launch shape is chosen for (work split, stream
+// balance): after an in-place rebuild the last kept count the host saw (the
+// kept set seldom moves much), else fe_plan_n; buffers and grids still cover
+// fe_plan_n
+inline int64_t fe_plan_est(const fe_table* t, int L) {
+  return t->counts_stale && t->n_L[L] > 0 ? std::min(t->n_L[L], t->cap_L[L]) : fe_plan_n(t, L);
 }
 // one device allocation for a table: statistics, records (cap[L] per class),
 // flat offsets and reference-order indices, plus n_slots ints of slot map

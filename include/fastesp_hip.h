@@ -224,8 +224,12 @@ int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
                  void* stream, double* F);
 
 /* Tuning knobs of the single-table evaluation (autotuner output).
- * key 0: points per lane of every class (1, 2 or 4); 10 + L: of class L only;
- * key 1: target workgroups per class launch (sets the pair-chunk count);
+ * key 0: points per lane of every class (1, 2 or 4); 10 + L: of class L only
+ *        (both set the value and the value + field kernels; defaults 2, with
+ *        4 for class 6 of the value kernels and 1 for classes 5 and 6 with the
+ *        field);
+ * key 1: target workgroups per class launch (sets the pair-chunk count;
+ *        default 32 per compute unit);
  * key 3: Morton-order the points before evaluation (0/1, default 1);
  * key 4: FP32 routing threshold of FE_PREC_MIXED in units of 1e-12 a.u.
  *        (default 1e8, i.e. 1e-4 a.u.; 0 sends every pair to FP64);
