@@ -100,8 +100,9 @@ def parse():
                     help="point counts of the build + FP64 forward legs ('' = skip)")
     ap.add_argument("--autograd-steps", type=int, default=10,
                     help="timed steps of the autograd / graph-captured leg (0 = skip it)")
-    ap.add_argument("--eval-streams", type=int, choices=[1, 2], default=1,
-                    help="streams of the evaluation class launches (1 = library default)")
+    ap.add_argument("--eval-streams", type=int, choices=[0, 1, 2], default=0,
+                    help="streams of the evaluation class launches (0 = library default: two "
+                         "for the value kernels, one with the field)")
     ap.add_argument("--dg-streams", type=int, choices=[1, 2, 3], default=2,
                     help="streams of the dL/dG class launches (2 = library default; 1 runs "
                          "them one at a time, so per-kernel counters and times are their own)")
@@ -744,7 +745,8 @@ def main() -> int:
     w = synth.config1(a.points)
     ctx = _lib.Context.get(0)
     ctx.set_tuning(6, 0 if a.dg_streams == 1 else a.dg_streams)
-    ctx.set_tuning(7, int(a.eval_streams == 2))
+    if a.eval_streams:
+        ctx.set_tuning(7, int(a.eval_streams == 2))
     if a.ppl:
         ctx.set_tuning(0, a.ppl)
     if a.ppl_classes:
@@ -949,7 +951,9 @@ def main() -> int:
     counters_file = ROOT / "profiles" / f"esp_cfg1_30k_{key}_valu_counters.md"
     # with two streams the dL/dG classes run two at a time
     overlapped = frozenset(16 + L for L in range(7)) if grad and a.dg_streams >= 2 else frozenset()
-    if a.eval_streams == 2:
+    # the evaluation classes run two at a time when asked (2), or by the
+    # library default (0) for the value kernels
+    if a.eval_streams == 2 or (a.eval_streams == 0 and not grad):
         overlapped |= frozenset(8 + L for L in range(7))
     roof = roofline(head["_phases"], counts, grad, shape["ppl_L"], shape["fp32_classes"],
                     a.precision, ppl, n, traffic_file, counters_file, overlapped)

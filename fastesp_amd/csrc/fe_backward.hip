@@ -84,7 +84,7 @@ extern "C" int fe_esp_forward_grad(fe_table* t, const double* pts, int64_t n, in
     if (rc) return rc;
     fe_prof_end(c, ps, st);
   }
-  FwdPlan pl = fe_plan(t, n, c->ppl_grad_L, c->target_wg, c->split_eval);
+  FwdPlan pl = fe_plan(t, n, c->ppl_grad_L, c->target_wg, c->split_eval_grad);
   const size_t plane = (size_t)std::max(pl.rows, 1) * pl.ld;
   const int groups = fe_row_groups((int)n, pl.rows);
   const size_t gplane = (size_t)groups * pl.ld;

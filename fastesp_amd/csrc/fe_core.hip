@@ -373,6 +373,7 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       return FE_OK;
     case 7:
       c->split_eval = value != 0;
+      c->split_eval_grad = value == 1;
       return FE_OK;
     case 20: case 21: case 22: case 23: case 24: case 25: case 26:  // FP32 contraction of class L
       c->mixed_L[key - 20] = value != 0;

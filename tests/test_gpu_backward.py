@@ -204,7 +204,7 @@ def test_stream_modes_agree():
             out[(dg2, ev2)] = (V, F, dG, dP)
     finally:
         ctx.set_tuning(6, 1)
-        ctx.set_tuning(7, 0)
+        ctx.set_tuning(7, 2)
     for a, b, c in zip(out[(1, 0)], out[(0, 0)], out[(3, 0)]):
         assert np.array_equal(a, b) and np.array_equal(a, c)
     V0, F0, dG0, dP0 = out[(1, 0)]
