@@ -101,8 +101,9 @@ def parse():
     ap.add_argument("--autograd-steps", type=int, default=10,
                     help="timed steps of the autograd / graph-captured leg (0 = skip it)")
     ap.add_argument("--eval-streams", type=int, choices=[0, 1, 2], default=0,
-                    help="streams of the evaluation class launches (0 = library default: two "
-                         "for the value kernels, one with the field)")
+                    help="streams of the evaluation class launches (0 = library default: the "
+                         "value kernels in two halves, with the field classes 5 and 6 beside "
+                         "the others; 1 one stream; 2 two halves)")
     ap.add_argument("--dg-streams", type=int, choices=[1, 2, 3], default=2,
                     help="streams of the dL/dG class launches (2 = library default; 1 runs "
                          "them one at a time, so per-kernel counters and times are their own)")
@@ -952,9 +953,12 @@ def main() -> int:
     # with two streams the dL/dG classes run two at a time
     overlapped = frozenset(16 + L for L in range(7)) if grad and a.dg_streams >= 2 else frozenset()
     # the evaluation classes run two at a time when asked (2), or by the
-    # library default (0) for the value kernels
+    # library default (0): the value kernels in two halves, the value + field
+    # kernels with classes 5 and 6 beside classes 0 and 1
     if a.eval_streams == 2 or (a.eval_streams == 0 and not grad):
         overlapped |= frozenset(8 + L for L in range(7))
+    elif a.eval_streams == 0:
+        overlapped |= frozenset({8, 9, 13, 14})
     roof = roofline(head["_phases"], counts, grad, shape["ppl_L"], shape["fp32_classes"],
                     a.precision, ppl, n, traffic_file, counters_file, overlapped)
     fl_launch = launch_flops(counts, grad)
@@ -976,7 +980,7 @@ def main() -> int:
                          tflops=round(v / (head["_phases"][k][0] / head["_phases"][k][1] * 1e-3) / 1e12, 2))
                          if k not in overlapped else dict(
                          span_ms=round(head["_phases"][k][0] / head["_phases"][k][1], 4),
-                         gflop=round(v / 1e9, 2), note="runs beside another dL/dG class"))
+                         gflop=round(v / 1e9, 2), note="runs beside a launch on the other stream"))
                          for k, v in sorted(fl_launch.items()) if k in head["_phases"]},
                      work_counts=counts)
     for leg in legs.values():

@@ -372,8 +372,8 @@ extern "C" int fe_set_tuning(fe_ctx* c, int key, int64_t value) {
       c->dg_streams = value <= 0 ? 1 : value == 1 ? 2 : (int)std::min<int64_t>(value, 3);
       return FE_OK;
     case 7:
-      c->split_eval = value != 0;
-      c->split_eval_grad = value == 1;
+      c->split_eval = value != 0 ? 1 : 0;
+      c->split_eval_grad = value == 1 ? 1 : value >= 2 ? 2 : 0;
       return FE_OK;
     case 20: case 21: case 22: case 23: case 24: case 25: case 26:  // FP32 contraction of class L
       c->mixed_L[key - 20] = value != 0;
@@ -787,7 +787,7 @@ __global__ __launch_bounds__(256) void k_boys_eval(const double* __restrict__ T,
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
-FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, bool split) {
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, int split) {
   FwdPlan pl{};
   pl.ld = (int)((n + 1023) / 1024 * 1024);
   pl.split = split;
@@ -813,7 +813,7 @@ FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, boo
   int rows = 0;
   const int n_sets = split ? 2 : 1;
   for (int g = 0; g < n_sets; ++g) {
-    const int* order = split ? FE_ORDER_SPLIT[g] : FE_ORDER_ONE;
+    const int* order = split == 2 ? FE_ORDER_SMALL[g] : split ? FE_ORDER_SPLIT[g] : FE_ORDER_ONE;
     int set_rows = 0;
     for (int k = 0; order[k] >= 0; ++k) {
       const int L = order[k];

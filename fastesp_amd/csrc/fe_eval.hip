@@ -119,9 +119,24 @@ int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, dou
     for (int k = 0; FE_ORDER_ONE[k] >= 0; ++k)
       if (FE_ORDER_ONE[k] <= FE_LMAX)
         launch_eval_class(FE_ORDER_ONE[k], t, pl, pts, n, partial, plane, grad, mixed, tau, st);
+  } else if (pl.split == 2) {
+    // classes 6 and 5 on the second stream beside classes 0 and 1; joined
+    // before class 2, so the large classes run alone
+    hipStream_t s2 = st;
+    int rc = fe_fork(t->ctx, st, &s2);
+    if (rc) return rc;
+    launch_eval_class(6, t, pl, pts, n, partial, plane, grad, mixed, tau, s2);
+    launch_eval_class(0, t, pl, pts, n, partial, plane, grad, mixed, tau, st);
+    launch_eval_class(5, t, pl, pts, n, partial, plane, grad, mixed, tau, s2);
+    launch_eval_class(1, t, pl, pts, n, partial, plane, grad, mixed, tau, st);
+    rc = fe_join(t->ctx, st);
+    if (rc) return rc;
+    for (int L = 2; L <= 4; ++L)
+      launch_eval_class(L, t, pl, pts, n, partial, plane, grad, mixed, tau, st);
   } else {
-    // the two row sets are independent: the odd classes on the caller's
-    // stream, the even ones on the context's second stream
+    // the two row sets are independent: the first on the caller's stream,
+    // the second on the context's second stream
+    const auto& order = pl.split == 2 ? FE_ORDER_SMALL : FE_ORDER_SPLIT;
     hipStream_t s2 = st;
     int rc = fe_fork(t->ctx, st, &s2);
     if (rc) return rc;
@@ -130,10 +145,9 @@ int fe_launch_eval(fe_table* t, const FwdPlan& pl, const double* pts, int n, dou
     for (int k = 0; k <= FE_LMAX; ++k)
       for (int g = 0; g < 2; ++g) {
         int len = 0;
-        while (FE_ORDER_SPLIT[g][len] >= 0) ++len;
-        if (k < len && FE_ORDER_SPLIT[g][k] <= FE_LMAX)
-          launch_eval_class(FE_ORDER_SPLIT[g][k], t, pl, pts, n, partial, plane, grad, mixed, tau,
-                            sg[g]);
+        while (order[g][len] >= 0) ++len;
+        if (k < len && order[g][k] <= FE_LMAX)
+          launch_eval_class(order[g][k], t, pl, pts, n, partial, plane, grad, mixed, tau, sg[g]);
       }
     rc = fe_join(t->ctx, st);
     if (rc) return rc;

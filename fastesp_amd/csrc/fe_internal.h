@@ -167,10 +167,11 @@ struct fe_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
   int dg_streams = 2;       // streams of the dL/dG classes, 1-3 (tuning key 6)
   // the evaluation classes on two streams, each with its own partial rows
-  // (tuning key 7): the value kernels by default (forward 4.31 -> 4.16 ms at
-  // config 1), the value + field kernels on request
-  bool split_eval = true;
-  bool split_eval_grad = false;
+  // (tuning key 7, FwdPlan::split): the value kernels in two balanced halves
+  // (forward 4.31 -> 4.16 ms at config 1), the value + field kernels with
+  // classes 5 and 6 beside the first large ones
+  int split_eval = 1;
+  int split_eval_grad = 2;
 };
 
 // fork / join of the context's second stream with the caller's stream st
@@ -302,11 +303,14 @@ int fe_table_rebuild_ws(fe_table* t, const double* dens, int n_points, const dou
 // context's second stream, each set with rows of its own.  -1 ends a list.
 static constexpr int FE_ORDER_ONE[FE_LMAX + 2] = {0, 1, 2, 3, 4, 5, 6, -1};
 static constexpr int FE_ORDER_SPLIT[2][FE_LMAX + 2] = {{1, 3, 5, -1}, {2, 0, 4, 6, -1}};
+// the two small, latency-bound classes beside classes 0 and 1 (the row sets
+// of FwdPlan::split 2; fe_launch_eval joins the streams before class 2)
+static constexpr int FE_ORDER_SMALL[2][FE_LMAX + 2] = {{0, 1, 2, 3, 4, -1}, {6, 5, -1}};
 
 struct FwdPlan {
   int ld = 0;     // leading dimension of a partial row (>= n, multiple of 1024)
   int rows = 0;   // partial rows (all row sets)
-  bool split = false;           // two streams, two row sets
+  int split = 0;  // 0 one stream; two streams, two row sets: 1 FE_ORDER_SPLIT, 2 FE_ORDER_SMALL
   int n_pb[FE_LMAX + 1] = {};   // point blocks of the class launch
   int ppl[FE_LMAX + 1] = {};    // points per lane of the class launch (1, 2, 4)
   int chunks[FE_LMAX + 1] = {};
@@ -323,7 +327,7 @@ int fe_table_set_atoms(fe_table* t, int n_atoms, const double* Z, const double* 
 // adjoint, whose far-field test is per wavefront.  rec4 holds (Px, Py, Pz, p)
 // at a stride of `stride` doubles; returns a stable permutation.
 std::vector<int> fe_pair_order(const double* rec4, size_t stride, int64_t n);
-FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, bool split = false);
+FwdPlan fe_plan(const fe_table* t, int64_t n, const int* ppl, int target_wg, int split = 0);
 
 // dispatch a templated class launcher on the plan's points-per-lane of class L
 #define FE_DISPATCH_PPL(pl, L, FN, ...)                  \

@@ -240,9 +240,10 @@ int fe_boys_eval(fe_ctx* ctx, const double* T, int64_t n, int M, int ptr_kind,
  *        before the call returns; largest class first, each to the least
  *        loaded stream): 0 one stream, 1 or 2 two (default), 3 three;
  * key 7: run the evaluation class launches on two streams, each with its own
- *        partial rows: 0 never, 1 always, 2 (default) for the value kernels
- *        only (config 1: forward 4.31 -> 4.16 ms; value + field 0.6% faster,
- *        not worth the overlapped launches in its per-kernel measurement);
+ *        partial rows: 0 never; 1 always, in two balanced halves; 2 (default)
+ *        the value kernels in two balanced halves (config 1: forward 4.31 ->
+ *        4.16 ms) and the value + field kernels with the small classes 5 and
+ *        6 beside the first large ones;
  * key 20 + L: class L takes the FP32 contraction under FE_PREC_MIXED (0/1;
  *        default 1 for L >= 3, else the class runs the FP64 kernel). */
 int fe_set_tuning(fe_ctx* ctx, int key, int64_t value);
