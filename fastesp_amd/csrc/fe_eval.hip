@@ -96,6 +96,7 @@ static void launch_eval_LP(const fe_table* t, const FwdPlan& pl, const double* p
 static void launch_eval_class(int L, const fe_table* t, const FwdPlan& pl, const double* pts,
                               int n, double* partial, size_t plane, bool grad, bool mixed,
                               float tau, hipStream_t st) {
+  if (pl.chunks[L] == 0 || pl.n_pb[L] == 0) return;  // no pairs (or no points)
   double* rows = partial + (size_t)pl.row0[L] * pl.ld;
   switch (L) {
     case 0: FE_DISPATCH_PPL(pl, 0, launch_eval_LP, t, pl, pts, n, rows, plane, grad, mixed, tau, st); break;
