@@ -147,9 +147,10 @@ __device__ __forceinline__ void fe_dg_block(const double* __restrict__ rec, cons
 }
 
 // 1-D Hermite coefficients E^{ab}_t, t = 0..L (zero above a + b), of
-// (u+xa)^a (u+xb)^b e^{-p u^2} for a, b <= 3, a + b <= L, by the recurrence of
-// fe_e1d_table raising a, then b; h = 1/(2p).  Compile-time bounds and
-// predicated steps keep e in registers.
+// (u+xa)^a (u+xb)^b e^{-p u^2} for a, b <= 3, a + b <= L, by the recurrence
+// E^{i+1,j}_t = E^{ij}_{t-1} h + xa E^{ij}_t + (t+1) E^{ij}_{t+1} (fastesp/ints.py:62-81)
+// raising a, then b; h = 1/(2p).  Compile-time bounds and predicated steps
+// keep e in registers.
 template <int L>
 __device__ __forceinline__ void fe_e1d_ab(const int a, const int b, const double xa,
                                           const double xb, const double h, double* e) {
