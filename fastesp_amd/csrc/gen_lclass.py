@@ -14,7 +14,13 @@ independently (own implementation of the a-table recurrence of
 * ``fe_contract_grad<L>``: the same plus the gradient with respect to C;
 * ``fe_dg_accum<L>``: the per-term products w_m R^alpha for dL/dG (and
   ``fe_dg_scale<L>``, the per-order factors applied once per pair);
-* ``fe_contract_f32<L>``: the mixed-precision form (FP32 monomials/products).
+* ``fe_contract_f32<L>``: the mixed-precision form (FP32 monomials/products);
+* ``fe_fill_kelvin<L>``, ``fe_far_grad<L>``, ``fe_far_value<L>``: the far-field
+  (Kelvin) block of a record and its evaluation;
+* ``fe_det_from_dg<L>``: dL/dG -> d etab, the adjoint of the table contraction;
+* ``fe_g_from_etab<L>``, ``fe_etab_bound<L>`` and, per pair of shell l's,
+  ``fe_pair_etab<LI, LJ>``: the table build's etab, G and screening bound of
+  one primitive pair.
 
 Run: ``python gen_lclass.py [LMAX]`` (default 6).  Output is deterministic.
 """
